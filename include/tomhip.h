@@ -1,0 +1,135 @@
+/*
+ * tomhip.h — C ABI of libtomhip.so, the gfx950 (MI355X) implementation of the
+ * TOM Toolbox template-matching / subtomogram-scoring hot path.
+ *
+ * Conventions (those of the reference MEX gateways):
+ *   - volumes are float32, x fastest (MATLAB column-major), dims (nx, ny, nz);
+ *   - every data pointer may be host or device memory (detected per call);
+ *     host pointers are copied in/out around the call (PCIe-inclusive);
+ *   - angle indices are 0-based here (MATLAB gateways add 1);
+ *   - Euler angles are [phi psi theta] in degrees, ZXZ, tom_rotate order;
+ *   - every call returns 0 on success or a negative TOM-style error code
+ *     (-1 wrong input, -2 no memory, -3 unsupported, -7 HIP error,
+ *     -8 internal) and leaves a message in tomhip_last_error(ctx);
+ *     no C++ exception crosses the ABI.
+ */
+#ifndef TOMHIP_H
+#define TOMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tomhip_ctx tomhip_ctx;
+typedef struct tomhip_search tomhip_search;
+
+/* search modes */
+#define TOMHIP_MODE_FLCF 0 /* tom_os3_findTemplate / tom_os3_corr normalised score */
+#define TOMHIP_MODE_XCF 1  /* tom_oscar unnormalised cross-correlation           */
+
+/* ---------------------------------------------------------------- context */
+/* One context per device and stream. No reference counterpart (the MEX
+ * gateways are stateless); the context owns twiddle tables and workspaces. */
+int tomhip_init(int device, tomhip_ctx** ctx);
+void tomhip_destroy(tomhip_ctx* ctx);
+const char* tomhip_last_error(tomhip_ctx* ctx);
+/* Launch on a caller-owned hipStream_t (NULL = the context's own stream). */
+int tomhip_set_stream(tomhip_ctx* ctx, void* hip_stream);
+int tomhip_synchronize(tomhip_ctx* ctx);
+
+/* ---------------------------------------------------------------- rotation */
+/* tom_rotatec(in, out, euler, 'linear', center): replaces the MEX core of
+ * Sptrans/tom_rotatec.c:146-401 (gateway :407-480, called from
+ * Sptrans/tom_rotate.m:123-128). neuler = 3 ([phi psi theta] deg) or 9
+ * (matrix entries rm00 rm10 rm20 rm01 rm11 rm21 rm02 rm12 rm22, i.e. a 3x3
+ * MATLAB matrix in column order). center = 0-based pivot (3 floats) or NULL
+ * for ceil(n/2). Bit-identical to the reference. */
+int tomhip_rotate3d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, int64_t nz,
+                        const float* euler, int neuler, const float* center);
+
+/* Batch form: count rotations of one volume, dst[r] at dst + r*nx*ny*nz,
+ * euler = [count][3] degrees; optional mask multiplies every output. No
+ * reference counterpart (the reference loops tom_rotate per angle,
+ * tom_os3_findTemplate.m:264). */
+int tomhip_rotate3d_batch_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, int64_t nz,
+                              const float* euler, int64_t count, const float* center, const float* mask);
+
+/* ---------------------------------------------------------------- peaks */
+/* tom_peakinc(in, dims, max, pos): global maximum, first position in x-fastest
+ * order, strict '>' from -99999 (Analysis/tom_peakinc.c:23-77, gateway :60).
+ * pos is 0-based (the reference MEX adds 1). */
+int tomhip_peak_f32(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val, int32_t pos[3]);
+
+/* ---------------------------------------------------------------- masks, weights */
+/* tom_os3_sphereMask(template) (Oscar3/matlabPicker/search/tom_os3_sphereMask.m:4-36):
+ * binary r < floor(n/2)-1 for cubic boxes, ones otherwise; *count = #{M>0}. */
+int tomhip_spheremask_os3_f32(tomhip_ctx* ctx, const int64_t n[3], float* out, int64_t* count);
+/* tom::init_spheremask (src/tom/core/volume_fcn.cpp:1013-1030): 1 inside
+ * radius, Gaussian edge exp(-((r-radius)/sigma)^2) up to max_radius (default
+ * radius+sqrt(2 sigma) when max_radius < radius); sigma <= 0: binary r < radius. */
+int tomhip_spheremask_f32(tomhip_ctx* ctx, const int64_t n[3], float radius, float sigma, float max_radius,
+                          float* out);
+/* tom_wedge(ones(n), semi_angle) .* tom_bandpass annulus lo <= |k| <= hi on the
+ * centred (fftshift) grid (Filtrans/tom_wedge.m:62-77, tom_bandpass.m:43-52).
+ * semi_angle_deg <= 0 disables the wedge, hi <= 0 disables the bandpass. */
+int tomhip_weight_f32(tomhip_ctx* ctx, const int64_t n[3], double semi_angle_deg, double lo, double hi,
+                      float* out);
+
+/* ---------------------------------------------------------------- template prep */
+/* tom_norm(1e5*(T+1),'mean0+1std') then tom_os3_modifyImage (box-size W,
+ * FP64) and tom_os3_normUnderMask (tom_os3_findTemplate.m:157, 233-234).
+ * W: centred weight of the template's size or NULL. */
+int tomhip_prepare_template_f32(tomhip_ctx* ctx, const float* tmpl, const int64_t n[3], const float* mask,
+                                const float* W, float* tmpl_out, double* mu_t, double* sig_t);
+
+/* ---------------------------------------------------------------- search plan */
+/* Exhaustive search state (tom_os3_findTemplate job / tom_oscar loop):
+ * F_I and, in FLCF mode, the FP64 local mean/std of the volume under the mask
+ * (tom_os3_mean.m:44-63, tom_os3_std.m:43-68) are computed once here.
+ * batch = angles per launch group (1..16). */
+int tomhip_search_create(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* mask,
+                         const int64_t n[3], int mode, int batch, tomhip_search** out);
+/* tmpl: prepared template (tomhip_prepare_template_f32 output in FLCF mode,
+ * raw template in XCF mode); W: centred box weight or NULL; remask = 1 masks
+ * the rotated (filtered) template again as tom_os3_corr.m:151 does inside the
+ * findTemplate loop, 0 masks it once (tom_os3_corr called directly). Resets
+ * the running maxima. */
+int tomhip_search_set_template(tomhip_search* s, const float* tmpl, const float* W, double mu_t, double sig_t,
+                               int remask);
+/* Scan count angles (host [count][3] degrees) whose indices are
+ * first_index .. first_index+count-1. May be called repeatedly. */
+int tomhip_search_run(tomhip_search* s, const float* euler, int64_t count, int64_t first_index);
+/* FLCF: score = (c/m - mu_I mu_T)/(sig_T sig_I) of the best angle (0 where
+ * sig_I = 0); XCF: tom_oscar's ccfmax in its orientation. angle: 0-based. */
+int tomhip_search_finalize(tomhip_search* s, float* score, int32_t* angle);
+/* Per-kernel timing (HIP events; serialises launches while enabled).
+ * ms/counts: 8 entries (rotate, box weight, template fwd, pass A, pass B,
+ * pass C, -, -). */
+int tomhip_search_profile(tomhip_search* s, int enable, double* ms, int64_t* counts);
+void tomhip_search_destroy(tomhip_search* s);
+
+/* ---------------------------------------------------------------- one-shot drivers */
+/* ccc-only tom_os3_findTemplate (Oscar3/matlabPicker/search/tom_os3_findTemplate.m:157-307
+ * with tom_os3_bestPeak.m ccc2d): best score and 0-based best angle per voxel.
+ * mask: template-sized or NULL (tom_os3_sphereMask); W: centred template-sized
+ * weight or NULL; angles [A][3] degrees; batch 0 = default. */
+int tomhip_flcf_search_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
+                           const int64_t n[3], const float* mask, const float* W, const float* angles, int64_t A,
+                           int batch, float* score, int32_t* angle);
+/* tom_os3_corr(img, template, options) FLCF branch
+ * (Oscar3/matlabPicker/search/tom_os3_corr.m:77-192): mask NULL uses
+ * tom_os3_sphereMask(template). */
+int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], const float* tmpl,
+                        const int64_t n[3], const float* mask, float* out);
+/* tom_oscar(vol, templ, ...) (Filtrans/tom_oscar.m:1-82) over an explicit angle
+ * list: ccfmax and the 0-based index of the best angle (-1 never set). */
+int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
+                     const int64_t n[3], const float* angles, int64_t A, float* ccfmax, int32_t* angle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMHIP_H */

@@ -1,0 +1,394 @@
+"""ORACLE — CPU restatement of the TOM template-matching path (test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import this module, and only as the checker: the product package
+``tom_amd`` never imports it.
+
+Everything here is FP64 numpy except the rotation, which calls the float-exact C
+restatement in ``oracle/tomoracle.c`` (``liboracle.so``). Volumes use MATLAB
+indexing: a numpy array ``a`` of shape ``(nx, ny, nz)`` with ``a[x, y, z]`` equal
+to MATLAB's ``a(x+1, y+1, z+1)``; the C side sees them Fortran-ordered
+(x fastest).
+
+Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
+  angle_list            Oscar3/matlabPicker/search/tom_os3_findTemplate.m:255 loop
+                        order over tools/tom_os3_angleList.m:62-75 (no dedup)
+                        -> explicit enumeration count/order
+  rotate                Sptrans/tom_rotate.m:89-128 + Sptrans/tom_rotatec.c:146-401
+                        -> bit-exact vs the reference's rot3dlin (oracle/_ref) and
+                           golden vectors; axis invariants tom_rotate.m:59-61
+  sphere_mask_os3       search/tom_os3_sphereMask.m:4-61 -> closed form
+  sphere_mask_gauss     src/tom/core/volume_fcn.cpp:932-953, 1013-1030 -> closed form
+  wedge                 Filtrans/tom_wedge.m:62-77 -> grid predicate, kz=0 kept
+  bandpass_weight       Filtrans/tom_bandpass.m:43-52 -> grid predicate
+  apply_weight          Filtrans/tom_apply_weight_function.m:56-62 -> W=1 identity,
+                        Parseval, linearity
+  tom_norm_mean0_1std   Sptrans/tom_norm.m:83-119 + Analysis/tom_dev.m:55-60
+                        -> mean 0 / population std 1
+  norm_under_mask       statistics/tom_os3_normUnderMask.m:1-8 -> direct sums
+  local_stats           statistics/tom_os3_mean.m:44-63, tom_os3_std.m:43-68
+                        -> brute-force direct sums (orc_flcf_direct)
+  flcf                  search/tom_os3_corr.m:77-192 (FLCF branch)
+                        -> orc_flcf_direct; planted copy = 1; affine invariance
+  find_template         search/tom_os3_findTemplate.m:157-307 (ccc-only) +
+                        search/tom_os3_bestPeak.m:13-47,94-98
+                        -> brute force over the angle list
+  oscar                 Filtrans/tom_oscar.m:55-80 -> brute force
+  peak / intpeak        Analysis/tom_peak.m:80-108, 437-525 -> the 5x5 KAT of
+                        tom_peak.m:27-50 (value and spline position)
+  find_max              Analysis/tom_peakinc.c:23-52 -> numpy argmax
+  fourier               Filtrans/tom_fourier.m:42-50 -> the 4x4 KAT tom_fourier.m:15-22
+
+Deliberate deviations shared with the product (documented, not bugs):
+  * Local variance is formed from the globally mean-shifted volume and shifted
+    back with the exact correction terms (same mathematics, better FP64
+    conditioning), and a local variance at or below 1e-12 x the global
+    variance counts as zero (score 0), so constant regions give 0 instead of
+    the reference's division by rounding noise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+VAR_FLOOR_REL = 1e-12
+
+
+def lib():
+    """Load (building if needed) the C part of the oracle."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lp = ctypes.POINTER(ctypes.c_long)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_rotmat.argtypes = [fp, ctypes.c_int, fp]
+        L.orc_rot3d.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, ctypes.c_long, fp,
+                                ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.orc_find_max.argtypes = [fp, ip, fp, ip]
+        L.orc_flcf_direct.argtypes = [fp, lp, dp, dp, lp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, dp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+def build() -> None:
+    import subprocess
+    src = os.path.join(_HERE, "tomoracle.c")
+    out = os.path.join(_HERE, "liboracle.so")
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
+                           "-o", out, src, "-lm"])
+
+
+def _fptr(a, t=ctypes.c_float):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+# ----------------------------------------------------------------------------
+# angles
+# ----------------------------------------------------------------------------
+def angle_list(phi: Sequence[float], psi: Sequence[float], theta: Sequence[float]) -> np.ndarray:
+    """[phi psi theta] rows, phi outermost, theta innermost (raw product, no dedup)."""
+    out = [(a, b, c) for a in phi for b in psi for c in theta]
+    return np.asarray(out, dtype=np.float64).reshape(-1, 3)
+
+
+def angle_grid(step_phi: float, step_psi: float, step_theta: float) -> np.ndarray:
+    """phi, psi in [0, 360) and theta in [0, 180] inclusive at the given steps."""
+    phi = np.arange(0, 360, step_phi, dtype=np.float64)
+    psi = np.arange(0, 360, step_psi, dtype=np.float64)
+    the = np.arange(0, 180 + 1e-9, step_theta, dtype=np.float64)
+    return angle_list(phi, psi, the)
+
+
+# ----------------------------------------------------------------------------
+# rotation (float-exact, C)
+# ----------------------------------------------------------------------------
+def rotmat(euler) -> np.ndarray:
+    e = np.ascontiguousarray(np.asarray(euler, dtype=np.float32).ravel())
+    rm = np.zeros(9, np.float32)
+    lib().orc_rotmat(_fptr(e), int(e.size), _fptr(rm))
+    return rm
+
+
+def rotate(vol: np.ndarray, euler, center=None) -> np.ndarray:
+    """tom_rotate(vol, euler, 'linear', center) with the default centre
+    ceil(size/2) used as a 0-based coordinate (Sptrans/tom_rotate.m:105-108)."""
+    v = np.asfortranarray(vol, dtype=np.float32)
+    sx, sy, sz = v.shape
+    if center is None:
+        center = [int(np.ceil(s / 2)) for s in v.shape]
+    e = np.ascontiguousarray(np.asarray(euler, dtype=np.float32).ravel())
+    out = np.zeros_like(v, order="F")
+    lib().orc_rot3d(_fptr(v), _fptr(out), sx, sy, sz, _fptr(e), int(e.size),
+                    float(center[0]), float(center[1]), float(center[2]))
+    return out
+
+
+# ----------------------------------------------------------------------------
+# masks and weights
+# ----------------------------------------------------------------------------
+def _centred_grid(shape):
+    ax = [np.arange(-(s // 2), -(s // 2) + s, dtype=np.float64) for s in shape]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+def sphere_mask_os3(n: int) -> Tuple[np.ndarray, int]:
+    """tom_os3_sphereMask for a cubic template: binary r < floor(n/2)-1 around
+    floor(n/2) (blurring forced to 0, tom_os3_sphereMask.m:22)."""
+    x, y, z = _centred_grid((n, n, n))
+    r = np.sqrt(x * x + y * y + z * z)
+    radius = n // 2 - 1
+    m = (r < radius).astype(np.float32)
+    return m, int((m > 0).sum())
+
+
+def sphere_mask_binary(n: int, radius: float) -> np.ndarray:
+    x, y, z = _centred_grid((n, n, n))
+    r = np.sqrt(x * x + y * y + z * z)
+    return (r < radius).astype(np.float32)
+
+
+def sphere_mask_gauss(n: int, radius: float, sigma: float, max_radius: float = -1.0) -> np.ndarray:
+    """tom::init_spheremask with sigma > 0 (float arithmetic as in the C++):
+    1 inside radius, exp(-((r-radius)/sigma)^2) up to max_radius
+    (default radius + sqrt(2 sigma)), 0 beyond; centre floor(n/2)."""
+    f = np.float32
+    if max_radius < radius:
+        max_radius = radius + np.sqrt(2.0 * sigma)
+    c = f(n // 2)
+    ax = np.arange(n, dtype=np.float32) - c
+    x, y, z = np.meshgrid(ax, ax, ax, indexing="ij")
+    r = np.sqrt((x * x + (y * y + z * z)).astype(np.float32)).astype(np.float32)
+    t = ((r - f(radius)) / f(sigma)).astype(np.float32)
+    g = np.exp(-(t * t)).astype(np.float32)
+    m = np.where(r <= f(radius), f(1), np.where(r > f(max_radius), f(0), g))
+    return m.astype(np.float32)
+
+
+def wedge(shape, semi_angle_deg: float) -> np.ndarray:
+    """tom_wedge(image, angle): 0 where tan(angle) > |x|/|z| on the centred grid;
+    the kz = 0 plane is always kept (|x|/0 is inf or NaN)."""
+    x, _, z = _centred_grid(shape)
+    a = semi_angle_deg * np.pi / 180.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = np.abs(x) / np.abs(z)
+        miss = np.tan(a) > q
+    w = np.ones(shape, np.float64)
+    w[miss] = 0
+    return w
+
+
+def bandpass_weight(shape, low: float, hi: float) -> np.ndarray:
+    """tom_bandpass fast version: keep low <= |k| <= hi on the centred grid."""
+    x, y, z = _centred_grid(shape)
+    ln = np.sqrt(x * x + y * y + z * z)
+    return ((ln <= hi) & (ln >= low)).astype(np.float64)
+
+
+def apply_weight(t: np.ndarray, W: np.ndarray) -> np.ndarray:
+    """tom_apply_weight_function(in, W): real(ifftn(ifftshift(fftshift(fftn(in)).*W)))."""
+    F = np.fft.fftshift(np.fft.fftn(np.asarray(t, np.float64)))
+    return np.real(np.fft.ifftn(np.fft.ifftshift(F * W)))
+
+
+def fourier(a: np.ndarray) -> np.ndarray:
+    """tom_fourier: unscaled forward DFT with MATLAB's sign."""
+    return np.fft.fftn(a)
+
+
+# ----------------------------------------------------------------------------
+# normalisation and statistics
+# ----------------------------------------------------------------------------
+def tom_norm_mean0_1std(T: np.ndarray) -> np.ndarray:
+    """Template preparation of tom_os3_findTemplate.m:157:
+    tom_norm(1e5*(T+1), 'mean0+1std') with tom_dev's population std."""
+    a = 1e5 * (np.asarray(T, np.float64) + 1.0)
+    a = a * 2 + 100
+    mea = a.mean()
+    st = a.std()
+    return (a - mea) / st if st != 0 else (a - mea)
+
+
+def norm_under_mask(vol: np.ndarray, mask: np.ndarray) -> Tuple[float, float]:
+    """tom_os3_normUnderMask: mean over mask~=0 (and vol~=0), population std."""
+    mk = mask != 0
+    n = mk.sum()
+    v = np.asarray(vol, np.float64)
+    mea = (v * mk * (v != 0)).sum() / n
+    st = np.sqrt((((~mk) * mea + v * mk - mea) ** 2).sum() / n)
+    return float(mea), float(st)
+
+
+def paste_center(shape, small: np.ndarray) -> np.ndarray:
+    """tom_os3_pasteCenter into zeros(shape): start floor(N/2)-floor(n/2)."""
+    out = np.zeros(shape, np.float64)
+    st = [s // 2 - t // 2 for s, t in zip(shape, small.shape)]
+    out[st[0]:st[0] + small.shape[0], st[1]:st[1] + small.shape[1],
+        st[2]:st[2] + small.shape[2]] = small
+    return out
+
+
+def local_stats(V: np.ndarray, M: np.ndarray, m: float) -> Tuple[np.ndarray, np.ndarray]:
+    """mu_I, sigma_I of tom_os3_mean / tom_os3_std (FFT convolution with the
+    pasted soft mask, ifftshift, divide by the count m), FP64.
+
+    Evaluated on the globally mean-shifted volume V - g for conditioning and
+    shifted back exactly: with s = sum(M)/m, mu = mu_s + g s and
+    E2 - mu^2 = (e2_s - mu_s^2) + (1 - s)(2 g mu_s + g^2 s), which is the
+    reference's E[x^2] - E[x]^2 (the (1 - s) term vanishes for binary masks).
+    A variance at or below VAR_FLOOR_REL x the global variance counts as 0."""
+    V = np.asarray(V, np.float64)
+    M = np.asarray(M, np.float64)
+    g = V.mean()
+    Vs = V - g
+    s = float(M.sum()) / m
+    FM = np.fft.fftn(paste_center(V.shape, M))
+    mu_s = np.real(np.fft.ifftshift(np.fft.ifftn(np.fft.fftn(Vs) * FM))) / m
+    e2 = np.real(np.fft.ifftshift(np.fft.ifftn(np.fft.fftn(Vs * Vs) * FM))) / m
+    var = (e2 - mu_s * mu_s) + (1.0 - s) * (2.0 * g * mu_s + g * g * s)
+    floor = VAR_FLOOR_REL * float((Vs * Vs).mean())
+    var = np.where(var > floor, var, 0.0)
+    return mu_s + g * s, np.sqrt(var)
+
+
+def correlation_raw(V: np.ndarray, tm: np.ndarray, FV: Optional[np.ndarray] = None) -> np.ndarray:
+    """real(ifftshift(ifftn(fftn(V) .* conj(fftn(pasteCenter(zeros, tm))))))."""
+    if FV is None:
+        FV = np.fft.fftn(np.asarray(V, np.float64))
+    FT = np.fft.fftn(paste_center(V.shape, np.asarray(tm, np.float64)))
+    return np.real(np.fft.ifftshift(np.fft.ifftn(FV * np.conj(FT))))
+
+
+def flcf(V, tm, m, mu_T, sig_T, mu_I, sig_I, FV=None) -> np.ndarray:
+    """Score map of the FLCF branch of tom_os3_corr for a masked template tm."""
+    c = correlation_raw(V, tm, FV) / m
+    den = sig_T * sig_I
+    out = np.zeros(V.shape, np.float64)
+    ok = den > 0
+    out[ok] = (c[ok] - mu_I[ok] * mu_T) / den[ok]
+    return out
+
+
+def prepare_template(T, M, W=None):
+    """Template normalisation and the fixed template statistics of
+    tom_os3_findTemplate.m:157, 233-234."""
+    Tn = tom_norm_mean0_1std(T)
+    tmp = apply_weight(Tn, W) if W is not None else Tn
+    mu_T, sig_T = norm_under_mask(tmp, M)
+    return Tn, mu_T, sig_T
+
+
+def find_template(V, T, M, angles, W=None, return_all=False):
+    """ccc-only tom_os3_findTemplate: returns (best score, 1-based best angle).
+
+    Per angle: t = rotate(Tn, a) .* M; t = apply_weight(t, W); score =
+    FLCF(V, t .* M); update where old < new (strict, first angle wins);
+    maps start at -1000."""
+    V = np.asarray(V, np.float64)
+    M = np.asarray(M, np.float64)
+    m = float((M > 0).sum())
+    Tn, mu_T, sig_T = prepare_template(T, M, W)
+    mu_I, sig_I = local_stats(V, M, m)
+    FV = np.fft.fftn(V)
+    ccc = np.full(V.shape, -1000.0)
+    ang = np.full(V.shape, -1000.0)
+    allmaps = []
+    for i, a in enumerate(np.asarray(angles).reshape(-1, 3)):
+        t = rotate(Tn.astype(np.float32), a).astype(np.float64) * M
+        if W is not None:
+            t = apply_weight(t, W)
+        s = flcf(V, t * M, m, mu_T, sig_T, mu_I, sig_I, FV)
+        if return_all:
+            allmaps.append(s)
+        upd = ccc < s
+        ccc[upd] = s[upd]
+        ang[upd] = i + 1
+    if return_all:
+        return ccc, ang, np.stack(allmaps)
+    return ccc, ang
+
+
+def oscar(V, T, angles):
+    """tom_oscar: unnormalised XCF, template pasted at N/2+1-n/2 (1-based),
+    real(ifftshift(ifftn(fftn(temp).*conj(fftn(vol))))), strict '>' update.
+    Returns (ccfmax, 0-based angle index map, -1 where never updated)."""
+    V = np.asarray(V, np.float64)
+    FVc = np.conj(np.fft.fftn(V))
+    ccf = np.full(V.shape, -1000.0)
+    idx = np.full(V.shape, -1, np.int64)
+    for i, a in enumerate(np.asarray(angles).reshape(-1, 3)):
+        t = rotate(np.asarray(T, np.float32), a).astype(np.float64)
+        temp = paste_center(V.shape, t)
+        c = np.real(np.fft.ifftshift(np.fft.ifftn(np.fft.fftn(temp) * FVc)))
+        upd = c > ccf
+        ccf[upd] = c[upd]
+        idx[upd] = i
+    return ccf, idx
+
+
+# ----------------------------------------------------------------------------
+# peaks
+# ----------------------------------------------------------------------------
+def find_max(a: np.ndarray):
+    """tom_peakinc find_max: (value, 0-based [x y z]) of the first maximum in
+    x-fastest order, start value -99999, strict '>'."""
+    v = np.asfortranarray(a, dtype=np.float32)
+    if v.ndim < 3:
+        v = v.reshape(v.shape + (1,) * (3 - v.ndim), order="F")
+    dims = np.asarray(v.shape, np.int32)
+    mv = np.zeros(1, np.float32)
+    pos = np.zeros(3, np.int32)
+    lib().orc_find_max(_fptr(v), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       _fptr(mv), pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return float(mv[0]), pos.copy()
+
+
+def _parabola_peak(p3):
+    """interp1(1:3, p3, 1:0.001:3, 'spline') with 3 points is the parabola
+    through them; max over the 0.001 grid (first on ties)."""
+    xs = np.round(np.arange(1.0, 3.0 + 1e-9, 0.001), 3)
+    a = (p3[0] - 2 * p3[1] + p3[2]) / 2.0
+    b = (p3[2] - p3[0]) / 2.0
+    y = p3[1] + b * (xs - 2) + a * (xs - 2) ** 2
+    i = int(np.argmax(y))
+    return float(y[i]), i
+
+
+def peak(a: np.ndarray, spline: bool = False):
+    """tom_peak(A) / tom_peak(A,'spline') for 2D/3D arrays (1-based output)."""
+    a = np.asarray(a, np.float64)
+    v = a.max()
+    first = np.argwhere(np.asfortranarray(a) == v)
+    # MATLAB find returns column-major order: first hit in x-fastest order
+    order = np.lexsort(tuple(first[:, d] for d in range(first.shape[1])))
+    s = first[order[0]] + 1
+    if not spline:
+        return s.astype(np.float64), float(v)
+    c = s.astype(np.float64)
+    vals = []
+    for d in range(a.ndim):
+        if s[d] == 1 or s[d] == a.shape[d]:
+            c[d] = s[d]
+            vals.append(v)
+            continue
+        idx = [int(x) - 1 for x in s]
+        p3 = []
+        for off in (-1, 0, 1):
+            j = list(idx)
+            j[d] += off
+            p3.append(a[tuple(j)])
+        val, pos = _parabola_peak(p3)
+        c[d] = s[d] - 1 + pos * 0.001
+        vals.append(val)
+    return c, float(max(vals))

@@ -1,0 +1,295 @@
+"""Pins the CPU oracle (oracle/) against the reference's own material:
+golden vectors from the reference C rotation core, the KATs in the
+reference's docstrings, closed forms, brute force and invariants.
+No GPU needed."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "rot3d_golden.npz")
+
+
+def _golden():
+    g = np.load(GOLD)
+    vols = {24: g["vol24"], 25: g["vol25"], 48: g["vol48"], 2030: g["vol2030"]}
+    return g, vols
+
+
+# ---------------------------------------------------------------- rotation
+def test_rotation_golden_bit_exact(oracle):
+    """rot3d restatement == reference rot3dlin on every golden case
+    (Sptrans/tom_rotatec.c:146-401)."""
+    g, vols = _golden()
+    for i in range(len(g["case_sha256"])):
+        v = vols[int(g["case_vol"][i])]
+        ne = int(g["case_neuler"][i])
+        o = oracle.rotate(v, g["case_euler"][i][:ne], g["case_center"][i])
+        assert hashlib.sha256(o.tobytes(order="F")).hexdigest() == g["case_sha256"][i], i
+    for k, i in enumerate(g["full_idx"]):
+        v = vols[int(g["case_vol"][i])]
+        o = oracle.rotate(v, g["case_euler"][i][:3], g["case_center"][i])
+        np.testing.assert_array_equal(o, g["full_out"][k])
+
+
+def test_rotation_against_built_reference(oracle):
+    """Extra random angles against oracle/_ref when it is built here."""
+    so = os.path.join(ROOT, "oracle", "_ref", "libtomref.so")
+    if not os.path.exists(so):
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    import ctypes
+    L = ctypes.CDLL(so)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.rot3dlin.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, ctypes.c_long, fp, ctypes.c_float,
+                           ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    rng = np.random.default_rng(7)
+    v = np.asfortranarray(rng.standard_normal((24, 20, 28)).astype(np.float32))
+    for _ in range(40):
+        e = rng.uniform(-400, 400, 3).astype(np.float32)
+        out = np.zeros_like(v, order="F")
+        L.rot3dlin(v.ctypes.data_as(fp), out.ctypes.data_as(fp), 24, 20, 28, e.ctypes.data_as(fp), 12, 10, 14, 1)
+        np.testing.assert_array_equal(oracle.rotate(v, e, [12, 10, 14]), out)
+
+
+def test_rotation_axis_invariants(oracle):
+    """Sptrans/tom_rotate.m:59-61: X-axis [0 0 a], Y-axis [270 90 a], Z-axis [a 0 0];
+    90-degree turns permute voxels exactly (exact cos/sin table)."""
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    # rotation about z by 90 degrees maps voxel (x,y) -> (c - (y-c), c + (x-c))
+    r = oracle.rotate(v, [90, 0, 0])
+    c = 8
+    for x in range(1, 16):
+        for y in range(1, 16):
+            # inverse mapping: out(i,j) = in(c + rm00*(i-c) + rm10*(j-c), ...)
+            rm = oracle.rotmat([90, 0, 0])
+            sx = c + rm[0] * (x - c) + rm[1] * (y - c)
+            sy = c + rm[3] * (x - c) + rm[4] * (y - c)
+            if 0 <= sx <= 15 and 0 <= sy <= 15:
+                assert r[x, y, 5] == v[int(sx), int(sy), 5]
+    # the three axis conventions are rotations about the named axis: the axis
+    # coordinate is preserved by the matrix
+    a = 37.0
+    rmx = oracle.rotmat([0, 0, a]).reshape(3, 3, order="F")  # rows i, cols j = rm_ij
+    rmy = oracle.rotmat([270, 90, a]).reshape(3, 3, order="F")
+    rmz = oracle.rotmat([a, 0, 0]).reshape(3, 3, order="F")
+    np.testing.assert_allclose(rmx @ [1, 0, 0], [1, 0, 0], atol=1e-6)
+    np.testing.assert_allclose(rmy @ [0, 1, 0], [0, 1, 0], atol=1e-6)
+    np.testing.assert_allclose(rmz @ [0, 0, 1], [0, 0, 1], atol=1e-6)
+
+
+def test_rotmat_formula(oracle):
+    """tom_rotatec.c:17-25 explicit matrix == Rz(psi) Rx(theta) Rz(phi)."""
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        phi, psi, the = rng.uniform(0, 360, 3)
+        rm = oracle.rotmat([phi, psi, the]).astype(np.float64)
+        f, p, t = np.deg2rad([np.float32(phi), np.float32(psi), np.float32(the)])
+
+        def rz(a):
+            return np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+
+        def rx(a):
+            return np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+
+        R = rz(p) @ rx(t) @ rz(f)
+        # rm_ij (i = line) stored rm00 rm10 rm20 rm01 ...: column order of R;
+        # the sampling is src = p + R^T (dst - p) (tom_rotatec.c:315-327)
+        M = rm.reshape(3, 3, order="F")  # M[i, j] = rm_ij
+        np.testing.assert_allclose(M, R, atol=2e-6)
+
+
+# ---------------------------------------------------------------- peaks
+def test_tom_peak_kat(oracle):
+    """Analysis/tom_peak.m:27-50 worked example."""
+    A = np.array([[1, 2, 3, 4, 17], [5, 6, 7, 8, 18], [10, 9, 40, 6, 20], [12, 16, 17, 20, 30],
+                  [5, 23, 35, 12, 6]], float)
+    c, v = oracle.peak(A)
+    assert list(c) == [3, 3] and v == 40
+    c, v = oracle.peak(A, spline=True)
+    np.testing.assert_allclose(c, [3.089, 2.977], atol=1e-9)
+    assert round(v, 4) == 40.2232
+    # the parabola vertex in closed form: 2 + 10/112 and 40 + 100/448
+    assert abs((2 + 10 / 112) - 2.0893) < 1e-4 and abs((40 + 100 / 448) - 40.2232) < 1e-4
+
+
+def test_find_max_first_in_x_fastest(oracle):
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 5, size=(7, 6, 5)).astype(np.float32)  # many ties
+    v, pos = oracle.find_max(a)
+    flat = np.asfortranarray(a).ravel(order="F")
+    i = int(np.argmax(flat))
+    assert v == flat[i]
+    assert list(pos) == [i % 7, (i // 7) % 6, i // 42]
+    v, pos = oracle.find_max(np.full((3, 3, 3), -1e6, np.float32))
+    assert v == -99999.0
+
+
+def test_fourier_kat(oracle):
+    """Filtrans/tom_fourier.m:18-22 (pins the forward sign and no scaling).
+
+    The printed matrix has two typos that contradict its own Hermitian
+    symmetry (a real input needs c(1,2) = conj(c(1,4)) and
+    c(2,2) = conj(c(4,4))): the docstring prints 3+19i and 50+9i there. Those
+    two entries are checked against the values their printed conjugate
+    partners imply (3-19i, 59+9i); the other 14 are checked as printed."""
+    a = np.array([[11, 3, -6, -14], [1, -13, 9, 6], [0, 8, 13, -3], [-8, 17, -15, 7]], float)
+    c = oracle.fourier(a)
+    printed = np.array([[16, 3 + 19j, -6, 3 + 19j], [-24 - 2j, 50 + 9j, 8 - 64j, 1 + 21j],
+                        [8, 5 - 37j, 54, 5 + 37j], [-24 + 2j, 1 - 21j, 8 + 64j, 59 - 9j]])
+    typo = np.zeros((4, 4), bool)
+    typo[0, 1] = typo[1, 1] = True
+    np.testing.assert_allclose(c[~typo], printed[~typo], atol=1e-9)
+    np.testing.assert_allclose(c[0, 1], np.conj(printed[0, 3]), atol=1e-9)
+    np.testing.assert_allclose(c[1, 1], np.conj(printed[3, 3]), atol=1e-9)
+
+
+# ---------------------------------------------------------------- masks/weights
+def test_masks_closed_form(oracle):
+    m, cnt = oracle.sphere_mask_os3(24)
+    x = np.arange(24) - 12
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    np.testing.assert_array_equal(m, (r < 11).astype(np.float32))
+    assert cnt == int((r < 11).sum())
+    g = oracle.sphere_mask_gauss(24, 10, 2)
+    assert g[12, 12, 12] == 1 and g[12, 12, 22] == 1  # r = 10
+    np.testing.assert_allclose(g[12, 12, 23], np.exp(-0.25), rtol=1e-6)  # r = 11
+    assert g[12, 12, 0] == 0 or r[12, 12, 0] <= 12  # r = 12 is the last ring kept
+    np.testing.assert_allclose(g[12, 12, 0], np.exp(-1.0), rtol=1e-6)
+    assert g[0, 0, 0] == 0
+
+
+def test_wedge_bandpass_predicates(oracle):
+    w = oracle.wedge((16, 16, 16), 30)
+    x = np.arange(16) - 8
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    assert np.all(w[:, :, 8] == 1)  # kz = 0 plane kept
+    miss = (Z != 0) & (np.tan(np.pi / 6) > np.abs(X) / np.maximum(np.abs(Z), 1e-300))
+    np.testing.assert_array_equal(w == 0, miss)
+    b = oracle.bandpass_weight((16, 16, 16), 2, 5)
+    L = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    np.testing.assert_array_equal(b == 1, (L >= 2) & (L <= 5))
+
+
+def test_apply_weight_identity_and_linearity(oracle):
+    rng = np.random.default_rng(2)
+    t = rng.standard_normal((12, 12, 12))
+    np.testing.assert_allclose(oracle.apply_weight(t, np.ones((12, 12, 12))), t, atol=1e-12)
+    W = oracle.bandpass_weight((12, 12, 12), 1, 4)
+    u = rng.standard_normal((12, 12, 12))
+    np.testing.assert_allclose(oracle.apply_weight(2 * t + u, W),
+                               2 * oracle.apply_weight(t, W) + oracle.apply_weight(u, W), atol=1e-10)
+    # projection: applying a binary weight twice changes nothing
+    a = oracle.apply_weight(t, W)
+    np.testing.assert_allclose(oracle.apply_weight(a, W), a, atol=1e-10)
+
+
+# ---------------------------------------------------------------- statistics / FLCF
+def _direct(oracle, V, tm, M, m, mu_T, sig_T):
+    import ctypes
+    L = oracle.lib()
+    V = np.asfortranarray(V, np.float32)
+    N = np.asarray(V.shape, np.int64)
+    n = np.asarray(M.shape, np.int64)
+    tmd = np.asfortranarray(tm, np.float64)
+    Md = np.asfortranarray(M, np.float64)
+    sc = np.zeros(V.shape, np.float64, order="F")
+    mu = np.zeros(V.shape, np.float64, order="F")
+    sg = np.zeros(V.shape, np.float64, order="F")
+    dp = ctypes.POINTER(ctypes.c_double)
+    lp = ctypes.POINTER(ctypes.c_long)
+    L.orc_flcf_direct(V.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), N.ctypes.data_as(lp),
+                      tmd.ctypes.data_as(dp), Md.ctypes.data_as(dp), n.ctypes.data_as(lp), m, mu_T, sig_T,
+                      sc.ctypes.data_as(dp), mu.ctypes.data_as(dp), sg.ctypes.data_as(dp))
+    return sc, mu, sg
+
+
+@pytest.mark.parametrize("N,n", [((12, 10, 8), (5, 5, 5)), ((11, 9, 13), (6, 4, 5)), ((8, 8, 8), (6, 6, 6))])
+def test_flcf_fft_matches_direct(oracle, N, n):
+    """FFT restatement (pasteCenter/ifftshift/convolution conventions of
+    tom_os3_corr/mean/std) == brute-force direct sums, even and odd sizes."""
+    rng = np.random.default_rng(sum(N) + sum(n))
+    V = rng.standard_normal(N).astype(np.float32) + 0.3
+    M = rng.uniform(0, 1, n) * (rng.uniform(0, 1, n) > 0.3)
+    tm = rng.standard_normal(n) * M
+    m = float((M > 0).sum())
+    mu_I, sig_I = oracle.local_stats(V, M, m)
+    s_fft = oracle.flcf(V, tm, m, 0.2, 1.3, mu_I, sig_I)
+    s_dir, mu_d, sg_d = _direct(oracle, V, tm, M, m, 0.2, 1.3)
+    np.testing.assert_allclose(mu_I, mu_d, atol=1e-12)
+    np.testing.assert_allclose(sig_I, sg_d, atol=1e-10)
+    np.testing.assert_allclose(s_fft, s_dir, atol=1e-10)
+
+
+def test_flcf_planted_copy_scores_one(oracle):
+    """A noise-free copy of the template inside a binary mask scores 1 at
+    its centre (Roseman's FLCF)."""
+    rng = np.random.default_rng(1)
+    n = 8
+    M = oracle.sphere_mask_binary(n, 3.5).astype(np.float64)
+    T = rng.standard_normal((n, n, n))
+    V = rng.standard_normal((24, 24, 24)) * 0.0 + 0.0
+    V = rng.standard_normal((24, 24, 24))
+    c = (10, 13, 9)
+    V[c[0] - 4:c[0] + 4, c[1] - 4:c[1] + 4, c[2] - 4:c[2] + 4] = T
+    m = float((M > 0).sum())
+    mu_T, sig_T = oracle.norm_under_mask(T, M)
+    mu_I, sig_I = oracle.local_stats(V, M, m)
+    s = oracle.flcf(V, T * M, m, mu_T, sig_T, mu_I, sig_I)
+    assert abs(s[c] - 1.0) < 1e-9
+    assert s.max() <= 1 + 1e-9
+
+
+def test_flcf_affine_invariance_and_zero_variance(oracle):
+    rng = np.random.default_rng(4)
+    V = rng.standard_normal((16, 16, 16))
+    M = oracle.sphere_mask_binary(6, 2.5).astype(np.float64)
+    T = rng.standard_normal((6, 6, 6))
+    m = float((M > 0).sum())
+    mu_T, sig_T = oracle.norm_under_mask(T, M)
+    tm = (T - mu_T) * M  # zero-mean under the mask: score is affine invariant in V
+    s1 = oracle.flcf(V, tm, m, 0.0, sig_T, *oracle.local_stats(V, M, m))
+    V2 = 3.5 * V - 7
+    s2 = oracle.flcf(V2, tm, m, 0.0, sig_T, *oracle.local_stats(V2, M, m))
+    np.testing.assert_allclose(s1, s2, atol=1e-9)
+    # constant volume: zero local variance -> score 0
+    Vc = np.full((16, 16, 16), 2.5)
+    s3 = oracle.flcf(Vc, tm, m, 0.0, sig_T, *oracle.local_stats(Vc, M, m))
+    assert np.all(s3 == 0)
+
+
+def test_find_template_bruteforce(oracle):
+    """The reduction: strict '<' update, first angle wins, 1-based index."""
+    rng = np.random.default_rng(9)
+    V = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    T = rng.standard_normal((6, 6, 6)).astype(np.float32)
+    M = oracle.sphere_mask_binary(6, 3).astype(np.float64)
+    angles = np.array([[0, 0, 0], [90, 0, 0], [0, 0, 0], [30, 45, 60]], float)  # duplicate angle -> tie
+    ccc, ang, allm = oracle.find_template(V, T, M, angles, return_all=True)
+    best = allm.max(axis=0)
+    np.testing.assert_array_equal(ccc, best)
+    first = np.argmax(allm == best[None], axis=0) + 1
+    np.testing.assert_array_equal(ang, first)
+    assert not np.any(ang == 3)  # the duplicate of angle 1 never wins
+
+
+def test_angle_grid_counts(oracle):
+    assert oracle.angle_grid(30, 30, 30).shape == (1008, 3)
+    assert oracle.angle_grid(15, 15, 15).shape == (7488, 3)
+    assert oracle.angle_grid(10, 10, 10).shape == (24624, 3)
+    g = oracle.angle_grid(30, 30, 30)
+    assert list(g[0]) == [0, 0, 0] and list(g[1]) == [0, 0, 30] and list(g[7]) == [0, 30, 0]
+
+
+def test_oscar_orientation(oracle):
+    """tom_oscar correlates the other way round: its map at k is the FLCF-style
+    raw correlation at -k."""
+    rng = np.random.default_rng(12)
+    V = rng.standard_normal((12, 12, 12))
+    T = rng.standard_normal((4, 4, 4)).astype(np.float32)
+    ccf, idx = oracle.oscar(V, T, np.zeros((1, 3)))
+    raw = oracle.correlation_raw(V, T.astype(np.float64))
+    neg = raw[np.ix_(*[(-np.arange(12)) % 12] * 3)]
+    np.testing.assert_allclose(ccf, neg, atol=1e-10)

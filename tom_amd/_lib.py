@@ -1,0 +1,70 @@
+"""ctypes binding of libtomhip.so (the C ABI declared in include/tomhip.h).
+
+The library is loaded from the package directory only; if it is missing the
+import fails loudly: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtomhip.so")
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every data pointer is passed as void*
+SIGNATURES = {
+    "tomhip_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+    "tomhip_destroy": (None, [vp]),
+    "tomhip_last_error": (ctypes.c_char_p, [vp]),
+    "tomhip_set_stream": (ctypes.c_int, [vp, vp]),
+    "tomhip_synchronize": (ctypes.c_int, [vp]),
+    "tomhip_rotate3d_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, vp,
+                                           ctypes.c_int, vp]),
+    "tomhip_rotate3d_batch_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, vp,
+                                                 ctypes.c_int64, vp, vp]),
+    "tomhip_peak_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "tomhip_spheremask_os3_f32": (ctypes.c_int, [vp, vp, vp, vp]),
+    "tomhip_spheremask_f32": (ctypes.c_int, [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
+    "tomhip_weight_f32": (ctypes.c_int, [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]),
+    "tomhip_prepare_template_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tomhip_search_create": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+    "tomhip_search_set_template": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "tomhip_search_run": (ctypes.c_int, [vp, vp, ctypes.c_int64, ctypes.c_int64]),
+    "tomhip_search_finalize": (ctypes.c_int, [vp, vp, vp]),
+    "tomhip_search_profile": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
+    "tomhip_search_destroy": (None, [vp]),
+    "tomhip_flcf_search_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp,
+                                              vp]),
+    "tomhip_os3_corr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "tomhip_oscar_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libtomhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tom_amd.build` "
+                              "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class TomHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tomhip error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,373 @@
+"""MATLAB-shaped host interface over libtomhip.so.
+
+Arrays follow MATLAB indexing: a numpy array of shape (nx, ny, nz) with
+``a[x, y, z]`` == MATLAB ``a(x+1, y+1, z+1)``; it is passed to the library
+x-fastest (Fortran order). A torch CUDA tensor is taken as already resident in
+device memory with shape (nz, ny, nx), C-contiguous (the same x-fastest bytes).
+Every compute call goes to the GPU library; there is no CPU path.
+
+Function                 reference counterpart
+tom_rotate               Sptrans/tom_rotate.m:89-128 -> tom_rotatec MEX
+tom_rotate_batch         loop of tom_rotate over an angle list
+tom_peakinc / tom_peak   Analysis/tom_peakinc.c:23-77, Analysis/tom_peak.m:80-108
+tom_os3_sphereMask       Oscar3/matlabPicker/search/tom_os3_sphereMask.m:4-36
+tom_spheremask_gauss     src/tom/core/volume_fcn.cpp:1013-1030 (init_spheremask)
+tom_wedge / weight       Filtrans/tom_wedge.m:62-77, Filtrans/tom_bandpass.m:43-52
+prepare_template         tom_os3_findTemplate.m:157, 233-234
+tom_os3_corr             Oscar3/matlabPicker/search/tom_os3_corr.m:77-192 (FLCF)
+tom_os3_findTemplate     search/tom_os3_findTemplate.m:157-307, ccc-only
+tom_oscar                Filtrans/tom_oscar.m:1-82
+SearchPlan               the per-job state of tom_os3_findTemplate (stats once,
+                         angle batches, reduction) exposed step by step
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import TomHipError, load
+
+try:  # torch is only plumbing for device memory and streams
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One libtomhip context (device + stream)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.tomhip_init(device, ctypes.byref(h))
+        if rc != 0:
+            raise TomHipError(rc, self.lib.tomhip_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def check(self, rc: int):
+        if rc != 0:
+            raise TomHipError(rc, self.lib.tomhip_last_error(self.h).decode())
+
+    def use_torch_stream(self):
+        if torch is not None and torch.cuda.is_available():
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            self.check(self.lib.tomhip_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def close(self):
+        if self.h:
+            self.lib.tomhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default: Optional[Context] = None
+
+
+def context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+# ---------------------------------------------------------------- buffers
+def _is_torch(a) -> bool:
+    return torch is not None and isinstance(a, torch.Tensor)
+
+
+class _Buf:
+    """Pointer + MATLAB dims of an input or output array."""
+
+    def __init__(self, a, dtype=np.float32):
+        if _is_torch(a):
+            tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+            if a.dtype != tdt or not a.is_contiguous():
+                a = a.to(tdt).contiguous()
+            self.obj = a
+            self.ptr = ctypes.c_void_p(a.data_ptr())
+            self.dims = tuple(reversed(a.shape))
+            self.torch = True
+        else:
+            a = np.asfortranarray(a, dtype=dtype)
+            self.obj = a
+            self.ptr = ctypes.c_void_p(a.ctypes.data)
+            self.dims = a.shape
+            self.torch = False
+
+
+def _dims3(dims) -> np.ndarray:
+    d = list(dims) + [1] * (3 - len(dims))
+    return np.asarray(d[:3], dtype=np.int64)
+
+
+def _p(arr: np.ndarray):
+    return ctypes.c_void_p(arr.ctypes.data)
+
+
+def _empty_like(buf: _Buf, dims, dtype=np.float32):
+    if buf.torch:
+        tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+        return torch.empty(tuple(reversed(tuple(dims))), dtype=tdt, device=buf.obj.device)
+    return np.zeros(tuple(dims), dtype=dtype, order="F")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.data_ptr()) if _is_torch(a) else ctypes.c_void_p(a.ctypes.data)
+
+
+def _prep_ctx(ctx, *bufs) -> Context:
+    ctx = ctx or context()
+    if any(b is not None and b.torch for b in bufs):
+        ctx.use_torch_stream()
+    return ctx
+
+
+# ---------------------------------------------------------------- rotation
+def tom_rotate(vol, euler, interp: str = "linear", center=None, ctx: Optional[Context] = None):
+    """out = tom_rotate(in, [phi psi theta] | 3x3 matrix, 'linear', [center]).
+
+    center is the 0-based pivot handed to tom_rotatec (default ceil(size/2))."""
+    if interp != "linear":
+        raise ValueError("only 'linear' interpolation exists (tom_rotatec.c:424-430)")
+    b = _Buf(vol)
+    ctx = _prep_ctx(ctx, b)
+    e = np.asarray(euler, dtype=np.float32)
+    neuler = 9 if e.size == 9 else 3
+    if e.size == 9 and e.ndim == 2:
+        e = np.asfortranarray(e)
+        e = e.ravel(order="F")
+    e = np.ascontiguousarray(e.ravel())
+    if e.size not in (3, 9):
+        raise ValueError("euler must hold 3 angles or 9 matrix entries")
+    d = _dims3(b.dims)
+    out = _empty_like(b, b.dims)
+    c = None if center is None else np.ascontiguousarray(np.asarray(center, np.float32).ravel()[:3])
+    ctx.check(ctx.lib.tomhip_rotate3d_f32(ctx.h, b.ptr, _ptr(out), int(d[0]), int(d[1]), int(d[2]), _p(e), neuler,
+                                          None if c is None else _p(c)))
+    return out
+
+
+def tom_rotate_batch(vol, angles, center=None, mask=None, ctx: Optional[Context] = None):
+    """Rotations of one volume by every [phi psi theta] row of angles; the
+    result has a trailing angle axis (nx, ny, nz, A) (torch: (A, nz, ny, nx))."""
+    b = _Buf(vol)
+    mb = _Buf(mask) if mask is not None else None
+    ctx = _prep_ctx(ctx, b, mb)
+    ang = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1, 3))
+    A = ang.shape[0]
+    d = _dims3(b.dims)
+    if b.torch:
+        out = torch.empty((A, int(d[2]), int(d[1]), int(d[0])), dtype=torch.float32, device=b.obj.device)
+    else:
+        out = np.zeros((int(d[0]), int(d[1]), int(d[2]), A), np.float32, order="F")
+    c = None if center is None else np.ascontiguousarray(np.asarray(center, np.float32).ravel()[:3])
+    ctx.check(ctx.lib.tomhip_rotate3d_batch_f32(ctx.h, b.ptr, _ptr(out), int(d[0]), int(d[1]), int(d[2]), _p(ang),
+                                                A, None if c is None else _p(c), None if mb is None else mb.ptr))
+    return out
+
+
+# ---------------------------------------------------------------- peaks
+def tom_peakinc(a, ctx: Optional[Context] = None) -> Tuple[float, np.ndarray]:
+    """(max value, 0-based [x y z]) of the first maximum in x-fastest order."""
+    b = _Buf(a)
+    ctx = _prep_ctx(ctx, b)
+    d = np.asarray(_dims3(b.dims), np.int32)
+    val = np.zeros(1, np.float32)
+    pos = np.zeros(3, np.int32)
+    ctx.check(ctx.lib.tomhip_peak_f32(ctx.h, b.ptr, _p(d), _p(val), _p(pos)))
+    return float(val[0]), pos
+
+
+def tom_peak(a, ctx: Optional[Context] = None):
+    """[c val] = tom_peak(A): 1-based position of the (first) maximum."""
+    v, p = tom_peakinc(a, ctx)
+    nd = np.asarray(a).ndim if not _is_torch(a) else a.dim()
+    return (p[:max(nd, 2)] + 1).astype(np.float64), v
+
+
+# ---------------------------------------------------------------- masks/weights
+def tom_os3_sphereMask(template_shape, ctx: Optional[Context] = None):
+    ctx = ctx or context()
+    d = _dims3(template_shape)
+    out = np.zeros(tuple(int(x) for x in d), np.float32, order="F")
+    cnt = np.zeros(1, np.int64)
+    ctx.check(ctx.lib.tomhip_spheremask_os3_f32(ctx.h, _p(d), _p(out), _p(cnt)))
+    return out, int(cnt[0])
+
+
+def tom_spheremask_gauss(shape, radius: float, sigma: float, max_radius: float = -1.0,
+                         ctx: Optional[Context] = None):
+    ctx = ctx or context()
+    d = _dims3(shape)
+    out = np.zeros(tuple(int(x) for x in d), np.float32, order="F")
+    ctx.check(ctx.lib.tomhip_spheremask_f32(ctx.h, _p(d), float(radius), float(sigma), float(max_radius),
+                                            _p(out)))
+    return out
+
+
+def weight(shape, semi_angle_deg: float = 0.0, low: float = 0.0, high: float = 0.0,
+           ctx: Optional[Context] = None):
+    """Centred wedge (tom_wedge semi-angle) times bandpass annulus."""
+    ctx = ctx or context()
+    d = _dims3(shape)
+    out = np.zeros(tuple(int(x) for x in d), np.float32, order="F")
+    ctx.check(ctx.lib.tomhip_weight_f32(ctx.h, _p(d), float(semi_angle_deg), float(low), float(high), _p(out)))
+    return out
+
+
+def tom_wedge(shape, semi_angle_deg: float, ctx: Optional[Context] = None):
+    return weight(shape, semi_angle_deg, 0.0, 0.0, ctx)
+
+
+# ---------------------------------------------------------------- template prep
+def prepare_template(template, mask, W=None, ctx: Optional[Context] = None):
+    """(Tn, mu_T, sig_T): tom_norm(1e5*(T+1),'mean0+1std') and the fixed
+    template statistics of the (W-filtered) normalised template."""
+    tb, mb = _Buf(template), _Buf(mask)
+    wb = _Buf(W) if W is not None else None
+    ctx = _prep_ctx(ctx, tb, mb, wb)
+    d = _dims3(tb.dims)
+    out = _empty_like(tb, tb.dims)
+    mu = np.zeros(1)
+    sg = np.zeros(1)
+    ctx.check(ctx.lib.tomhip_prepare_template_f32(ctx.h, tb.ptr, _p(d), mb.ptr, None if wb is None else wb.ptr,
+                                                  _ptr(out), _p(mu), _p(sg)))
+    return out, float(mu[0]), float(sg[0])
+
+
+# ---------------------------------------------------------------- search
+class SearchPlan:
+    """Exhaustive search over one volume (FLCF or tom_oscar XCF)."""
+
+    def __init__(self, volume, template_shape, mask=None, mode: str = "flcf", batch: int = 4,
+                 ctx: Optional[Context] = None):
+        vb = _Buf(volume)
+        mb = _Buf(mask) if mask is not None else None
+        self.ctx = _prep_ctx(ctx, vb, mb)
+        self.N = _dims3(vb.dims)
+        self.n = _dims3(template_shape)
+        self._torch = vb.torch
+        self._device = vb.obj.device if vb.torch else None
+        self.mode = {"flcf": 0, "xcf": 1}[mode]
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.tomhip_search_create(self.ctx.h, vb.ptr, _p(self.N),
+                                                         None if mb is None else mb.ptr, _p(self.n), self.mode,
+                                                         int(batch), ctypes.byref(h)))
+        self.h = h
+
+    def set_template(self, tmpl, W=None, mu_t: float = 0.0, sig_t: float = 1.0, remask: bool = True):
+        tb = _Buf(tmpl)
+        wb = _Buf(W) if W is not None else None
+        self.ctx.check(self.ctx.lib.tomhip_search_set_template(self.h, tb.ptr, None if wb is None else wb.ptr,
+                                                               float(mu_t), float(sig_t), int(remask)))
+
+    def run(self, angles, first_index: int = 0):
+        ang = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1, 3))
+        self.ctx.check(self.ctx.lib.tomhip_search_run(self.h, _p(ang), ang.shape[0], int(first_index)))
+
+    def finalize(self):
+        """(score, 0-based angle index) maps, numpy (nx,ny,nz) or torch (nz,ny,nx)."""
+        dims = tuple(int(x) for x in self.N)
+        if self._torch:
+            sc = torch.empty(tuple(reversed(dims)), dtype=torch.float32, device=self._device)
+            an = torch.empty(tuple(reversed(dims)), dtype=torch.int32, device=self._device)
+        else:
+            sc = np.zeros(dims, np.float32, order="F")
+            an = np.zeros(dims, np.int32, order="F")
+        self.ctx.check(self.ctx.lib.tomhip_search_finalize(self.h, _ptr(sc), _ptr(an)))
+        return sc, an
+
+    def profile(self, enable: bool):
+        """Enable/disable per-kernel event timing; returns (ms[8], counts[8])
+        accumulated since the previous call."""
+        ms = np.zeros(8)
+        cnt = np.zeros(8, np.int64)
+        self.ctx.check(self.ctx.lib.tomhip_search_profile(self.h, int(enable), _p(ms), _p(cnt)))
+        return ms, cnt
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.tomhip_search_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tom_os3_findTemplate(volume, template, mask=None, angles=None, W=None, batch: int = 4,
+                         ctx: Optional[Context] = None):
+    """ccc-only tom_os3_findTemplate: (ccc map, 1-based angle index map)."""
+    vb, tb = _Buf(volume), _Buf(template)
+    mb = _Buf(mask) if mask is not None else None
+    wb = _Buf(W) if W is not None else None
+    ctx = _prep_ctx(ctx, vb, tb, mb, wb)
+    ang = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(-1, 3))
+    N, n = _dims3(vb.dims), _dims3(tb.dims)
+    sc = _empty_like(vb, vb.dims)
+    an = _empty_like(vb, vb.dims, np.int32)
+    ctx.check(ctx.lib.tomhip_flcf_search_f32(ctx.h, vb.ptr, _p(N), tb.ptr, _p(n), None if mb is None else mb.ptr,
+                                             None if wb is None else wb.ptr, _p(ang), ang.shape[0], int(batch),
+                                             _ptr(sc), _ptr(an)))
+    return sc, an + 1
+
+
+def tom_os3_corr(img, template, mask=None, ctx: Optional[Context] = None):
+    """FLCF score map of tom_os3_corr(img, template, options)."""
+    vb, tb = _Buf(img), _Buf(template)
+    mb = _Buf(mask) if mask is not None else None
+    ctx = _prep_ctx(ctx, vb, tb, mb)
+    N, n = _dims3(vb.dims), _dims3(tb.dims)
+    out = _empty_like(vb, vb.dims)
+    ctx.check(ctx.lib.tomhip_os3_corr_f32(ctx.h, vb.ptr, _p(N), tb.ptr, _p(n), None if mb is None else mb.ptr,
+                                          _ptr(out)))
+    return out
+
+
+def _colon(a: float, step: float, b: float) -> np.ndarray:
+    """MATLAB a:step:b."""
+    if step == 0 or (step > 0 and a > b) or (step < 0 and a < b):
+        return np.zeros(0)
+    n = int(np.floor((b - a) / step + 1e-10)) + 1
+    return a + step * np.arange(n)
+
+
+def tom_oscar(vol, template, phi_start, phi_end, phi_incr, psi_start, psi_end, psi_incr, the_start, the_end,
+              the_incr, ctx: Optional[Context] = None):
+    """[ccfmax, phi_max, psi_max, theta_max] = tom_oscar(...) (int16 angle
+    maps, -1 where no angle improved on the initial -1000)."""
+    vb, tb = _Buf(vol), _Buf(template)
+    ctx = _prep_ctx(ctx, vb, tb)
+    ang = np.asarray([(p, s, t) for p in _colon(phi_start, phi_incr, phi_end)
+                      for s in _colon(psi_start, psi_incr, psi_end)
+                      for t in _colon(the_start, the_incr, the_end)], np.float32).reshape(-1, 3)
+    N, n = _dims3(vb.dims), _dims3(tb.dims)
+    sc = _empty_like(vb, vb.dims)
+    an = _empty_like(vb, vb.dims, np.int32)
+    ctx.check(ctx.lib.tomhip_oscar_f32(ctx.h, vb.ptr, _p(N), tb.ptr, _p(n), _p(np.ascontiguousarray(ang)),
+                                       ang.shape[0], _ptr(sc), _ptr(an)))
+    idx = an.cpu().numpy() if _is_torch(an) else an
+    ccf = sc.cpu().numpy() if _is_torch(sc) else sc
+    # tom_oscar keeps -1000 / -1 where nothing beat the initial value
+    never = (idx < 0) | (ccf <= -1000)
+    safe = np.where(never, 0, idx)
+    maps = [np.where(never, -1, ang[safe, k]).astype(np.int16) for k in range(3)]
+    ccf = np.where(never, -1000.0, ccf)
+    return ccf, maps[0], maps[1], maps[2]

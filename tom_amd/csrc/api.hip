@@ -1,0 +1,545 @@
+// C ABI of libtomhip.so (declared in include/tomhip.h).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tomhip.h"
+#include "rotate.h"
+#include "search.h"
+#include "util.h"
+
+using namespace tomhip;
+
+struct tomhip_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::unique_ptr<Twiddles> tw;
+};
+
+struct tomhip_search {
+  tomhip_ctx* ctx = nullptr;
+  std::unique_ptr<Search> s;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(tomhip_ctx* ctx, F&& f) {
+  try {
+    if (ctx) TOMHIP_CHECK(hipSetDevice(ctx->device));
+    f();
+    return 0;
+  } catch (const Error& e) {
+    (ctx ? ctx->err : g_err) = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    (ctx ? ctx->err : g_err) = "out of host memory";
+    return ERR_NO_MEMORY;
+  } catch (const std::exception& e) {
+    (ctx ? ctx->err : g_err) = e.what();
+    return ERR_INTERNAL;
+  }
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
+}
+
+// Device view of a caller buffer: the pointer itself when it is device memory,
+// else a device copy (input) and/or a copy back on commit() (output).
+class DevBuf {
+ public:
+  DevBuf(hipStream_t s, const void* p, size_t bytes, bool in, bool out)
+      : s_(s), host_(const_cast<void*>(p)), bytes_(bytes), out_(out) {
+    if (!p || bytes == 0) return;
+    if (is_device_ptr(p)) {
+      dev_ = const_cast<void*>(p);
+      host_ = nullptr;
+      return;
+    }
+    TOMHIP_CHECK(hipMalloc(&dev_, bytes));
+    own_ = true;
+    if (in) TOMHIP_CHECK(hipMemcpyAsync(dev_, p, bytes, hipMemcpyHostToDevice, s_));
+  }
+  ~DevBuf() {
+    if (own_) (void)hipFree(dev_);
+  }
+  template <typename T>
+  T* get() const { return static_cast<T*>(dev_); }
+  void commit() {
+    if (own_ && out_) {
+      TOMHIP_CHECK(hipMemcpyAsync(host_, dev_, bytes_, hipMemcpyDeviceToHost, s_));
+      TOMHIP_CHECK(hipStreamSynchronize(s_));
+    }
+  }
+
+ private:
+  hipStream_t s_;
+  void* host_;
+  void* dev_ = nullptr;
+  size_t bytes_;
+  bool out_;
+  bool own_ = false;
+};
+
+void check_dims3(const int64_t* n, const char* what) {
+  TOMHIP_REQUIRE(n != nullptr, ERR_WRONG_INPUT, std::string(what) + " dims missing");
+  for (int d = 0; d < 3; ++d)
+    TOMHIP_REQUIRE(n[d] >= 1 && n[d] <= (1 << 20), ERR_WRONG_INPUT, std::string(what) + " dims out of range");
+}
+
+void require_finite(const float* rm, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    TOMHIP_REQUIRE(std::isfinite(rm[i]), ERR_WRONG_INPUT, "rotation angles/matrix must be finite");
+}
+
+std::vector<float> rotmats(const float* euler, int64_t count) {
+  std::vector<float> rm((size_t)count * 9);
+  for (int64_t a = 0; a < count; ++a) rotmat_ref(euler + 3 * a, 3, rm.data() + 9 * a);
+  require_finite(rm.data(), rm.size());
+  return rm;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tomhip_init(int device, tomhip_ctx** out) {
+  return guard(nullptr, [&] {
+    TOMHIP_REQUIRE(out != nullptr, ERR_WRONG_INPUT, "null output pointer");
+    int n = 0;
+    TOMHIP_CHECK(hipGetDeviceCount(&n));
+    TOMHIP_REQUIRE(device >= 0 && device < n, ERR_WRONG_INPUT, "no such device");
+    TOMHIP_CHECK(hipSetDevice(device));
+    auto c = std::make_unique<tomhip_ctx>();
+    c->device = device;
+    TOMHIP_CHECK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    c->stream = c->own;
+    c->tw = std::make_unique<Twiddles>();
+    *out = c.release();
+  });
+}
+
+void tomhip_destroy(tomhip_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->tw.reset();
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+}
+
+const char* tomhip_last_error(tomhip_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int tomhip_set_stream(tomhip_ctx* ctx, void* st) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx, ERR_WRONG_INPUT, "null context");
+    ctx->stream = st ? static_cast<hipStream_t>(st) : ctx->own;
+  });
+}
+
+int tomhip_synchronize(tomhip_ctx* ctx) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx, ERR_WRONG_INPUT, "null context");
+    TOMHIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int tomhip_rotate3d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, int64_t nz,
+                        const float* euler, int neuler, const float* center) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && src && dst && euler, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(neuler == 3 || neuler == 9, ERR_WRONG_INPUT, "euler must have 3 or 9 entries");
+    TOMHIP_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz < (1L << 31), ERR_WRONG_INPUT, "bad dims");
+    const size_t bytes = sizeof(float) * nx * ny * nz;
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, src, bytes, true, false);
+    DevBuf out(s, dst, bytes, false, true);
+    float rm[9];
+    rotmat_ref(euler, neuler, rm);
+    require_finite(rm, 9);
+    for (int d = 0; d < 3 && center; ++d)
+      TOMHIP_REQUIRE(std::isfinite(center[d]), ERR_WRONG_INPUT, "center must be finite");
+    float* drm = nullptr;
+    TOMHIP_CHECK(hipMalloc(&drm, sizeof(rm)));
+    TOMHIP_CHECK(hipMemcpyAsync(drm, rm, sizeof(rm), hipMemcpyHostToDevice, s));
+    const float c[3] = {center ? center[0] : (float)((nx + 1) / 2), center ? center[1] : (float)((ny + 1) / 2),
+                        center ? center[2] : (float)((nz + 1) / 2)};
+    launch_rotate(in.get<float>(), out.get<float>(), drm, 1, nullptr, (int)nx, (int)ny, (int)nz, c[0], c[1],
+                  c[2], 0, s);
+    out.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    hipFree(drm);
+  });
+}
+
+int tomhip_rotate3d_batch_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, int64_t nz,
+                              const float* euler, int64_t count, const float* center, const float* mask) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && src && dst && euler, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(count >= 1 && count <= 65535, ERR_WRONG_INPUT, "count must be in [1, 65535]");
+    TOMHIP_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz < (1L << 31), ERR_WRONG_INPUT, "bad dims");
+    const long n3 = nx * ny * nz;
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, src, sizeof(float) * n3, true, false);
+    DevBuf mk_(s, mask, sizeof(float) * n3, true, false);
+    DevBuf out(s, dst, sizeof(float) * n3 * count, false, true);
+    std::vector<float> rm = rotmats(euler, count);
+    float* drm = nullptr;
+    TOMHIP_CHECK(hipMalloc(&drm, sizeof(float) * rm.size()));
+    TOMHIP_CHECK(hipMemcpyAsync(drm, rm.data(), sizeof(float) * rm.size(), hipMemcpyHostToDevice, s));
+    const float c[3] = {center ? center[0] : (float)((nx + 1) / 2), center ? center[1] : (float)((ny + 1) / 2),
+                        center ? center[2] : (float)((nz + 1) / 2)};
+    launch_rotate(in.get<float>(), out.get<float>(), drm, (int)count, mask ? mk_.get<float>() : nullptr, (int)nx,
+                  (int)ny, (int)nz, c[0], c[1], c[2], n3, s);
+    out.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    hipFree(drm);
+  });
+}
+
+int tomhip_peak_f32(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val, int32_t pos[3]) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && dims && max_val && pos, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, ERR_WRONG_INPUT, "bad dims");
+    const long n = (long)dims[0] * dims[1] * dims[2];
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, map, sizeof(float) * n, true, false);
+    float v;
+    long i;
+    global_max(in.get<float>(), n, &v, &i, s);
+    // find_max starts at -99999 and only moves on a strict '>'
+    if (!(v > -99999.0f)) {
+      *max_val = -99999.0f;
+      return;  // positions are left as passed in, like the reference
+    }
+    *max_val = v;
+    pos[0] = (int32_t)(i % dims[0]);
+    pos[1] = (int32_t)((i / dims[0]) % dims[1]);
+    pos[2] = (int32_t)(i / ((long)dims[0] * dims[1]));
+  });
+}
+
+int tomhip_spheremask_os3_f32(tomhip_ctx* ctx, const int64_t n[3], float* out, int64_t* count) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(n, "mask");
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf o(s, out, sizeof(float) * n3, false, true);
+    launch_spheremask_os3((int)n[0], (int)n[1], (int)n[2], o.get<float>(), s);
+    if (count) {
+      std::vector<float> h(n3);
+      TOMHIP_CHECK(hipMemcpyAsync(h.data(), o.get<float>(), sizeof(float) * n3, hipMemcpyDeviceToHost, s));
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+      int64_t c = 0;
+      for (float v : h) c += (v > 0);
+      *count = c;
+    }
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_spheremask_f32(tomhip_ctx* ctx, const int64_t n[3], float radius, float sigma, float max_radius,
+                          float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(n, "mask");
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf o(s, out, sizeof(float) * n3, false, true);
+    launch_spheremask((int)n[0], (int)n[1], (int)n[2], radius, sigma, max_radius, o.get<float>(), s);
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_weight_f32(tomhip_ctx* ctx, const int64_t n[3], double semi_angle_deg, double lo, double hi,
+                      float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(n, "weight");
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf o(s, out, sizeof(float) * n3, false, true);
+    launch_weight((int)n[0], (int)n[1], (int)n[2], semi_angle_deg > 0, semi_angle_deg, hi > 0, lo, hi, 0,
+                  o.get<float>(), s);
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+// ifftshift of a centred weight (device in, device out)
+static float* shifted_weight(hipStream_t s, const float* Wc, const int64_t* n) {
+  const long n3 = n[0] * n[1] * n[2];
+  std::vector<float> h(n3), o(n3);
+  TOMHIP_CHECK(hipMemcpyAsync(h.data(), Wc, sizeof(float) * n3, hipMemcpyDeviceToHost, s));
+  TOMHIP_CHECK(hipStreamSynchronize(s));
+  for (long z = 0; z < n[2]; ++z)
+    for (long y = 0; y < n[1]; ++y)
+      for (long x = 0; x < n[0]; ++x) {
+        const long sx = (x + n[0] / 2) % n[0], sy = (y + n[1] / 2) % n[1], sz = (z + n[2] / 2) % n[2];
+        o[(z * n[1] + y) * n[0] + x] = h[(sz * n[1] + sy) * n[0] + sx];
+      }
+  float* d = nullptr;
+  TOMHIP_CHECK(hipMalloc(&d, sizeof(float) * n3));
+  TOMHIP_CHECK(hipMemcpyAsync(d, o.data(), sizeof(float) * n3, hipMemcpyHostToDevice, s));
+  TOMHIP_CHECK(hipStreamSynchronize(s));
+  return d;
+}
+
+int tomhip_prepare_template_f32(tomhip_ctx* ctx, const float* tmpl, const int64_t n[3], const float* mask,
+                                const float* W, float* tmpl_out, double* mu_t, double* sig_t) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && tmpl && mask && tmpl_out && mu_t && sig_t, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(n, "template");
+    TOMHIP_REQUIRE(n[0] * n[1] * n[2] <= 64 * 64 * 64, ERR_UNSUPPORTED, "template larger than 64^3");
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf m(s, mask, sizeof(float) * n3, true, false);
+    DevBuf w(s, W, sizeof(float) * n3, true, false);
+    DevBuf o(s, tmpl_out, sizeof(float) * n3, false, true);
+    float* wsh = W ? shifted_weight(s, w.get<float>(), n) : nullptr;
+    try {
+      prepare_template(*ctx->tw, s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], m.get<float>(), wsh,
+                       o.get<float>(), mu_t, sig_t);
+    } catch (...) {
+      hipFree(wsh);
+      throw;
+    }
+    hipFree(wsh);
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_search_create(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* mask,
+                         const int64_t n[3], int mode, int batch, tomhip_search** out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && volume && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "volume");
+    check_dims3(n, "template");
+    const long NN = N[0] * N[1] * N[2], n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf v(s, volume, sizeof(float) * NN, true, false);
+    DevBuf m(s, mask, sizeof(float) * n3, true, false);
+    auto h = std::make_unique<tomhip_search>();
+    h->ctx = ctx;
+    h->s = std::make_unique<Search>(*ctx->tw, s, v.get<float>(), (int)N[0], (int)N[1], (int)N[2],
+                                    mask ? m.get<float>() : nullptr, (int)n[0], (int)n[1], (int)n[2], mode,
+                                    batch > 0 ? batch : 4);
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    *out = h.release();
+  });
+}
+
+int tomhip_search_set_template(tomhip_search* h, const float* tmpl, const float* W, double mu_t, double sig_t,
+                               int remask) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h && tmpl, ERR_WRONG_INPUT, "null pointer");
+    Search& S = *h->s;
+    const int64_t n[3] = {S.nx, S.ny, S.nz};
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf w(s, W, sizeof(float) * n3, true, false);
+    float* wsh = W ? shifted_weight(s, w.get<float>(), n) : nullptr;
+    try {
+      S.set_template(t.get<float>(), wsh, mu_t, sig_t, remask != 0);
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      hipFree(wsh);
+      throw;
+    }
+    hipFree(wsh);
+  });
+}
+
+int tomhip_search_run(tomhip_search* h, const float* euler, int64_t count, int64_t first_index) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h && (euler || count == 0), ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(count >= 0, ERR_WRONG_INPUT, "negative count");
+    std::vector<float> rm = rotmats(euler, count);
+    h->s->run(rm.data(), count, first_index);
+  });
+}
+
+int tomhip_search_finalize(tomhip_search* h, float* score, int32_t* angle) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h && score && angle, ERR_WRONG_INPUT, "null pointer");
+    const long N = h->s->N;
+    hipStream_t s = ctx->stream;
+    DevBuf sc(s, score, sizeof(float) * N, false, true);
+    DevBuf an(s, angle, sizeof(int32_t) * N, false, true);
+    h->s->finalize(sc.get<float>(), an.get<int32_t>());
+    sc.commit();
+    an.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_search_profile(tomhip_search* h, int enable, double* ms, int64_t* counts) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h, ERR_WRONG_INPUT, "null search");
+    const KernelTimes& t = h->s->times();
+    for (int i = 0; i < 8; ++i) {
+      if (ms) ms[i] = t.ms[i];
+      if (counts) counts[i] = t.n[i];
+    }
+    h->s->reset_times();
+    h->s->set_profile(enable != 0);
+  });
+}
+
+void tomhip_search_destroy(tomhip_search* h) {
+  if (!h) return;
+  if (h->ctx) {
+    (void)hipSetDevice(h->ctx->device);
+    (void)hipStreamSynchronize(h->ctx->stream);
+  }
+  delete h;
+}
+
+int tomhip_flcf_search_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
+                           const int64_t n[3], const float* mask, const float* W, const float* angles, int64_t A,
+                           int batch, float* score, int32_t* angle) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && volume && tmpl && angles && score && angle, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(A >= 1, ERR_WRONG_INPUT, "no angles");
+    check_dims3(N, "volume");
+    check_dims3(n, "template");
+    const long n3 = n[0] * n[1] * n[2], NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf v(s, volume, sizeof(float) * NN, true, false);
+    DevBuf w(s, W, sizeof(float) * n3, true, false);
+    DevBuf mk_(s, mask, sizeof(float) * n3, true, false);
+    float* own_mask = nullptr;
+    float* tn = nullptr;
+    float* wsh = nullptr;
+    auto cleanup = [&] {
+      hipFree(own_mask);
+      hipFree(tn);
+      hipFree(wsh);
+    };
+    try {
+      const float* M = mask ? mk_.get<float>() : nullptr;
+      if (!M) {
+        TOMHIP_CHECK(hipMalloc(&own_mask, sizeof(float) * n3));
+        launch_spheremask_os3((int)n[0], (int)n[1], (int)n[2], own_mask, s);
+        M = own_mask;
+      }
+      TOMHIP_CHECK(hipMalloc(&tn, sizeof(float) * n3));
+      if (W) wsh = shifted_weight(s, w.get<float>(), n);
+      double mu_t = 0, sig_t = 1;
+      prepare_template(*ctx->tw, s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], M, wsh, tn, &mu_t, &sig_t);
+      Search S(*ctx->tw, s, v.get<float>(), (int)N[0], (int)N[1], (int)N[2], M, (int)n[0], (int)n[1], (int)n[2],
+               MODE_FLCF, batch > 0 ? batch : 4);
+      S.set_template(tn, wsh, mu_t, sig_t, true);
+      std::vector<float> rm = rotmats(angles, A);
+      S.run(rm.data(), A, 0);
+      DevBuf sc(s, score, sizeof(float) * NN, false, true);
+      DevBuf an(s, angle, sizeof(int32_t) * NN, false, true);
+      S.finalize(sc.get<float>(), an.get<int32_t>());
+      sc.commit();
+      an.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
+int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], const float* tmpl,
+                        const int64_t n[3], const float* mask, float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && img && tmpl && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "image");
+    check_dims3(n, "template");
+    const long n3 = n[0] * n[1] * n[2], NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf v(s, img, sizeof(float) * NN, true, false);
+    DevBuf mk_(s, mask, sizeof(float) * n3, true, false);
+    float* own_mask = nullptr;
+    int32_t* ang = nullptr;
+    auto cleanup = [&] {
+      hipFree(own_mask);
+      hipFree(ang);
+    };
+    try {
+      const float* M = mask ? mk_.get<float>() : nullptr;
+      if (!M) {
+        TOMHIP_CHECK(hipMalloc(&own_mask, sizeof(float) * n3));
+        launch_spheremask_os3((int)n[0], (int)n[1], (int)n[2], own_mask, s);
+        M = own_mask;
+      }
+      double mu_t = 0, sig_t = 1;
+      template_stats(s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], M, &mu_t, &sig_t);
+      Search S(*ctx->tw, s, v.get<float>(), (int)N[0], (int)N[1], (int)N[2], M, (int)n[0], (int)n[1], (int)n[2],
+               MODE_FLCF, 1);
+      S.set_template(t.get<float>(), nullptr, mu_t, sig_t, false);
+      const float zero[3] = {0, 0, 0};
+      std::vector<float> rm = rotmats(zero, 1);  // exact identity
+      S.run(rm.data(), 1, 0);
+      TOMHIP_CHECK(hipMalloc(&ang, sizeof(int32_t) * NN));
+      DevBuf o(s, out, sizeof(float) * NN, false, true);
+      S.finalize(o.get<float>(), ang);
+      o.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
+int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
+                     const int64_t n[3], const float* angles, int64_t A, float* ccfmax, int32_t* angle) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && volume && tmpl && angles && ccfmax && angle, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(A >= 1, ERR_WRONG_INPUT, "no angles");
+    check_dims3(N, "volume");
+    check_dims3(n, "template");
+    for (int d = 0; d < 3; ++d)
+      TOMHIP_REQUIRE(N[d] % 2 == 0 && n[d] % 2 == 0, ERR_WRONG_INPUT, "tom_oscar needs even sizes");
+    const long n3 = n[0] * n[1] * n[2], NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf v(s, volume, sizeof(float) * NN, true, false);
+    Search S(*ctx->tw, s, v.get<float>(), (int)N[0], (int)N[1], (int)N[2], nullptr, (int)n[0], (int)n[1],
+             (int)n[2], MODE_XCF, 4);
+    S.set_template(t.get<float>(), nullptr, 0.0, 1.0, false);
+    std::vector<float> rm = rotmats(angles, A);
+    S.run(rm.data(), A, 0);
+    DevBuf sc(s, ccfmax, sizeof(float) * NN, false, true);
+    DevBuf an(s, angle, sizeof(int32_t) * NN, false, true);
+    S.finalize(sc.get<float>(), an.get<int32_t>());
+    sc.commit();
+    an.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// Trilinear ZXZ rotation (tom_rotatec semantics) on gfx950.
+//
+// Reference: Sptrans/tom_rotatec.c rot3d:146-401 (identical arithmetic to
+// Oscar3/c++Picker/src/tom/core/rotate.c rot3dlin:719-843). The per-voxel
+// arithmetic below follows the reference's float evaluation order exactly and
+// is compiled with FP contraction off, so results are bit-identical to it.
+//
+// Design: one workgroup produces one 8x8x8 output brick of one rotation. It
+// first computes the source-space bounding box of the brick from its 8
+// corners, loads that box (at most 18^3 floats) into LDS with coalesced,
+// wavefront-cooperative row loads, then interpolates every output voxel from
+// LDS. Boxes that do not fit (non-rotation 3x3 matrices with scaling) fall
+// back to gathering straight from global memory. A batch of rotations of the
+// same volume runs in one launch (grid.z = rotation index); an optional mask
+// multiplies the result (t = rot(T) .* M of the template-matching loop).
+#include "common.h"
+#include "rotate.h"
+
+#pragma clang fp contract(off)
+
+namespace tomhip {
+
+namespace {
+
+constexpr int BR = 8;            // output brick edge
+constexpr int BOX = 18;          // max LDS box edge (floats)
+constexpr int NT_ROT = 256;
+
+struct RotParams {
+  const float* src;
+  float* dst;
+  const float* rm;      // [nrot][9]: rm00 rm10 rm20 rm01 rm11 rm21 rm02 rm12 rm22
+  const float* mask;    // optional, same size as src
+  int sx, sy, sz;
+  float px, py, pz;
+  long dst_stride;      // elements between consecutive rotations in dst
+};
+
+__device__ __forceinline__ float interp(const float* img, int b, int sx, int sxy, int ex, int ey,
+                                        int ez, float vx2, float vx1, float vy2, float vy1, float vz2,
+                                        float vz1) {
+  const float i0 = ex ? 0.f : img[b + 1];
+  const float i1 = ey ? 0.f : img[b + sx];
+  const float i2 = (ex || ey) ? 0.f : img[b + sx + 1];
+  const float i3 = ez ? 0.f : img[b + sxy];
+  const float i4 = (ex || ez) ? 0.f : img[b + sxy + 1];
+  const float i5 = (ey || ez) ? 0.f : img[b + sx + sxy];
+  const float i6 = (ex || ey || ez) ? 0.f : img[b + sx + sxy + 1];
+  const float a = img[b];
+  const float AA = a + (i0 - a) * vx2;
+  const float BB = i1 * vx1 + i2 * vx2;
+  const float CC = i3 * vx1 + i4 * vx2;
+  const float DD = i5 * vx1 + i6 * vx2;
+  return (AA * vy1 + BB * vy2) * vz1 + (CC * vy1 + DD * vy2) * vz2;
+}
+
+__global__ __launch_bounds__(NT_ROT) void k_rotate_brick(RotParams P) {
+  __shared__ float box[BOX * BOX * BOX];
+  __shared__ int sh_lo[3], sh_hi[3], sh_fit;
+
+  const int nbx = (P.sx + BR - 1) / BR, nby = (P.sy + BR - 1) / BR;
+  const int bid = blockIdx.x;
+  const int bx0 = (bid % nbx) * BR, by0 = ((bid / nbx) % nby) * BR, bz0 = (bid / (nbx * nby)) * BR;
+  const int rot = blockIdx.y;
+  const float* rmp = P.rm + rot * 9;
+  const float rm00 = rmp[0], rm10 = rmp[1], rm20 = rmp[2];
+  const float rm01 = rmp[3], rm11 = rmp[4], rm21 = rmp[5];
+  const float rm02 = rmp[6], rm12 = rmp[7], rm22 = rmp[8];
+  const float px = P.px, py = P.py, pz = P.pz;
+  const int sx = P.sx, sy = P.sy, sz = P.sz, sxy = sx * sy;
+
+  if (threadIdx.x == 0) {
+    // Source bounding box of the brick: the mapped coordinate is affine in
+    // the truncated offsets (pi, pj, pk), which are monotone in (i, j, k), so
+    // the extremes sit at the brick's corners. One voxel of margin each side
+    // covers float rounding; +1 more on the high side for the second tap.
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    const int ie = min(bx0 + BR, sx) - 1, je = min(by0 + BR, sy) - 1, ke = min(bz0 + BR, sz) - 1;
+    for (int c = 0; c < 8; ++c) {
+      const int i = (c & 1) ? ie : bx0, j = (c & 2) ? je : by0, k = (c & 4) ? ke : bz0;
+      const float pi = (float)(int)((float)i - px), pj = (float)(int)((float)j - py),
+                  pk = (float)(int)((float)k - pz);
+      const float r[3] = {px + rm00 * pi + rm10 * pj + rm20 * pk, py + rm01 * pi + rm11 * pj + rm21 * pk,
+                          pz + rm02 * pi + rm12 * pj + rm22 * pk};
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = fminf(lo[d], r[d]);
+        hi[d] = fmaxf(hi[d], r[d]);
+      }
+    }
+    const int n[3] = {sx, sy, sz};
+    int fit = 1;
+    for (int d = 0; d < 3; ++d) {
+      int l = (int)floorf(lo[d]) - 1, h = (int)floorf(hi[d]) + 2;
+      l = max(l, 0);
+      h = min(h, n[d] - 1);
+      if (h < l) h = l;  // brick maps entirely outside: nothing is read
+      sh_lo[d] = l;
+      sh_hi[d] = h;
+      if (h - l + 1 > BOX) fit = 0;
+    }
+    sh_fit = fit;
+  }
+  __syncthreads();
+  const int fit = sh_fit;
+  const int lx = sh_lo[0], ly = sh_lo[1], lz = sh_lo[2];
+  const int ex_ = sh_hi[0] - lx + 1, ey_ = sh_hi[1] - ly + 1, ez_ = sh_hi[2] - lz + 1;
+  const float* img = P.src;
+  if (fit) {
+    const int tot = ex_ * ey_ * ez_;
+    for (int t = threadIdx.x; t < tot; t += NT_ROT) {
+      const int x = t % ex_, y = (t / ex_) % ey_, z = t / (ex_ * ey_);
+      box[(z * BOX + y) * BOX + x] = img[(long)(lz + z) * sxy + (long)(ly + y) * sx + (lx + x)];
+    }
+  }
+  __syncthreads();
+
+  float* out = P.dst + (long)rot * P.dst_stride;
+  for (int v = threadIdx.x; v < BR * BR * BR; v += NT_ROT) {
+    const int i = bx0 + (v % BR), j = by0 + ((v / BR) % BR), k = bz0 + v / (BR * BR);
+    if (i >= sx || j >= sy || k >= sz) continue;
+    const long o = (long)k * sxy + (long)j * sx + i;
+    const float pi = (float)(int)((float)i - px);
+    const float pj = (float)(int)((float)j - py);
+    const float pk = (float)(int)((float)k - pz);
+    float val = 0.f;
+    const float rx = px + rm00 * pi + rm10 * pj + rm20 * pk;
+    if (!(rx < 0 || rx > (float)(sx - 1))) {
+      const float ry = py + rm01 * pi + rm11 * pj + rm21 * pk;
+      if (!(ry < 0 || ry > (float)(sy - 1))) {
+        const float rz = pz + rm02 * pi + rm12 * pj + rm22 * pk;
+        if (!(rz < 0 || rz > (float)(sz - 1))) {
+          const int fx = (int)rx, fy = (int)ry, fz = (int)rz;
+          const float vx2 = rx - (float)fx, vx1 = 1 - vx2;
+          const float vy2 = ry - (float)fy, vy1 = 1 - vy2;
+          const float vz2 = rz - (float)fz, vz1 = 1 - vz2;
+          const int ex = (fx == sx - 1), ey = (fy == sy - 1), ez = (fz == sz - 1);
+          if (fit) {
+            const int b = ((fz - lz) * BOX + (fy - ly)) * BOX + (fx - lx);
+            val = interp(box, b, BOX, BOX * BOX, ex, ey, ez, vx2, vx1, vy2, vy1, vz2, vz1);
+          } else {
+            const int b = fx + fy * sx + fz * sxy;
+            val = interp(img, b, sx, sxy, ex, ey, ez, vx2, vx1, vy2, vy1, vz2, vz1);
+          }
+        }
+      }
+    }
+    if (P.mask) val = val * P.mask[o];
+    out[o] = val;
+  }
+}
+
+}  // namespace
+
+// Rotation matrix exactly as rot3d builds it (tom_rotatec.c:146-291): exact
+// cos/sin for the 16 table angles, else cos/sin in double of the float-rounded
+// radian value; products in float.
+static void cossin_ref(float deg, float* c, float* s) {
+  static const float tab[16] = {0, 30, 45, 60, 90, 120, 135, 150, 180, 210, 225, 240, 270, 300, 315, 330};
+  const float tc[16] = {1.0f, (float)(sqrt(3.0) / 2), (float)(sqrt(2.0) / 2), 0.5f, 0.0f, -0.5f,
+                        (float)(-sqrt(2.0) / 2), (float)(-sqrt(3.0) / 2), -1.0f, (float)(-sqrt(3.0) / 2),
+                        (float)(-sqrt(2.0) / 2), -0.5f, 0.0f, 0.5f, (float)(sqrt(2.0) / 2), (float)(sqrt(3.0) / 2)};
+  const float ts[16] = {0.0f, 0.5f, (float)(sqrt(2.0) / 2), (float)(sqrt(3.0) / 2), 1.0f, (float)(sqrt(3.0) / 2),
+                        (float)(sqrt(2.0) / 2), 0.5f, 0.0f, -0.5f, (float)(-sqrt(2.0) / 2),
+                        (float)(-sqrt(3.0) / 2), -1.0f, (float)(-sqrt(3.0) / 2), (float)(-sqrt(2.0) / 2), -0.5f};
+  for (int i = 0; i < 16; ++i)
+    if (tab[i] == deg) {
+      *c = tc[i];
+      *s = ts[i];
+      return;
+    }
+  const double kPi = 3.14159265358979323846264338327950288419716939937510;
+  const float rad = (float)(deg * kPi / 180);
+  *c = (float)cos((double)rad);
+  *s = (float)sin((double)rad);
+}
+
+void rotmat_ref(const float* euler, int neuler, float* rm) {
+  if (neuler != 3) {
+    for (int i = 0; i < 9; ++i) rm[i] = euler[i];
+    return;
+  }
+  float cphi, sphi, cpsi, spsi, cthe, sthe;
+  cossin_ref(euler[0], &cphi, &sphi);
+  cossin_ref(euler[1], &cpsi, &spsi);
+  cossin_ref(euler[2], &cthe, &sthe);
+  // volatile temporaries pin the float rounding of every product
+  volatile float t;
+  t = cpsi * cphi; float a = t; t = cthe * spsi; t = t * sphi; rm[0] = a - t;
+  t = spsi * cphi; a = t; t = cthe * cpsi; t = t * sphi; rm[1] = a + t;
+  t = sthe * sphi; rm[2] = t;
+  t = -cpsi * sphi; a = t; t = cthe * spsi; t = t * cphi; rm[3] = a - t;
+  t = -spsi * sphi; a = t; t = cthe * cpsi; t = t * cphi; rm[4] = a + t;
+  t = sthe * cphi; rm[5] = t;
+  t = sthe * spsi; rm[6] = t;
+  t = -sthe * cpsi; rm[7] = t;
+  rm[8] = cthe;
+}
+
+void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, const float* mask, int sx,
+                   int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream) {
+  RotParams P;
+  P.src = src;
+  P.dst = dst;
+  P.rm = rm_dev;
+  P.mask = mask;
+  P.sx = sx;
+  P.sy = sy;
+  P.sz = sz;
+  P.px = px;
+  P.py = py;
+  P.pz = pz;
+  P.dst_stride = dst_stride;
+  const int nb = ((sx + BR - 1) / BR) * ((sy + BR - 1) / BR) * ((sz + BR - 1) / BR);
+  dim3 grid(nb, nrot);
+  hipLaunchKernelGGL(k_rotate_brick, grid, dim3(NT_ROT), 0, stream, P);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tomhip

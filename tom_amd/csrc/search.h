@@ -1,0 +1,91 @@
+// Exhaustive FLCF template-matching engine (host side of search.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.h"
+#include "fft_kernels.h"
+
+namespace tomhip {
+
+enum SearchMode : int {
+  MODE_FLCF = 0,  // tom_os3_findTemplate / tom_os3_corr normalised score
+  MODE_XCF = 1,   // tom_oscar unnormalised cross-correlation
+};
+
+struct KernelTimes {
+  // accumulated milliseconds and launch counts of the per-angle kernels
+  double ms[8] = {0};
+  long n[8] = {0};
+};
+enum KernelId : int { K_ROT = 0, K_BOX = 1, K_TFWD = 2, K_PASSA = 3, K_PASSB = 4, K_PASSC = 5 };
+
+class Search {
+ public:
+  // volume: device, x-fastest X*Y*Z floats. mask: device nx*ny*nz (FLCF mode;
+  // may be null in XCF mode). mask_count = #{mask > 0}.
+  Search(Twiddles& tw, hipStream_t s, const float* volume, int X, int Y, int Z, const float* mask, int nx,
+         int ny, int nz, int mode, int batch);
+  ~Search();
+  Search(const Search&) = delete;
+  Search& operator=(const Search&) = delete;
+
+  // Prepared template (device, nx*ny*nz, already tom_norm'ed in FLCF mode),
+  // optional ifftshift-ed box weight (device, nx*ny*nz or null), fixed
+  // template statistics.
+  // remask: mask the rotated (filtered) template a second time, as
+  // tom_os3_corr does inside the findTemplate loop.
+  void set_template(const float* tmpl, const float* wsh, double mu_t, double sig_t, bool remask);
+  // Rotation matrices (host, [count][9]) for angles first_index..+count-1.
+  void run(const float* rm_host, long count, long first_index);
+  // Writes score (float) and 0-based best-angle index maps (device or host).
+  void finalize(float* score, int32_t* angle);
+
+  void set_profile(bool on) { profile_ = on; }
+  const KernelTimes& times() const { return times_; }
+  void reset_times() { times_ = KernelTimes(); }
+  int batch() const { return KA_; }
+
+  // sizes
+  int X, Y, Z, Xh, nx, ny, nz, mode;
+  long N;
+  double m = 1.0, mu_t = 0.0, sig_t = 1.0;
+  double mask_sum_ = 0.0;  // sum of the mask values
+
+ private:
+  void precompute_stats(const float* volume);
+  void run_batch(const float* rm_dev, int ka, long first_index);
+  template <class F>
+  void timed(int kid, F&& f);
+
+  Twiddles& tw_;
+  hipStream_t s_;
+  int KA_;
+  bool profile_ = false;
+  KernelTimes times_;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+  // device buffers
+  const float* mask_ = nullptr;
+  c32* FI_ = nullptr;     // [Z][Y][Xh]   F_I, kx < X/2
+  c32* FIn_ = nullptr;    // [Y][Z]       F_I at kx = X/2
+  double* mu_ = nullptr;  // [N] local mean (FLCF)
+  double* sig_ = nullptr; // [N] local std  (FLCF)
+  float* maxc_ = nullptr; // [N] running max of the raw correlation
+  int32_t* argm_ = nullptr;  // [N] best angle index
+  float* tmpl_ = nullptr;    // [n3] prepared template (owned copy)
+  float* wsh_ = nullptr;     // [n3] ifftshift(W) or null
+  float* rm_ = nullptr;      // [cap][9] rotation matrices of the current run
+  long rm_cap_ = 0;
+  float* t1_ = nullptr;      // [KA][n3] rotated (masked, filtered) templates
+  c32* tbox_ = nullptr;      // [KA][n3] box filter workspace
+  c32* Ty_ = nullptr;        // [KA][nz][Y][Xh]
+  c32* Tyn_ = nullptr;       // [KA][nz][Y]
+  c32* Q_ = nullptr;         // [KA][Z][Y][Xh]
+  bool started_ = false;
+  bool remask_ = true;
+};
+
+}  // namespace tomhip

@@ -1,0 +1,730 @@
+// Exhaustive FLCF template matching on gfx950: the per-angle hot path and the
+// once-per-tomogram FP64 statistics.
+//
+// Reference semantics (ccc-only tom_os3_findTemplate,
+// Oscar3/matlabPicker/search/tom_os3_findTemplate.m:157-307):
+//   t      = tom_rotate(Tn, angle, 'linear') .* M                    (:264)
+//   t      = tom_apply_weight_function(t, W)   (box-size W, if any)   (:267)
+//   c      = real(ifftshift(ifftn(F_I .* conj(fftn(pasteCenter(t.*M))))))
+//   score  = (c/m - mu_I mu_T) / (sig_T sig_I)          (tom_os3_corr.m:151-176)
+//   update where old < new, maps start at -1000   (tom_os3_bestPeak.m:13-47)
+// and tom_oscar (Filtrans/tom_oscar.m:55-80) in MODE_XCF.
+//
+// Pipeline per batch of KA angles (all FP32 except the statistics):
+//   rotate    t1 = rot(Tn) .* M                 (rotate.hip, LDS bricks)
+//   box W     t1 = real(ifftn(fftn(t1) .* ifftshift(W))) .* M   (5 small passes)
+//   tfwd      x: direct DFT of the n non-zero samples for each kx of a tile,
+//             y: Stockham FFT; template placed with its centre at the origin
+//             (wrap-around), which folds pasteCenter + ifftshift into the
+//             layout, so no shift pass exists
+//   passA     per (ky, kx tile): z forward FFT of the n non-zero planes,
+//             P = F_I conj(F_T), pack P(kx=0) + i P(kx=X/2) into column 0,
+//             z inverse FFT, write Q
+//   passB     per (z, kx tile): y inverse FFT in place
+//   passC     per block of x lines: half-length c2r (pre-twiddle + complex
+//             inverse FFT of X/2) straight into a running per-voxel max and
+//             argmax over the batch's angles, then one compare/update against
+//             the resident max volume
+// With the template statistics fixed per search (mu_T, sig_T from the
+// unrotated filtered template, tom_os3_findTemplate.m:233-234) and sig_I > 0,
+// the score is a strictly increasing function of c at every voxel, so the
+// max/argmax runs on c and the normalisation is applied once in finalize().
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "rotate.h"
+#include "search.h"
+
+namespace tomhip {
+
+namespace {
+
+// ---------------------------------------------------------------- statistics
+__global__ void k_partial_sum(const float* v, long n, double shift, int sq, double* part) {
+  __shared__ double red[NT];
+  double acc = 0;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double d = (double)v[i] - shift;
+    acc += sq ? d * d : d;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// A = (V - g) + i (V - g)^2
+__global__ void k_make_c(const float* v, long n, double g, c64* A) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double d = (double)v[i] - g;
+    A[i] = mk(d, d * d);
+  }
+}
+
+// F_I(k) = (A(k) + conj(A(-k)))/2 + g N delta(k), rounded to FP32, for
+// kx in [0, X/2]. A = FFT(V - g + i (V-g)^2).
+__global__ void k_extract_fi(const c64* A, int X, int Y, int Z, double gN, c32* FI, c32* FIn) {
+  const int Xh = X / 2;
+  const long tot = (long)Z * Y * (Xh + 1);
+  for (long t = (long)blockIdx.x * NT + threadIdx.x; t < tot; t += (long)gridDim.x * NT) {
+    const int kx = (int)(t % (Xh + 1));
+    const long r = t / (Xh + 1);
+    const int ky = (int)(r % Y), kz = (int)(r / Y);
+    const int nx = (X - kx) % X, ny = (Y - ky) % Y, nz = (Z - kz) % Z;
+    const c64 a = A[((long)kz * Y + ky) * X + kx];
+    const c64 b = A[((long)nz * Y + ny) * X + nx];
+    double re = 0.5 * (a.x + b.x), im = 0.5 * (a.y - b.y);
+    if (kx == 0 && ky == 0 && kz == 0) re += gN;
+    const c32 f = mk((float)re, (float)im);
+    if (kx < Xh)
+      FI[((long)kz * Y + ky) * Xh + kx] = f;
+    else
+      FIn[(long)ky * Z + kz] = f;
+  }
+}
+
+// B = mask placed so that voxel u of the mask sits at (u - floor(n/2)) mod N:
+// the FFT convolution then needs no ifftshift (tom_os3_mean.m:57-62).
+__global__ void k_place_mask(const float* M, int nx, int ny, int nz, int X, int Y, int Z, c64* B) {
+  const long tot = (long)X * Y * Z;
+  const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < tot; i += (long)gridDim.x * NT) {
+    const int x = (int)(i % X);
+    const long r = i / X;
+    const int y = (int)(r % Y), z = (int)(r / Y);
+    const int ux = (x + hx) % X, uy = (y + hy) % Y, uz = (z + hz) % Z;
+    double v = 0;
+    if (ux < nx && uy < ny && uz < nz) v = (double)M[((long)uz * ny + uy) * nx + ux];
+    B[i] = mk(v, 0.0);
+  }
+}
+
+__global__ void k_mul_c64(c64* A, const c64* B, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) A[i] = cmul(A[i], B[i]);
+}
+
+// Local statistics of the shifted volume V - g (A = conv(V-g, M) + i
+// conv((V-g)^2, M)), shifted back exactly: with s = sum(M)/m,
+//   mu  = mu_s + g s
+//   var = (e2_s - mu_s^2) + (1 - s)(2 g mu_s + g^2 s)  = E[x^2] - E[x]^2
+// of tom_os3_std.m:57-66 (the (1-s) term vanishes for binary masks); a
+// variance at or below var_floor counts as zero.
+__global__ void k_stats(const c64* A, long n, double inv_nm, double g, double s, double var_floor, double* mu,
+                        double* sig) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double m1 = A[i].x * inv_nm;
+    const double e2 = A[i].y * inv_nm;
+    const double var = (e2 - m1 * m1) + (1.0 - s) * (2.0 * g * m1 + g * g * s);
+    mu[i] = m1 + g * s;
+    sig[i] = (var > var_floor) ? sqrt(var) : 0.0;
+  }
+}
+
+__global__ void k_fill_max(float* maxc, int32_t* argm, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    maxc[i] = -FLT_MAX;
+    argm[i] = -1;
+  }
+}
+
+__global__ void k_square(const float* a, float* b, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) b[i] = a[i] * a[i];
+}
+
+// ---------------------------------------------------------------- box filter
+// real lines of L -> complex, forward FFT along x
+template <int L>
+__global__ __launch_bounds__(NT) void k_box_x_fwd(const float* t, c32* out, long nlines, const c32* twg) {
+  constexpr int NL = contig_lines<float, L>();
+  __shared__ c32 buf[NL * L];
+  __shared__ c32 tw[L];
+  load_table<float, L, NT>(tw, twg);
+  const long l0 = (long)blockIdx.x * NL;
+  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * L);
+  for (int i = threadIdx.x; i < NL * L; i += NT) buf[i] = mk((i < nval) ? t[l0 * L + i] : 0.f, 0.f);
+  __syncthreads();
+  lds_fft<float, L, NL, NT, -1, Contig<L>>(buf, tw);
+  for (int i = threadIdx.x; i < nval; i += NT) out[l0 * L + i] = buf[i];
+}
+
+// inverse FFT along x, real part / n3, times mask -> float
+template <int L>
+__global__ __launch_bounds__(NT) void k_box_x_inv(const c32* in, float* t, const float* M, long nlines,
+                                                  long lines_per_vol, float inv_n3, const c32* twg) {
+  constexpr int NL = contig_lines<float, L>();
+  __shared__ c32 buf[NL * L];
+  __shared__ c32 tw[L];
+  load_table<float, L, NT>(tw, twg);
+  const long l0 = (long)blockIdx.x * NL;
+  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * L);
+  for (int i = threadIdx.x; i < NL * L; i += NT) buf[i] = (i < nval) ? in[l0 * L + i] : mk(0.f, 0.f);
+  __syncthreads();
+  lds_fft<float, L, NL, NT, 1, Contig<L>>(buf, tw);
+  for (int i = threadIdx.x; i < nval; i += NT) {
+    const long g = l0 * L + i;
+    const long vox = ((g / L) % lines_per_vol) * L + (g % L);
+    t[g] = buf[i].x * inv_n3 * M[vox];
+  }
+}
+
+// ---------------------------------------------------------------- template fwd
+// grid (Xh/TW, nz, ka). Output Ty[a][uz][ky][kx] for kx < Xh, Tyn[a][uz][ky]
+// for kx = Xh (tile 0 only).
+template <int Y, int TW>
+__global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c32* Tyn, const c32* twXg,
+                                                    const c32* twYg, int X, int nx, int ny, int nz) {
+  constexpr int P = TW + 1;
+  __shared__ c32 buf[Y * P];
+  __shared__ c32 twY[Y];
+  __shared__ c32 twX[1024];
+  __shared__ float plane[64 * 64];
+  const int Xh = X / 2;
+  const int tile = blockIdx.x, uz = blockIdx.y, a = blockIdx.z;
+  const bool nyq = (tile == 0);
+  const int NLa = nyq ? P : TW;
+  load_table<float, Y, NT>(twY, twYg);
+  for (int i = threadIdx.x; i < X; i += NT) twX[i] = twXg[i];
+  const float* src = t2 + ((long)a * nz + uz) * ny * nx;
+  for (int i = threadIdx.x; i < nx * ny; i += NT) plane[i] = src[i];
+  for (int i = threadIdx.x; i < Y * P; i += NT) buf[i] = mk(0.f, 0.f);
+  __syncthreads();
+  const int hx = nx / 2, hy = ny / 2;
+  const int sY = 2 * (Y / 2) - Y;
+  for (int i = threadIdx.x; i < ny * NLa; i += NT) {
+    const int uy = i / NLa, c = i % NLa;
+    const int kx = (c < TW) ? tile * TW + c : Xh;
+    // phase index (kx * (ux - hx)) mod X, advanced by kx per sample
+    int m = (int)(((long)kx * hx) % X);
+    m = (m == 0) ? 0 : X - m;
+    float re = 0.f, im = 0.f;
+    const float* row = plane + uy * nx;
+    for (int ux = 0; ux < nx; ++ux) {
+      const c32 w = twX[m];
+      const float v = row[ux];
+      re += v * w.x;
+      im -= v * w.y;
+      m += kx;
+      if (m >= X) m -= X;
+    }
+    int ry = (uy - hy + sY) % Y;
+    if (ry < 0) ry += Y;
+    buf[ry * P + c] = mk(re, im);
+  }
+  __syncthreads();
+  if (nyq)
+    lds_fft<float, Y, P, NT, -1, Inter<P>>(buf, twY);
+  else
+    lds_fft<float, Y, TW, NT, -1, Inter<P>>(buf, twY);
+  c32* dst = Ty + ((long)a * nz + uz) * Y * Xh + tile * TW;
+  for (int i = threadIdx.x; i < Y * TW; i += NT) {
+    const int ky = i / TW, c = i % TW;
+    dst[(long)ky * Xh + c] = buf[ky * P + c];
+  }
+  if (nyq) {
+    c32* dn = Tyn + ((long)a * nz + uz) * Y;
+    for (int ky = threadIdx.x; ky < Y; ky += NT) dn[ky] = buf[ky * P + TW];
+  }
+}
+
+// ---------------------------------------------------------------- pass A
+// grid (Xh/TW, Y). For each angle of the batch: z forward FFT of the template
+// lines, product with F_I, Nyquist packing, z inverse FFT -> Q.
+template <int Z, int TW>
+__global__ __launch_bounds__(NT) void k_passA(const c32* Ty, const c32* Tyn, const c32* FI, const c32* FIn,
+                                              c32* Q, const c32* twZg, int Y, int Xh, int nz, int ka) {
+  constexpr int P = TW + 1;
+  constexpr int E = Z * P;
+  constexpr int PER = (E + NT - 1) / NT;
+  __shared__ c32 buf[E];
+  __shared__ c32 twZ[Z];
+  const int tile = blockIdx.x, ky = blockIdx.y;
+  const bool nyq = (tile == 0);
+  load_table<float, Z, NT>(twZ, twZg);
+  c32 fi[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = threadIdx.x + p * NT;
+    c32 v = mk(0.f, 0.f);
+    if (i < E) {
+      const int kz = i / P, c = i % P;
+      if (c < TW)
+        v = FI[((long)kz * Y + ky) * Xh + tile * TW + c];
+      else if (nyq)
+        v = FIn[(long)ky * Z + kz];
+    }
+    fi[p] = v;
+  }
+  const int hz = nz / 2;
+  const int sZ = 2 * (Z / 2) - Z;
+  for (int a = 0; a < ka; ++a) {
+    const c32* ty = Ty + (long)a * nz * Y * Xh;
+    const c32* tyn = Tyn + (long)a * nz * Y;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i < E) {
+        const int r = i / P, c = i % P;
+        int uz = (r + hz - sZ) % Z;
+        if (uz < 0) uz += Z;
+        c32 v = mk(0.f, 0.f);
+        if (uz < nz) {
+          if (c < TW)
+            v = ty[((long)uz * Y + ky) * Xh + tile * TW + c];
+          else if (nyq)
+            v = tyn[(long)uz * Y + ky];
+        }
+        buf[i] = v;
+      }
+    }
+    __syncthreads();
+    if (nyq)
+      lds_fft<float, Z, P, NT, -1, Inter<P>>(buf, twZ);
+    else
+      lds_fft<float, Z, TW, NT, -1, Inter<P>>(buf, twZ);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = threadIdx.x + p * NT;
+      if (i < E) buf[i] = cmulc(fi[p], buf[i]);
+    }
+    __syncthreads();
+    if (nyq) {
+      for (int kz = threadIdx.x; kz < Z; kz += NT) {
+        const c32 p0 = buf[kz * P], pn = buf[kz * P + TW];
+        buf[kz * P] = mk(p0.x - pn.y, p0.y + pn.x);
+      }
+      __syncthreads();
+    }
+    lds_fft<float, Z, TW, NT, 1, Inter<P>>(buf, twZ);
+    c32* q = Q + (long)a * Z * Y * Xh + (long)ky * Xh + tile * TW;
+    for (int i = threadIdx.x; i < Z * TW; i += NT) {
+      const int z = i / TW, c = i % TW;
+      q[(long)z * Y * Xh + c] = buf[z * P + c];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+// grid (Xh/TW, Z, ka): y inverse FFT in place.
+template <int Y, int TW>
+__global__ __launch_bounds__(NT) void k_passB(c32* Q, const c32* twYg, int Z, int Xh) {
+  __shared__ c32 buf[Y * TW];
+  __shared__ c32 twY[Y];
+  load_table<float, Y, NT>(twY, twYg);
+  c32* base = Q + ((long)blockIdx.z * Z + blockIdx.y) * Y * Xh + blockIdx.x * TW;
+  for (int i = threadIdx.x; i < Y * TW; i += NT) {
+    const int e = i / TW, c = i % TW;
+    buf[i] = base[(long)e * Xh + c];
+  }
+  __syncthreads();
+  lds_fft<float, Y, TW, NT, 1, Inter<TW>>(buf, twY);
+  for (int i = threadIdx.x; i < Y * TW; i += NT) {
+    const int e = i / TW, c = i % TW;
+    base[(long)e * Xh + c] = buf[i];
+  }
+}
+
+// ---------------------------------------------------------------- pass C
+template <int XH>
+constexpr int passc_lines() {
+  return (4096 / XH) > 0 ? 4096 / XH : 1;
+}
+
+// grid (ceil(Y*Z / NL)). Half-length c2r along x of every angle of the batch,
+// running max/argmax in registers, one update of the resident maps.
+template <int XH>
+__global__ __launch_bounds__(NT) void k_passC(const c32* Q, float* maxc, int32_t* argm, const c32* twHg,
+                                              const c32* twXg, long nlines, long qstride, int ka,
+                                              int first_index) {
+  constexpr int NL = passc_lines<XH>();
+  constexpr int E = NL * XH;
+  constexpr int PER = (E + NT - 1) / NT;
+  constexpr int NP = XH / 2 + 1;
+  __shared__ c32 buf[E];
+  __shared__ c32 twH[XH];
+  __shared__ c32 twR[XH];
+  load_table<float, XH, NT>(twH, twHg);
+  load_table<float, XH, NT>(twR, twXg);  // first XH entries of the length-2XH table
+  const long l0 = (long)blockIdx.x * NL;
+  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * XH);
+  float bx[PER], by[PER];
+  int ix[PER], iy[PER];
+  for (int a = 0; a < ka; ++a) {
+    const c32* src = Q + (long)a * qstride + l0 * XH;
+    for (int i = threadIdx.x; i < E; i += NT) buf[i] = (i < nval) ? src[i] : mk(0.f, 0.f);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NL * NP; i += NT) {
+      const int l = i / NP, k = i % NP;
+      c32* row = buf + l * XH;
+      if (k == 0) {
+        const c32 g = row[0];  // (G[0], G[X/2]), both real
+        row[0] = mk(g.x + g.y, g.x - g.y);
+      } else if (2 * k == XH) {
+        const c32 g = row[k];
+        row[k] = mk(2.f * g.x, -2.f * g.y);
+      } else if (2 * k < XH) {
+        const c32 gk = row[k], gm = row[XH - k];
+        const c32 A = mk(gk.x + gm.x, gk.y - gm.y);
+        const c32 B = cmul(mk(gk.x - gm.x, gk.y + gm.y), twR[k]);
+        row[k] = mk(A.x - B.y, A.y + B.x);
+        row[XH - k] = mk(A.x + B.y, B.x - A.y);
+      }
+    }
+    __syncthreads();
+    lds_fft<float, XH, NL, NT, 1, Contig<XH>>(buf, twH);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int e = threadIdx.x + p * NT;
+      if (e < E) {
+        const c32 z = buf[e];
+        if (a == 0 || z.x > bx[p]) { bx[p] = z.x; ix[p] = a; }
+        if (a == 0 || z.y > by[p]) { by[p] = z.y; iy[p] = a; }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = threadIdx.x + p * NT;
+    if (e < nval) {
+      const long v = (l0 * XH + e) * 2;
+      const float2 cur = *reinterpret_cast<const float2*>(maxc + v);
+      if (bx[p] > cur.x) {
+        maxc[v] = bx[p];
+        argm[v] = first_index + ix[p];
+      }
+      if (by[p] > cur.y) {
+        maxc[v + 1] = by[p];
+        argm[v + 1] = first_index + iy[p];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- finalize
+__global__ void k_finalize_flcf(const float* maxc, const int32_t* argm, const double* mu, const double* sig,
+                                float* score, int32_t* angle, long n, double inv_nm, double mu_t,
+                                double sig_t) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double den = sig_t * sig[i];
+    const int32_t a = argm[i];
+    if (den > 0 && a >= 0) {
+      score[i] = (float)(((double)maxc[i] * inv_nm - mu[i] * mu_t) / den);
+      angle[i] = a;
+    } else {
+      score[i] = 0.f;
+      angle[i] = 0;
+    }
+  }
+}
+
+// tom_oscar orientation: out[k] = c[-k]
+__global__ void k_finalize_xcf(const float* maxc, const int32_t* argm, float* score, int32_t* angle, int X,
+                               int Y, int Z, double inv_n) {
+  const long n = (long)X * Y * Z;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const int x = (int)(i % X);
+    const long r = i / X;
+    const int y = (int)(r % Y), z = (int)(r / Y);
+    const long j = ((long)((Z - z) % Z) * Y + (Y - y) % Y) * X + (X - x) % X;
+    score[i] = (float)((double)maxc[j] * inv_n);
+    angle[i] = argm[j];
+  }
+}
+
+inline unsigned grid_for(long n) {
+  long b = (n + NT - 1) / NT;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+// ============================================================== host side
+Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_, int Z_, const float* mask,
+               int nx_, int ny_, int nz_, int mode_, int batch)
+    : X(X_), Y(Y_), Z(Z_), Xh(X_ / 2), nx(nx_), ny(ny_), nz(nz_), mode(mode_), tw_(tw), s_(s),
+      KA_(batch) {
+  N = (long)X * Y * Z;
+  TOMHIP_REQUIRE(X % 32 == 0 && supported_len(X / 2) && supported_len(Y) && supported_len(Z), ERR_UNSUPPORTED,
+                 "volume size must be X = 2*L, Y = L, Z = L with L in the supported FFT lengths and X a "
+                 "multiple of 32");
+  TOMHIP_REQUIRE(X <= 1024, ERR_UNSUPPORTED, "X must be <= 1024");
+  TOMHIP_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nx <= 64 && ny <= 64 && nz <= 64, ERR_UNSUPPORTED,
+                 "template edges must be in [1, 64]");
+  TOMHIP_REQUIRE(nx <= X && ny <= Y && nz <= Z, ERR_WRONG_INPUT, "template larger than the volume");
+  TOMHIP_REQUIRE(KA_ >= 1 && KA_ <= 16, ERR_WRONG_INPUT, "batch must be in [1, 16]");
+  TOMHIP_REQUIRE(mode == MODE_FLCF || mode == MODE_XCF, ERR_WRONG_INPUT, "unknown mode");
+  TOMHIP_REQUIRE(mode == MODE_XCF || mask != nullptr, ERR_WRONG_INPUT, "FLCF needs a mask");
+  const long n3 = (long)nx * ny * nz;
+
+  TOMHIP_CHECK(hipMalloc(&FI_, sizeof(c32) * (long)Z * Y * Xh));
+  TOMHIP_CHECK(hipMalloc(&FIn_, sizeof(c32) * (long)Y * Z));
+  TOMHIP_CHECK(hipMalloc(&maxc_, sizeof(float) * N));
+  TOMHIP_CHECK(hipMalloc(&argm_, sizeof(int32_t) * N));
+  TOMHIP_CHECK(hipMalloc(&tmpl_, sizeof(float) * n3));
+  TOMHIP_CHECK(hipMalloc(&t1_, sizeof(float) * n3 * KA_));
+  TOMHIP_CHECK(hipMalloc(&tbox_, sizeof(c32) * n3 * KA_));
+  TOMHIP_CHECK(hipMalloc(&Ty_, sizeof(c32) * (long)KA_ * nz * Y * Xh));
+  TOMHIP_CHECK(hipMalloc(&Tyn_, sizeof(c32) * (long)KA_ * nz * Y));
+  TOMHIP_CHECK(hipMalloc(&Q_, sizeof(c32) * (long)KA_ * Z * Y * Xh));
+  if (mode == MODE_FLCF) {
+    TOMHIP_CHECK(hipMalloc(&mu_, sizeof(double) * N));
+    TOMHIP_CHECK(hipMalloc(&sig_, sizeof(double) * N));
+    float* mk_ = nullptr;
+    TOMHIP_CHECK(hipMalloc(&mk_, sizeof(float) * n3 * 2));
+    TOMHIP_CHECK(hipMemcpyAsync(mk_, mask, sizeof(float) * n3, hipMemcpyDeviceToDevice, s_));
+    hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(NT), 0, s_, mk_, mk_ + n3, n3);
+    TOMHIP_CHECK(hipGetLastError());
+    mask_ = mk_;
+    // m = #{M > 0}
+    std::vector<float> hm(n3);
+    TOMHIP_CHECK(hipMemcpyAsync(hm.data(), mask, sizeof(float) * n3, hipMemcpyDeviceToHost, s_));
+    TOMHIP_CHECK(hipStreamSynchronize(s_));
+    long cnt = 0;
+    double msum = 0;
+    for (float v : hm) {
+      cnt += (v > 0);
+      msum += (double)v;
+    }
+    TOMHIP_REQUIRE(cnt > 0, ERR_WRONG_INPUT, "mask has no positive voxel");
+    m = (double)cnt;
+    mask_sum_ = msum;
+  }
+  TOMHIP_CHECK(hipEventCreate(&ev0_));
+  TOMHIP_CHECK(hipEventCreate(&ev1_));
+  precompute_stats(volume);
+}
+
+Search::~Search() {
+  hipFree(FI_);
+  hipFree(FIn_);
+  hipFree(maxc_);
+  hipFree(argm_);
+  hipFree(tmpl_);
+  hipFree(t1_);
+  hipFree(tbox_);
+  hipFree(Ty_);
+  hipFree(Tyn_);
+  hipFree(Q_);
+  hipFree(mu_);
+  hipFree(sig_);
+  hipFree(wsh_);
+  hipFree(rm_);
+  hipFree(const_cast<float*>(mask_));
+  if (ev0_) hipEventDestroy(ev0_);
+  if (ev1_) hipEventDestroy(ev1_);
+}
+
+void Search::precompute_stats(const float* volume) {
+  // global mean and variance (FP64)
+  const int nb = 1024;
+  double* part = nullptr;
+  TOMHIP_CHECK(hipMalloc(&part, sizeof(double) * nb));
+  std::vector<double> hp(nb);
+  hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s_, volume, N, 0.0, 0, part);
+  TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s_));
+  TOMHIP_CHECK(hipStreamSynchronize(s_));
+  double sum = 0;
+  for (double v : hp) sum += v;
+  const double g = sum / (double)N;
+  hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s_, volume, N, g, 1, part);
+  TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s_));
+  TOMHIP_CHECK(hipStreamSynchronize(s_));
+  double ss = 0;
+  for (double v : hp) ss += v;
+  const double varg = ss / (double)N;
+  hipFree(part);
+
+  c64* A = nullptr;
+  TOMHIP_CHECK(hipMalloc(&A, sizeof(c64) * N));
+  hipLaunchKernelGGL(k_make_c, dim3(grid_for(N)), dim3(NT), 0, s_, volume, N, g, A);
+  TOMHIP_CHECK(hipGetLastError());
+  fft3_c2c<double>(A, X, Y, Z, 1, -1, tw_, s_);
+  hipLaunchKernelGGL(k_extract_fi, dim3(grid_for((long)Z * Y * (Xh + 1))), dim3(NT), 0, s_, A, X, Y, Z,
+                     g * (double)N, FI_, FIn_);
+  TOMHIP_CHECK(hipGetLastError());
+  if (mode == MODE_FLCF) {
+    c64* B = nullptr;
+    TOMHIP_CHECK(hipMalloc(&B, sizeof(c64) * N));
+    hipLaunchKernelGGL(k_place_mask, dim3(grid_for(N)), dim3(NT), 0, s_, mask_, nx, ny, nz, X, Y, Z, B);
+    TOMHIP_CHECK(hipGetLastError());
+    fft3_c2c<double>(B, X, Y, Z, 1, -1, tw_, s_);
+    hipLaunchKernelGGL(k_mul_c64, dim3(grid_for(N)), dim3(NT), 0, s_, A, (const c64*)B, N);
+    TOMHIP_CHECK(hipGetLastError());
+    fft3_c2c<double>(A, X, Y, Z, 1, 1, tw_, s_);
+    const double inv_nm = 1.0 / ((double)N * m);
+    hipLaunchKernelGGL(k_stats, dim3(grid_for(N)), dim3(NT), 0, s_, (const c64*)A, N, inv_nm, g, mask_sum_ / m,
+                       1e-12 * varg, mu_, sig_);
+    TOMHIP_CHECK(hipGetLastError());
+    TOMHIP_CHECK(hipStreamSynchronize(s_));
+    hipFree(B);
+  }
+  TOMHIP_CHECK(hipStreamSynchronize(s_));
+  hipFree(A);
+}
+
+void Search::set_template(const float* tmpl, const float* wsh, double mu_t_, double sig_t_, bool remask) {
+  const long n3 = (long)nx * ny * nz;
+  TOMHIP_CHECK(hipMemcpyAsync(tmpl_, tmpl, sizeof(float) * n3, hipMemcpyDefault, s_));
+  remask_ = remask;
+  if (wsh) {
+    TOMHIP_REQUIRE(mode == MODE_FLCF && remask, ERR_WRONG_INPUT, "a weight needs FLCF mode with remask");
+    TOMHIP_REQUIRE(supported_len(nx) && supported_len(ny) && supported_len(nz), ERR_UNSUPPORTED,
+                   "weighting needs template edges in the supported FFT lengths");
+    if (!wsh_) TOMHIP_CHECK(hipMalloc(&wsh_, sizeof(float) * n3));
+    TOMHIP_CHECK(hipMemcpyAsync(wsh_, wsh, sizeof(float) * n3, hipMemcpyDefault, s_));
+  } else if (wsh_) {
+    hipFree(wsh_);
+    wsh_ = nullptr;
+  }
+  mu_t = mu_t_;
+  sig_t = sig_t_;
+  hipLaunchKernelGGL(k_fill_max, dim3(grid_for(N)), dim3(NT), 0, s_, maxc_, argm_, N);
+  TOMHIP_CHECK(hipGetLastError());
+  started_ = true;
+}
+
+template <class F>
+void Search::timed(int kid, F&& f) {
+  if (!profile_) {
+    f();
+    return;
+  }
+  TOMHIP_CHECK(hipEventRecord(ev0_, s_));
+  f();
+  TOMHIP_CHECK(hipEventRecord(ev1_, s_));
+  TOMHIP_CHECK(hipEventSynchronize(ev1_));
+  float ms = 0;
+  TOMHIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  times_.ms[kid] += ms;
+  times_.n[kid] += 1;
+}
+
+void Search::run(const float* rm_host, long count, long first_index) {
+  TOMHIP_REQUIRE(started_, ERR_WRONG_INPUT, "set_template must be called before run");
+  if (count <= 0) return;
+  TOMHIP_REQUIRE(first_index >= 0 && first_index + count <= INT32_MAX, ERR_WRONG_INPUT, "angle index range");
+  if (count > rm_cap_) {
+    hipFree(rm_);
+    TOMHIP_CHECK(hipMalloc(&rm_, sizeof(float) * 9 * count));
+    rm_cap_ = count;
+  }
+  TOMHIP_CHECK(hipMemcpyAsync(rm_, rm_host, sizeof(float) * 9 * count, hipMemcpyHostToDevice, s_));
+  for (long b = 0; b < count; b += KA_) {
+    const int ka = (int)((count - b < KA_) ? (count - b) : KA_);
+    run_batch(rm_ + 9 * b, ka, first_index + b);
+  }
+}
+
+void Search::run_batch(const float* rm_dev, int ka, long first_index) {
+  const long n3 = (long)nx * ny * nz;
+  const float px = (float)((nx + 1) / 2), py = (float)((ny + 1) / 2), pz = (float)((nz + 1) / 2);
+  // rotate; FLCF: times M (or M^2 when no weight follows, since the masked
+  // template is masked again inside tom_os3_corr)
+  const float* rmask = nullptr;
+  if (mode == MODE_FLCF) rmask = (wsh_ || !remask_) ? mask_ : mask_ + n3;
+  timed(K_ROT, [&] { launch_rotate(tmpl_, t1_, rm_dev, ka, rmask, nx, ny, nz, px, py, pz, n3, s_); });
+
+  if (wsh_) {
+    timed(K_BOX, [&] {
+      const long nl = (long)ka * ny * nz;
+      dispatch_len(nx, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int NL = contig_lines<float, L>();
+        hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, t1_, tbox_, nl,
+                           tw_.f32(L));
+      });
+      TOMHIP_CHECK(hipGetLastError());
+      const long XY = (long)nx * ny;
+      dispatch_len(ny, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int TW = strided_tw<float, L>();
+        hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, nz, ka), dim3(NT), 0, s_,
+                           tbox_, nx, (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
+      });
+      TOMHIP_CHECK(hipGetLastError());
+      dispatch_len(nz, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int TW = strided_tw<float, L>();
+        hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, ny, ka), dim3(NT), 0, s_,
+                           tbox_, nx, XY, (long)nx, n3, tw_.f32(L), (const float*)wsh_);
+      });
+      TOMHIP_CHECK(hipGetLastError());
+      dispatch_len(ny, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int TW = strided_tw<float, L>();
+        hipLaunchKernelGGL((k_c2c_strided<float, L, 1>), dim3((nx + TW - 1) / TW, nz, ka), dim3(NT), 0, s_,
+                           tbox_, nx, (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
+      });
+      TOMHIP_CHECK(hipGetLastError());
+      dispatch_len(nx, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int NL = contig_lines<float, L>();
+        hipLaunchKernelGGL(k_box_x_inv<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_,
+                           (const c32*)tbox_, t1_, mask_, nl, (long)ny * nz, (float)(1.0 / (double)n3),
+                           tw_.f32(L));
+      });
+      TOMHIP_CHECK(hipGetLastError());
+    });
+  }
+
+  timed(K_TFWD, [&] {
+    dispatch_len(Y, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if constexpr (L >= 1024) {
+        hipLaunchKernelGGL((k_tmpl_fwd_xy<L, 8>), dim3(Xh / 8, nz, ka), dim3(NT), 0, s_, (const float*)t1_, Ty_,
+                           Tyn_, tw_.f32(X), tw_.f32(L), X, nx, ny, nz);
+      } else {
+        hipLaunchKernelGGL((k_tmpl_fwd_xy<L, 16>), dim3(Xh / 16, nz, ka), dim3(NT), 0, s_, (const float*)t1_,
+                           Ty_, Tyn_, tw_.f32(X), tw_.f32(L), X, nx, ny, nz);
+      }
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  });
+  timed(K_PASSA, [&] {
+    dispatch_len(Z, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      hipLaunchKernelGGL((k_passA<L, 16>), dim3(Xh / 16, Y), dim3(NT), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
+                         (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  });
+  timed(K_PASSB, [&] {
+    dispatch_len(Y, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      hipLaunchKernelGGL((k_passB<L, 16>), dim3(Xh / 16, Z, ka), dim3(NT), 0, s_, Q_, tw_.f32(L), Z, Xh);
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  });
+  timed(K_PASSC, [&] {
+    dispatch_len(Xh, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      constexpr int NL = passc_lines<L>();
+      const long nl = (long)Y * Z;
+      hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const c32*)Q_, maxc_,
+                         argm_, tw_.f32(L), tw_.f32(X), nl, (long)Z * Y * Xh, ka, (int)first_index);
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  });
+}
+
+void Search::finalize(float* score, int32_t* angle) {
+  if (mode == MODE_FLCF) {
+    const double inv_nm = 1.0 / ((double)N * m);
+    hipLaunchKernelGGL(k_finalize_flcf, dim3(grid_for(N)), dim3(NT), 0, s_, (const float*)maxc_,
+                       (const int32_t*)argm_, (const double*)mu_, (const double*)sig_, score, angle, N,
+                       inv_nm, mu_t, sig_t);
+  } else {
+    hipLaunchKernelGGL(k_finalize_xcf, dim3(grid_for(N)), dim3(NT), 0, s_, (const float*)maxc_,
+                       (const int32_t*)argm_, score, angle, X, Y, Z, 1.0 / (double)N);
+  }
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tomhip

@@ -1,0 +1,21 @@
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_kernels.h"
+
+namespace tomhip {
+// tom_norm 'mean0+1std' of 1e5*(T+1) (FP32 output Tn32), then the fixed
+// template statistics of tom_os3_normUnderMask over the (optionally W-filtered
+// in FP64) normalised template. wsh = ifftshift(W) or null.
+void prepare_template(Twiddles& tw, hipStream_t s, const float* T, int nx, int ny, int nz, const float* M,
+                      const float* wsh, float* Tn32, double* mu_t, double* sig_t);
+// tom_os3_normUnderMask of T as given (no normalisation, no weight).
+void template_stats(hipStream_t s, const float* T, int nx, int ny, int nz, const float* M, double* mu_t,
+                    double* sig_t);
+void launch_spheremask_os3(int nx, int ny, int nz, float* out, hipStream_t s);
+void launch_spheremask(int nx, int ny, int nz, float radius, float sigma, float max_radius, float* out,
+                       hipStream_t s);
+void launch_weight(int nx, int ny, int nz, int use_wedge, double semi_angle_deg, int use_bp, double lo,
+                   double hi, int shifted, float* out, hipStream_t s);
+void global_max(const float* a, long n, float* val, long* idx, hipStream_t s);
+}  // namespace tomhip
