@@ -1,0 +1,314 @@
+#!/usr/bin/env python3
+"""Throughput of exhaustive FLCF template matching on one MI355X.
+
+Workload (default, "c2"): a 512x512x256 float32 tomogram holding 300 copies of a
+48^3 phantom (12 Gaussian blobs, sigma 2, within radius 18) at seeded positions
+(min spacing 40) and uniform random orientations, plus N(0,1) noise at SNR 0.1;
+missing wedge +-60 deg about Y (tom_wedge semi-angle 30 deg) applied at box size;
+binary spherical mask radius 22; Euler grid 15 deg: 24 x 24 x 13 = 7,488
+orientations. A step is one launch group of `--batch` orientations scanned over
+the whole tomogram; the default times every group of the full angle set.
+
+value = voxel-orientations/s = X*Y*Z * orientations scanned / timed seconds,
+with the tomogram, F_I and the FP64 local statistics already resident in HBM
+(their one-off precompute is reported separately as precompute_ms).
+
+Prints one JSON line (see the end of main()).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FLCF voxel-orientations/s; surrogate vs plain wall-time at >=99% peak recall"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: volume (X,Y,Z), template edge, blobs, sigma, radius, particles,
+    #       min spacing, SNR, wedge semi-angle, mask radius, angle step, seed
+    "c1": dict(N=(64, 64, 64), n=24, k=6, sigma=1.5, R=8, P=5, spacing=24, snr=0.5, wedge=30.0,
+               mask=("gauss", 10.0, 2.0), bp=(2.0, 20.0), step=(30, 30, 30), seed=1,
+               desc="C1 plain template matching 64^3, 24^3 template, 30 deg (1,008 orientations)"),
+    "c2": dict(N=(512, 512, 256), n=48, k=12, sigma=2.0, R=18, P=300, spacing=40, snr=0.1, wedge=30.0,
+               mask=("binary", 22.0, 0.0), bp=None, step=(15, 15, 15), seed=2,
+               desc="C2 plain exhaustive FLCF 512x512x256, 48^3 template, 15 deg (7,488 orientations)"),
+    "c5": dict(N=(1024, 1024, 384), n=48, k=12, sigma=2.0, R=18, P=2000, spacing=48, snr=0.05, wedge=40.0,
+               mask=("binary", 22.0, 0.0), bp=None, step=(10, 10, 10), seed=5,
+               desc="C5 exhaustive FLCF 1024x1024x384, 48^3 template, 10 deg (24,624 orientations)"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def angle_grid(step):
+    phi = np.arange(0, 360, step[0], dtype=np.float64)
+    psi = np.arange(0, 360, step[1], dtype=np.float64)
+    the = np.arange(0, 180 + 1e-9, step[2], dtype=np.float64)
+    return np.asarray([(a, b, c) for a in phi for b in psi for c in the], np.float32)
+
+
+def phantom(rng, n, k, sigma, R):
+    x = np.arange(n, dtype=np.float64) - n // 2
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    T = np.zeros((n, n, n))
+    for _ in range(k):
+        while True:
+            c = rng.uniform(-R, R, 3)
+            if np.linalg.norm(c) <= R:
+                break
+        T += rng.uniform(0.5, 1.0) * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * sigma ** 2))
+    return T.astype(np.float32)
+
+
+def place(rng, N, n, P, spacing):
+    pos = []
+    lo = n // 2
+    tries = 0
+    while len(pos) < P:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place particles")
+        c = np.array([rng.integers(lo, N[d] - n + lo + 1) for d in range(3)])
+        if all(np.linalg.norm(c - q) >= spacing for q in pos):
+            pos.append(c)
+    return np.asarray(pos)
+
+
+def make_tomogram(cfg, api, torch, dev):
+    """Seeded synthetic tomogram on the GPU (rotations by the library itself)."""
+    rng = np.random.default_rng(cfg["seed"])
+    N, n = cfg["N"], cfg["n"]
+    T = phantom(rng, n, cfg["k"], cfg["sigma"], cfg["R"])
+    pos = place(rng, N, n, cfg["P"], cfg["spacing"])
+    eul = np.stack([rng.uniform(0, 360, cfg["P"]), rng.uniform(0, 360, cfg["P"]),
+                    np.degrees(np.arccos(rng.uniform(-1, 1, cfg["P"])))], 1).astype(np.float32)
+    Td = torch.from_numpy(np.ascontiguousarray(T.transpose(2, 1, 0))).to(dev)
+    rot = api.tom_rotate_batch(Td, eul)  # (P, n, n, n) as (z, y, x)
+    S = torch.zeros((N[2], N[1], N[0]), dtype=torch.float32, device=dev)
+    h = n // 2
+    for p in range(cfg["P"]):
+        x, y, z = (int(v) - h for v in pos[p])
+        S[z:z + n, y:y + n, x:x + n] += rot[p]
+    S *= math.sqrt(cfg["snr"] / float(S.double().var()))
+    g = torch.Generator(device=dev)
+    g.manual_seed(cfg["seed"])
+    V = S + torch.randn(S.shape, generator=g, device=dev, dtype=torch.float32)
+    return V, T, pos, eul
+
+
+def masks(cfg, api):
+    n = cfg["n"]
+    kind, r, s = cfg["mask"]
+    M = api.tom_spheremask_gauss((n, n, n), r, s if kind == "gauss" else 0.0)
+    lo, hi = cfg["bp"] if cfg["bp"] else (0.0, 0.0)
+    W = api.weight((n, n, n), cfg["wedge"], lo, hi)
+    return M, W
+
+
+def byte_model(cfg, batch):
+    """Algorithmic HBM bytes per launch of each per-angle kernel."""
+    X, Y, Z = cfg["N"]
+    n = cfg["n"]
+    Xh = X // 2
+    c8 = 8
+    q = Z * Y * Xh * c8  # one complex half-spectrum (kx < X/2)
+    return {
+        # template y-spectra (n z-planes) + F_I once + Q written per angle
+        "passA": batch * n * Y * (Xh + 1) * c8 + Z * Y * (Xh + 1) * c8 + batch * q,
+        # y inverse in place: read + write Q per angle
+        "passB": 2 * batch * q,
+        # c2r: read Q per angle + read the resident max map once (updates not counted)
+        "passC": batch * q + X * Y * Z * 4,
+    }
+
+
+def cpu_baseline(cfg, V_host, T, M, W, angles, k, workers):
+    """The oracle's per-orientation FLCF (rotate, mask, weight, re-mask, two
+    FFTs of the full tomogram, normalise) on a bounded sample of k orientations,
+    FP64, pocketfft with `workers` threads. Returns (vox-orient/s, maps)."""
+    from oracle import tom_oracle as O
+    O.set_fft_workers(workers)
+    Vd = V_host.astype(np.float64)
+    Mx = M.astype(np.float64)
+    m = float((Mx > 0).sum())
+    Tn, mu_T, sig_T = O.prepare_template(T, Mx, W.astype(np.float64))
+    t0 = time.perf_counter()
+    mu_I, sig_I = O.local_stats(Vd, Mx, m)
+    FV = O._fftn(Vd)
+    t_pre = time.perf_counter() - t0
+    maps = []
+    t0 = time.perf_counter()
+    for a in angles[:k]:
+        maps.append(O.flcf_angle(Vd, Tn, Mx, m, mu_T, sig_T, mu_I, sig_I, FV, a, W.astype(np.float64)))
+    dt = time.perf_counter() - t0
+    N = V_host.size
+    return N * k / dt, t_pre, dt, maps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="launch groups to time (0 = the full angle set)")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-angles", type=int, default=4, help="orientations in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-workers", type=int, default=16)
+    ap.add_argument("--no-profile", action="store_true", help="do not record per-kernel events")
+    ap.add_argument("--calib-copy", action="store_true", help="run one 1 GiB device copy (PMC calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the search would shard by angle subsets with one max/argmax merge; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from tom_amd import api
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    cfg = CONFIGS[args.config]
+    X, Y, Z = cfg["N"]
+    N = X * Y * Z
+    angles = angle_grid(cfg["step"])
+    A = angles.shape[0]
+    KA = args.batch
+    groups = (A + KA - 1) // KA
+    steps = args.steps if args.steps > 0 else groups
+    steps = min(steps, groups)
+
+    log(f"[bench] {cfg['desc']}: generating synthetic tomogram")
+    ctx = api.context()
+    V, T, pos, eul = make_tomogram(cfg, api, torch, dev)
+    M, W = masks(cfg, api)
+    torch.cuda.synchronize()
+
+    if args.calib_copy:
+        a = torch.empty(2 ** 28, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    # ---- one-off precompute: F_I + FP64 local statistics
+    tn, mu_t, sig_t = api.prepare_template(T, M, W)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = api.SearchPlan(V, (cfg["n"],) * 3, M, batch=KA)
+    torch.cuda.synchronize()
+    pre_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] precompute (F_I + FP64 local stats) {pre_ms:.1f} ms")
+
+    # ---- warmup
+    plan.set_template(tn, W, mu_t, sig_t)
+    plan.run(angles[:min(args.warmup * KA, A)], 0)
+    torch.cuda.synchronize()
+
+    # ---- timed region: `steps` launch groups from the start of the angle list
+    plan.set_template(tn, W, mu_t, sig_t)  # resets the resident maxima
+    ntimed = min(steps * KA, A)
+    if not args.no_profile:
+        plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run(angles[:ntimed], 0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_k, cnt_k = plan.profile(False) if not args.no_profile else (np.zeros(8), np.zeros(8, np.int64))
+    value = N * ntimed / dt
+    log(f"[bench] {ntimed} orientations in {dt:.3f} s -> {value:.4e} voxel-orientations/s "
+        f"({dt / ntimed * 1e3:.3f} ms/orientation)")
+
+    # ---- finalize + sanity: planted particles are found
+    t0 = time.perf_counter()
+    score, ang = plan.finalize()
+    torch.cuda.synchronize()
+    fin_ms = (time.perf_counter() - t0) * 1e3
+    sc = score.cpu().numpy().transpose(2, 1, 0)
+    found = float(np.mean([sc[tuple(p)] > 0.2 for p in pos])) if ntimed == A else None
+
+    # ---- roofline of the dominant kernel (event times over the timed region)
+    names = ["rotate", "box_weight", "tmpl_fwd", "passA", "passB", "passC"]
+    per_kernel = {names[i]: {"ms_total": round(float(ms_k[i]), 3), "launches": int(cnt_k[i]),
+                             "ms_avg": round(float(ms_k[i] / cnt_k[i]), 5) if cnt_k[i] else None}
+                  for i in range(6)}
+    roof = None
+    if cnt_k[3:6].sum() > 0:
+        model = byte_model(cfg, KA)
+        dom = max(["passA", "passB", "passC"], key=lambda k: per_kernel[k]["ms_total"])
+        i = names.index(dom)
+        full = (ntimed // KA) * KA == ntimed
+        avg_s = ms_k[i] / cnt_k[i] / 1e3
+        ach = model[dom] / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(model[dom]), "avg_launch_ms": round(avg_s * 1e3, 4),
+                "batch_full": full}
+
+    # ---- GPU error against FP64 on the CPU sample + CPU baseline
+    cpu = None
+    err = None
+    if args.cpu_angles > 0:
+        k = args.cpu_angles
+        # spread the sample over the grid
+        sample = angles[np.linspace(0, A - 1, k).round().astype(int)]
+        log(f"[bench] CPU baseline: oracle FP64 on {k} orientations, {args.cpu_workers} threads")
+        Vh = V.cpu().numpy().transpose(2, 1, 0)
+        rate, t_pre_cpu, t_ang, maps = cpu_baseline(cfg, Vh, T, M, W, sample, k, args.cpu_workers)
+        cpu = {"value": rate, "unit": "voxel-orientations/s", "cores": args.cpu_workers, "kind": "port",
+               "sample": f"{k} orientations of the {A} ({cfg['desc']}), full tomogram each, FP64 oracle "
+                         f"(pocketfft, {args.cpu_workers} threads): {t_ang:.1f} s; one-off local stats + F_I "
+                         f"{t_pre_cpu:.1f} s not included"}
+        # single-orientation GPU maps for the same sample
+        errs = []
+        for j, a in enumerate(sample):
+            plan.set_template(tn, W, mu_t, sig_t)
+            plan.run(a[None], 0)
+            s1, _ = plan.finalize()
+            s1 = s1.cpu().numpy().transpose(2, 1, 0)
+            errs.append(float(np.abs(s1 - maps[j]).max()))
+        err = max(errs)
+        log(f"[bench] max |score(GPU FP32) - score(FP64 oracle)| over the sample: {err:.3e}")
+
+    plan.close()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-orientations/s",
+        "n_gpus": 1,
+        "steps": int(ntimed // KA + (ntimed % KA > 0)),
+        "warmup": args.warmup,
+        "ms_per_step": dt / max(1, math.ceil(ntimed / KA)) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["desc"], "volume": list(cfg["N"]), "template": cfg["n"],
+                   "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA,
+                   "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
+                   "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

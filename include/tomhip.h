@@ -104,7 +104,8 @@ int tomhip_search_run(tomhip_search* s, const float* euler, int64_t count, int64
 /* FLCF: score = (c/m - mu_I mu_T)/(sig_T sig_I) of the best angle (0 where
  * sig_I = 0); XCF: tom_oscar's ccfmax in its orientation. angle: 0-based. */
 int tomhip_search_finalize(tomhip_search* s, float* score, int32_t* angle);
-/* Per-kernel timing (HIP events; serialises launches while enabled).
+/* Per-kernel timing: HIP event pairs recorded around every launch on the
+ * search stream (no extra synchronisation); this call resolves them.
  * ms/counts: 8 entries (rotate, box weight, template fwd, pass A, pass B,
  * pass C, -, -). */
 int tomhip_search_profile(tomhip_search* s, int enable, double* ms, int64_t* counts);

@@ -59,6 +59,29 @@ _LIB = None
 
 VAR_FLOOR_REL = 1e-12
 
+# FFT backend: numpy's pocketfft; set_fft_workers(k) switches to scipy.fft with
+# k threads (same algorithm family, used to time the CPU baseline).
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(k: int) -> None:
+    global _FFT_WORKERS
+    _FFT_WORKERS = max(1, int(k))
+
+
+def _fftn(a):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fftn(a, workers=_FFT_WORKERS)
+    return np.fft.fftn(a)
+
+
+def _ifftn(a):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifftn(a, workers=_FFT_WORKERS)
+    return np.fft.ifftn(a)
+
 
 def lib():
     """Load (building if needed) the C part of the oracle."""
@@ -253,9 +276,9 @@ def local_stats(V: np.ndarray, M: np.ndarray, m: float) -> Tuple[np.ndarray, np.
     g = V.mean()
     Vs = V - g
     s = float(M.sum()) / m
-    FM = np.fft.fftn(paste_center(V.shape, M))
-    mu_s = np.real(np.fft.ifftshift(np.fft.ifftn(np.fft.fftn(Vs) * FM))) / m
-    e2 = np.real(np.fft.ifftshift(np.fft.ifftn(np.fft.fftn(Vs * Vs) * FM))) / m
+    FM = _fftn(paste_center(V.shape, M))
+    mu_s = np.real(np.fft.ifftshift(_ifftn(_fftn(Vs) * FM))) / m
+    e2 = np.real(np.fft.ifftshift(_ifftn(_fftn(Vs * Vs) * FM))) / m
     var = (e2 - mu_s * mu_s) + (1.0 - s) * (2.0 * g * mu_s + g * g * s)
     floor = VAR_FLOOR_REL * float((Vs * Vs).mean())
     var = np.where(var > floor, var, 0.0)
@@ -265,9 +288,9 @@ def local_stats(V: np.ndarray, M: np.ndarray, m: float) -> Tuple[np.ndarray, np.
 def correlation_raw(V: np.ndarray, tm: np.ndarray, FV: Optional[np.ndarray] = None) -> np.ndarray:
     """real(ifftshift(ifftn(fftn(V) .* conj(fftn(pasteCenter(zeros, tm))))))."""
     if FV is None:
-        FV = np.fft.fftn(np.asarray(V, np.float64))
-    FT = np.fft.fftn(paste_center(V.shape, np.asarray(tm, np.float64)))
-    return np.real(np.fft.ifftshift(np.fft.ifftn(FV * np.conj(FT))))
+        FV = _fftn(np.asarray(V, np.float64))
+    FT = _fftn(paste_center(V.shape, np.asarray(tm, np.float64)))
+    return np.real(np.fft.ifftshift(_ifftn(FV * np.conj(FT))))
 
 
 def flcf(V, tm, m, mu_T, sig_T, mu_I, sig_I, FV=None) -> np.ndarray:
@@ -289,6 +312,15 @@ def prepare_template(T, M, W=None):
     return Tn, mu_T, sig_T
 
 
+def flcf_angle(V, Tn, M, m, mu_T, sig_T, mu_I, sig_I, FV, angle, W=None):
+    """One orientation of the findTemplate loop (tom_os3_findTemplate.m:264-270):
+    rotate, mask, weight, re-mask, FLCF score map."""
+    t = rotate(np.asarray(Tn, np.float32), angle).astype(np.float64) * M
+    if W is not None:
+        t = apply_weight(t, W)
+    return flcf(V, t * M, m, mu_T, sig_T, mu_I, sig_I, FV)
+
+
 def find_template(V, T, M, angles, W=None, return_all=False):
     """ccc-only tom_os3_findTemplate: returns (best score, 1-based best angle).
 
@@ -300,15 +332,12 @@ def find_template(V, T, M, angles, W=None, return_all=False):
     m = float((M > 0).sum())
     Tn, mu_T, sig_T = prepare_template(T, M, W)
     mu_I, sig_I = local_stats(V, M, m)
-    FV = np.fft.fftn(V)
+    FV = _fftn(V)
     ccc = np.full(V.shape, -1000.0)
     ang = np.full(V.shape, -1000.0)
     allmaps = []
     for i, a in enumerate(np.asarray(angles).reshape(-1, 3)):
-        t = rotate(Tn.astype(np.float32), a).astype(np.float64) * M
-        if W is not None:
-            t = apply_weight(t, W)
-        s = flcf(V, t * M, m, mu_T, sig_T, mu_I, sig_I, FV)
+        s = flcf_angle(V, Tn, M, m, mu_T, sig_T, mu_I, sig_I, FV, a, W)
         if return_all:
             allmaps.append(s)
         upd = ccc < s

@@ -3,11 +3,4 @@ matching / subtomogram scoring hot path, behind a C ABI (include/tomhip.h)
 and a MATLAB-shaped Python interface (tom_amd.api)."""
 from ._lib import LIB_PATH, TomHipError, load  # noqa: F401
 
-__all__ = ["LIB_PATH", "TomHipError", "load", "api"]
-
-
-def __getattr__(name):
-    if name == "api":
-        from . import api
-        return api
-    raise AttributeError(name)
+__all__ = ["LIB_PATH", "TomHipError", "load"]

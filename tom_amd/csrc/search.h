@@ -44,7 +44,9 @@ class Search {
   void finalize(float* score, int32_t* angle);
 
   void set_profile(bool on) { profile_ = on; }
-  const KernelTimes& times() const { return times_; }
+  // Resolves the recorded event pairs (synchronises the stream) and returns
+  // the accumulated per-kernel times.
+  const KernelTimes& times();
   void reset_times() { times_ = KernelTimes(); }
   int batch() const { return KA_; }
 
@@ -65,7 +67,12 @@ class Search {
   int KA_;
   bool profile_ = false;
   KernelTimes times_;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  struct EvRec {
+    int kid;
+    hipEvent_t a, b;
+  };
+  std::vector<EvRec> evpool_;  // recorded around launches, resolved lazily
+  size_t evused_ = 0;
 
   // device buffers
   const float* mask_ = nullptr;

@@ -496,8 +496,6 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     m = (double)cnt;
     mask_sum_ = msum;
   }
-  TOMHIP_CHECK(hipEventCreate(&ev0_));
-  TOMHIP_CHECK(hipEventCreate(&ev1_));
   precompute_stats(volume);
 }
 
@@ -517,8 +515,10 @@ Search::~Search() {
   hipFree(wsh_);
   hipFree(rm_);
   hipFree(const_cast<float*>(mask_));
-  if (ev0_) hipEventDestroy(ev0_);
-  if (ev1_) hipEventDestroy(ev1_);
+  for (auto& e : evpool_) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
 }
 
 void Search::precompute_stats(const float* volume) {
@@ -596,14 +596,31 @@ void Search::timed(int kid, F&& f) {
     f();
     return;
   }
-  TOMHIP_CHECK(hipEventRecord(ev0_, s_));
+  if (evused_ == evpool_.size()) {
+    EvRec r;
+    TOMHIP_CHECK(hipEventCreate(&r.a));
+    TOMHIP_CHECK(hipEventCreate(&r.b));
+    evpool_.push_back(r);
+  }
+  EvRec& r = evpool_[evused_++];
+  r.kid = kid;
+  TOMHIP_CHECK(hipEventRecord(r.a, s_));
   f();
-  TOMHIP_CHECK(hipEventRecord(ev1_, s_));
-  TOMHIP_CHECK(hipEventSynchronize(ev1_));
-  float ms = 0;
-  TOMHIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  times_.ms[kid] += ms;
-  times_.n[kid] += 1;
+  TOMHIP_CHECK(hipEventRecord(r.b, s_));
+}
+
+const KernelTimes& Search::times() {
+  if (evused_) {
+    TOMHIP_CHECK(hipStreamSynchronize(s_));
+    for (size_t i = 0; i < evused_; ++i) {
+      float ms = 0;
+      TOMHIP_CHECK(hipEventElapsedTime(&ms, evpool_[i].a, evpool_[i].b));
+      times_.ms[evpool_[i].kid] += ms;
+      times_.n[evpool_[i].kid] += 1;
+    }
+    evused_ = 0;
+  }
+  return times_;
 }
 
 void Search::run(const float* rm_host, long count, long first_index) {
