@@ -181,7 +181,7 @@ int tomhip_rotate3d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t n
                   c[2], 0, s);
     out.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
-    hipFree(drm);
+    (void)hipFree(drm);
   });
 }
 
@@ -206,7 +206,7 @@ int tomhip_rotate3d_batch_f32(tomhip_ctx* ctx, const float* src, float* dst, int
                   (int)ny, (int)nz, c[0], c[1], c[2], n3, s);
     out.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
-    hipFree(drm);
+    (void)hipFree(drm);
   });
 }
 
@@ -318,10 +318,10 @@ int tomhip_prepare_template_f32(tomhip_ctx* ctx, const float* tmpl, const int64_
       prepare_template(*ctx->tw, s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], m.get<float>(), wsh,
                        o.get<float>(), mu_t, sig_t);
     } catch (...) {
-      hipFree(wsh);
+      (void)hipFree(wsh);
       throw;
     }
-    hipFree(wsh);
+    (void)hipFree(wsh);
     o.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
   });
@@ -363,10 +363,10 @@ int tomhip_search_set_template(tomhip_search* h, const float* tmpl, const float*
       S.set_template(t.get<float>(), wsh, mu_t, sig_t, remask != 0);
       TOMHIP_CHECK(hipStreamSynchronize(s));
     } catch (...) {
-      hipFree(wsh);
+      (void)hipFree(wsh);
       throw;
     }
-    hipFree(wsh);
+    (void)hipFree(wsh);
   });
 }
 
@@ -436,9 +436,9 @@ int tomhip_flcf_search_f32(tomhip_ctx* ctx, const float* volume, const int64_t N
     float* tn = nullptr;
     float* wsh = nullptr;
     auto cleanup = [&] {
-      hipFree(own_mask);
-      hipFree(tn);
-      hipFree(wsh);
+      (void)hipFree(own_mask);
+      (void)hipFree(tn);
+      (void)hipFree(wsh);
     };
     try {
       const float* M = mask ? mk_.get<float>() : nullptr;
@@ -484,8 +484,8 @@ int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], c
     float* own_mask = nullptr;
     int32_t* ang = nullptr;
     auto cleanup = [&] {
-      hipFree(own_mask);
-      hipFree(ang);
+      (void)hipFree(own_mask);
+      (void)hipFree(ang);
     };
     try {
       const float* M = mask ? mk_.get<float>() : nullptr;

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "fft_io.h"
 #include "rotate.h"
 #include "search.h"
 
@@ -231,101 +232,92 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c3
 }
 
 // ---------------------------------------------------------------- pass A
-// grid (Xh/TW, Y). For each angle of the batch: z forward FFT of the template
-// lines, product with F_I, Nyquist packing, z inverse FFT -> Q.
+// grid (Xh/TW, Y), 512 threads. For each angle of the batch: z forward FFT of
+// the template lines (stage 0 reads the n non-zero planes straight from Ty),
+// product with F_I in the last forward stage, Nyquist packing, z inverse FFT
+// whose last stage writes Q directly. F_I stays in LDS across the batch when
+// it fits, else it is re-read (L2) per angle.
+constexpr int NT_A = 512;
 template <int Z, int TW>
-__global__ __launch_bounds__(NT) void k_passA(const c32* Ty, const c32* Tyn, const c32* FI, const c32* FIn,
-                                              c32* Q, const c32* twZg, int Y, int Xh, int nz, int ka) {
+constexpr bool passA_fi_lds() {
+  return (2 * Z * (TW + 1) + Z) * 8 <= 150 * 1024;
+}
+
+template <int Z, int TW, bool NYQ>
+__global__ __launch_bounds__(NT_A) void k_passA(const c32* Ty, const c32* Tyn, const c32* FI, const c32* FIn,
+                                                c32* Q, const c32* twZg, int Y, int Xh, int nz, int ka) {
+  // NYQ: the kx tile 0 launch, which also carries the kx = X/2 line (line TW)
   constexpr int P = TW + 1;
+  constexpr int NL = NYQ ? P : TW;
   constexpr int E = Z * P;
-  constexpr int PER = (E + NT - 1) / NT;
+  constexpr int S = plan_nstages(Z);
+  constexpr bool FL = passA_fi_lds<Z, TW>();
+  using Lay = Inter<P>;
   __shared__ c32 buf[E];
+  __shared__ c32 fis[FL ? E : 1];
   __shared__ c32 twZ[Z];
-  const int tile = blockIdx.x, ky = blockIdx.y;
-  const bool nyq = (tile == 0);
-  load_table<float, Z, NT>(twZ, twZg);
-  c32 fi[PER];
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int i = threadIdx.x + p * NT;
-    c32 v = mk(0.f, 0.f);
-    if (i < E) {
-      const int kz = i / P, c = i % P;
-      if (c < TW)
-        v = FI[((long)kz * Y + ky) * Xh + tile * TW + c];
-      else if (nyq)
-        v = FIn[(long)ky * Z + kz];
+  const int tile = NYQ ? 0 : blockIdx.x + 1, ky = blockIdx.y;
+  load_table<float, Z, NT_A>(twZ, twZg);
+  auto fi_at = [&](int c, int kz) -> c32 {
+    if (!NYQ || c < TW) return FI[((long)kz * Y + ky) * Xh + tile * TW + c];
+    return FIn[(long)ky * Z + kz];
+  };
+  if constexpr (FL) {
+    for (int i = threadIdx.x; i < Z * NL; i += NT_A) {
+      const int kz = i / NL, c = i % NL;
+      fis[kz * P + c] = fi_at(c, kz);
     }
-    fi[p] = v;
   }
+  __syncthreads();
   const int hz = nz / 2;
   const int sZ = 2 * (Z / 2) - Z;
   for (int a = 0; a < ka; ++a) {
-    const c32* ty = Ty + (long)a * nz * Y * Xh;
-    const c32* tyn = Tyn + (long)a * nz * Y;
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int i = threadIdx.x + p * NT;
-      if (i < E) {
-        const int r = i / P, c = i % P;
-        int uz = (r + hz - sZ) % Z;
-        if (uz < 0) uz += Z;
-        c32 v = mk(0.f, 0.f);
-        if (uz < nz) {
-          if (c < TW)
-            v = ty[((long)uz * Y + ky) * Xh + tile * TW + c];
-          else if (nyq)
-            v = tyn[(long)uz * Y + ky];
-        }
-        buf[i] = v;
-      }
-    }
-    __syncthreads();
-    if (nyq)
-      lds_fft<float, Z, P, NT, -1, Inter<P>>(buf, twZ);
-    else
-      lds_fft<float, Z, TW, NT, -1, Inter<P>>(buf, twZ);
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int i = threadIdx.x + p * NT;
-      if (i < E) buf[i] = cmulc(fi[p], buf[i]);
-    }
-    __syncthreads();
-    if (nyq) {
-      for (int kz = threadIdx.x; kz < Z; kz += NT) {
+    const c32* ty = Ty + (long)a * nz * Y * Xh + (long)ky * Xh + tile * TW;
+    const c32* tyn = Tyn + (long)a * nz * Y + ky;
+    auto ld = [&](int c, int z) -> c32 {
+      int uz = z + hz - sZ;
+      if (uz >= Z) uz -= Z;
+      if (uz >= nz) return mk(0.f, 0.f);
+      return (!NYQ || c < TW) ? ty[(long)uz * Y * Xh + c] : tyn[(long)uz * Y];
+    };
+    auto prod = [&](int c, int kz, c32 v) -> c32 {
+      if constexpr (FL)
+        return cmulc(fis[kz * P + c], v);
+      else
+        return cmulc(fi_at(c, kz), v);
+    };
+    stage0_g2l<float, Z, NL, NT_A, -1, Lay>(buf, ld);
+    StagesL2L<float, Z, 1, S - 1, NL, NT_A, -1, Lay>::run(buf, twZ);
+    stage_l2l<float, Z, S - 1, NL, NT_A, -1, Lay>(buf, twZ, prod);
+    if constexpr (NYQ) {
+      for (int kz = threadIdx.x; kz < Z; kz += NT_A) {
         const c32 p0 = buf[kz * P], pn = buf[kz * P + TW];
         buf[kz * P] = mk(p0.x - pn.y, p0.y + pn.x);
       }
       __syncthreads();
     }
-    lds_fft<float, Z, TW, NT, 1, Inter<P>>(buf, twZ);
     c32* q = Q + (long)a * Z * Y * Xh + (long)ky * Xh + tile * TW;
-    for (int i = threadIdx.x; i < Z * TW; i += NT) {
-      const int z = i / TW, c = i % TW;
-      q[(long)z * Y * Xh + c] = buf[z * P + c];
-    }
+    StagesL2L<float, Z, 0, S - 1, TW, NT_A, 1, Lay>::run(buf, twZ);
+    stageN_l2g<float, Z, TW, NT_A, 1, Lay>(buf, twZ, [&](int, int, int c, int z, c32 v) {
+      q[(long)z * Y * Xh + c] = v;
+    });
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------- pass B
-// grid (Xh/TW, Z, ka): y inverse FFT in place.
+// grid (Xh/TW, Z, ka), 512 threads: y inverse FFT in place; stage 0 loads and
+// the last stage stores global memory directly.
+constexpr int NT_B = 512;
 template <int Y, int TW>
-__global__ __launch_bounds__(NT) void k_passB(c32* Q, const c32* twYg, int Z, int Xh) {
+__global__ __launch_bounds__(NT_B) void k_passB(c32* Q, const c32* twYg, int Z, int Xh) {
   __shared__ c32 buf[Y * TW];
   __shared__ c32 twY[Y];
-  load_table<float, Y, NT>(twY, twYg);
+  load_table<float, Y, NT_B>(twY, twYg);
   c32* base = Q + ((long)blockIdx.z * Z + blockIdx.y) * Y * Xh + blockIdx.x * TW;
-  for (int i = threadIdx.x; i < Y * TW; i += NT) {
-    const int e = i / TW, c = i % TW;
-    buf[i] = base[(long)e * Xh + c];
-  }
-  __syncthreads();
-  lds_fft<float, Y, TW, NT, 1, Inter<TW>>(buf, twY);
-  for (int i = threadIdx.x; i < Y * TW; i += NT) {
-    const int e = i / TW, c = i % TW;
-    base[(long)e * Xh + c] = buf[i];
-  }
+  fft_g2g<float, Y, TW, NT_B, 1, Inter<TW>>(
+      buf, twY, [&](int c, int e) { return base[(long)e * Xh + c]; },
+      [&](int, int, int c, int e, c32 v) { base[(long)e * Xh + c] = v; });
 }
 
 // ---------------------------------------------------------------- pass C
@@ -333,73 +325,139 @@ template <int XH>
 constexpr int passc_lines() {
   return (4096 / XH) > 0 ? 4096 / XH : 1;
 }
+constexpr int NT_C = 512;
 
-// grid (ceil(Y*Z / NL)). Half-length c2r along x of every angle of the batch,
-// running max/argmax in registers, one update of the resident maps.
+// grid (ceil(Y*Z / NL)). Half-length c2r along x of every angle of the batch.
+// Stage 0 reads each butterfly j together with its mirror NJ0 - j, so the
+// real-to-complex pre-twiddle Z'[k] = A + iB, Z'[XH-k] = conj(A) + i conj(B),
+//   A = G[k] + conj(G[XH-k]),  B = (G[k] - conj(G[XH-k])) e^{2πik/X},
+// (G[0], G[X/2] packed in column 0) happens in registers. The last stage feeds
+// a per-thread running max/argmax over the batch; one compare/update of the
+// resident maps follows.
 template <int XH>
-__global__ __launch_bounds__(NT) void k_passC(const c32* Q, float* maxc, int32_t* argm, const c32* twHg,
-                                              const c32* twXg, long nlines, long qstride, int ka,
-                                              int first_index) {
+__global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void k_passC(const c32* Q, float* maxc, int32_t* argm, const c32* twHg,
+                                                const c32* twXg, long nlines, long qstride, int ka,
+                                                int first_index) {
   constexpr int NL = passc_lines<XH>();
-  constexpr int E = NL * XH;
-  constexpr int PER = (E + NT - 1) / NT;
-  constexpr int NP = XH / 2 + 1;
-  __shared__ c32 buf[E];
+  constexpr int S = plan_nstages(XH);
+  constexpr int R0 = plan_radix(XH, 0);
+  constexpr int NJ0 = XH / R0;
+  constexpr int NTASK = NJ0 / 2 + 1;
+  constexpr int TT = NL * NTASK;
+  constexpr int PERT = (TT + NT_C - 1) / NT_C;
+  static_assert(NJ0 % 2 == 0 && R0 % 2 == 0, "pass C pairing needs even NJ0 and R0");
+  using Lay = Contig<XH>;
+  using IN = StageInfo<XH, S - 1, NL, NT_C, Lay>;
+  __shared__ c32 buf[NL * XH];
   __shared__ c32 twH[XH];
   __shared__ c32 twR[XH];
-  load_table<float, XH, NT>(twH, twHg);
-  load_table<float, XH, NT>(twR, twXg);  // first XH entries of the length-2XH table
+  load_table<float, XH, NT_C>(twH, twHg);
+  load_table<float, XH, NT_C>(twR, twXg);  // first XH entries of the length-2XH table
+  __syncthreads();
   const long l0 = (long)blockIdx.x * NL;
-  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * XH);
-  float bx[PER], by[PER];
-  int ix[PER], iy[PER];
+  const int nlv = (int)((nlines - l0 < NL) ? (nlines - l0) : NL);
+  static_assert(IN::R <= 8, "batch indices are packed 4 bits per output");
+  float bx[IN::PER][IN::R], by[IN::PER][IN::R];
+  uint32_t ix[IN::PER], iy[IN::PER];  // 4-bit batch index per output r
   for (int a = 0; a < ka; ++a) {
     const c32* src = Q + (long)a * qstride + l0 * XH;
-    for (int i = threadIdx.x; i < E; i += NT) buf[i] = (i < nval) ? src[i] : mk(0.f, 0.f);
-    __syncthreads();
-    for (int i = threadIdx.x; i < NL * NP; i += NT) {
-      const int l = i / NP, k = i % NP;
-      c32* row = buf + l * XH;
-      if (k == 0) {
-        const c32 g = row[0];  // (G[0], G[X/2]), both real
-        row[0] = mk(g.x + g.y, g.x - g.y);
-      } else if (2 * k == XH) {
-        const c32 g = row[k];
-        row[k] = mk(2.f * g.x, -2.f * g.y);
-      } else if (2 * k < XH) {
-        const c32 gk = row[k], gm = row[XH - k];
-        const c32 A = mk(gk.x + gm.x, gk.y - gm.y);
-        const c32 B = cmul(mk(gk.x - gm.x, gk.y + gm.y), twR[k]);
-        row[k] = mk(A.x - B.y, A.y + B.x);
-        row[XH - k] = mk(A.x + B.y, B.x - A.y);
-      }
-    }
-    __syncthreads();
-    lds_fft<float, XH, NL, NT, 1, Contig<XH>>(buf, twH);
+#pragma unroll 1
+    for (int p = 0; p < PERT; ++p) {
+      const int t = threadIdx.x + p * NT_C;
+      if ((TT % NT_C == 0) || t < TT) {
+        const int line = t / NTASK, j = t % NTASK;
+        const bool ok = line < nlv;
+        const c32* g = src + (long)line * XH;
+        c32* row = buf + line * XH;
+        c32 va[R0], vb[R0], z1[R0], z2[R0];
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int e = threadIdx.x + p * NT;
-      if (e < E) {
-        const c32 z = buf[e];
-        if (a == 0 || z.x > bx[p]) { bx[p] = z.x; ix[p] = a; }
-        if (a == 0 || z.y > by[p]) { by[p] = z.y; iy[p] = a; }
+        for (int r = 0; r < R0; ++r) va[r] = ok ? g[j + r * NJ0] : mk(0.f, 0.f);
+        if (j == 0) {
+          z1[0] = mk(va[0].x + va[0].y, va[0].x - va[0].y);
+#pragma unroll
+          for (int r = 1; r < R0 / 2; ++r) {
+            const c32 gk = va[r], gm = va[R0 - r];
+            const c32 A = mk(gk.x + gm.x, gk.y - gm.y);
+            const c32 B = cmul(mk(gk.x - gm.x, gk.y + gm.y), twR[r * NJ0]);
+            z1[r] = mk(A.x - B.y, A.y + B.x);
+            z1[R0 - r] = mk(A.x + B.y, B.x - A.y);
+          }
+          z1[R0 / 2] = mk(2.f * va[R0 / 2].x, -2.f * va[R0 / 2].y);
+          Bfly<float, R0, 1>::run(z1);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) row[r] = z1[r];
+        } else if (j == NJ0 / 2) {
+#pragma unroll
+          for (int r = 0; r < R0 / 2; ++r) {
+            const c32 gk = va[r], gm = va[R0 - 1 - r];
+            const c32 A = mk(gk.x + gm.x, gk.y - gm.y);
+            const c32 B = cmul(mk(gk.x - gm.x, gk.y + gm.y), twR[j + r * NJ0]);
+            z1[r] = mk(A.x - B.y, A.y + B.x);
+            z1[R0 - 1 - r] = mk(A.x + B.y, B.x - A.y);
+          }
+          Bfly<float, R0, 1>::run(z1);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) row[j * R0 + r] = z1[r];
+        } else {
+          const int j2 = NJ0 - j;
+#pragma unroll
+          for (int r = 0; r < R0; ++r) vb[r] = ok ? g[j2 + r * NJ0] : mk(0.f, 0.f);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) {
+            const c32 gk = va[r], gm = vb[R0 - 1 - r];
+            const c32 A = mk(gk.x + gm.x, gk.y - gm.y);
+            const c32 B = cmul(mk(gk.x - gm.x, gk.y + gm.y), twR[j + r * NJ0]);
+            z1[r] = mk(A.x - B.y, A.y + B.x);
+            z2[R0 - 1 - r] = mk(A.x + B.y, B.x - A.y);
+          }
+          Bfly<float, R0, 1>::run(z1);
+          Bfly<float, R0, 1>::run(z2);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) {
+            row[j * R0 + r] = z1[r];
+            row[j2 * R0 + r] = z2[r];
+          }
+        }
       }
     }
+    __syncthreads();
+    StagesL2L<float, XH, 1, S - 1, NL, NT_C, 1, Lay>::run(buf, twH);
+    if (a == 0) {
+#pragma unroll
+      for (int p = 0; p < IN::PER; ++p) ix[p] = iy[p] = 0u;
+    }
+    stageN_l2g<float, XH, NL, NT_C, 1, Lay>(buf, twH, [&](int p, int r, int, int, c32 v) {
+      const uint32_t sh = 4u * r, msk = ~(15u << sh), av = (uint32_t)a << sh;
+      if (a == 0 || v.x > bx[p][r]) {
+        bx[p][r] = v.x;
+        ix[p] = (ix[p] & msk) | av;
+      }
+      if (a == 0 || v.y > by[p][r]) {
+        by[p][r] = v.y;
+        iy[p] = (iy[p] & msk) | av;
+      }
+    });
     __syncthreads();
   }
 #pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = threadIdx.x + p * NT;
-    if (e < nval) {
-      const long v = (l0 * XH + e) * 2;
-      const float2 cur = *reinterpret_cast<const float2*>(maxc + v);
-      if (bx[p] > cur.x) {
-        maxc[v] = bx[p];
-        argm[v] = first_index + ix[p];
-      }
-      if (by[p] > cur.y) {
-        maxc[v + 1] = by[p];
-        argm[v + 1] = first_index + iy[p];
+  for (int p = 0; p < IN::PER; ++p) {
+    if (IN::valid(p)) {
+      int line, j;
+      IN::idx(p, line, j);
+      if (line < nlv) {
+#pragma unroll
+        for (int r = 0; r < IN::R; ++r) {
+          const long v = ((l0 + line) * XH + j + r * IN::NS) * 2;
+          const float2 cur = *reinterpret_cast<const float2*>(maxc + v);
+          if (bx[p][r] > cur.x) {
+            maxc[v] = bx[p][r];
+            argm[v] = first_index + (int)((ix[p] >> (4 * r)) & 15u);
+          }
+          if (by[p][r] > cur.y) {
+            maxc[v + 1] = by[p][r];
+            argm[v + 1] = first_index + (int)((iy[p] >> (4 * r)) & 15u);
+          }
+        }
       }
     }
   }
@@ -433,6 +491,30 @@ __global__ void k_finalize_xcf(const float* maxc, const int32_t* argm, float* sc
     const long j = ((long)((Z - z) % Z) * Y + (Y - y) % Y) * X + (X - x) % X;
     score[i] = (float)((double)maxc[j] * inv_n);
     angle[i] = argm[j];
+  }
+}
+
+// Half-lengths X/2 the c2r pass supports (X a multiple of 32).
+template <class F>
+void dispatch_xh(int L, F&& f) {
+  switch (L) {
+#define TOMHIP_XH(n) \
+  case n:            \
+    f(std::integral_constant<int, n>{}); \
+    break;
+    TOMHIP_XH(16)
+    TOMHIP_XH(32)
+    TOMHIP_XH(48)
+    TOMHIP_XH(64)
+    TOMHIP_XH(96)
+    TOMHIP_XH(128)
+    TOMHIP_XH(192)
+    TOMHIP_XH(256)
+    TOMHIP_XH(384)
+    TOMHIP_XH(512)
+#undef TOMHIP_XH
+    default:
+      throw Error(ERR_UNSUPPORTED, "unsupported X/2 = " + std::to_string(L));
   }
 }
 
@@ -500,24 +582,24 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
 }
 
 Search::~Search() {
-  hipFree(FI_);
-  hipFree(FIn_);
-  hipFree(maxc_);
-  hipFree(argm_);
-  hipFree(tmpl_);
-  hipFree(t1_);
-  hipFree(tbox_);
-  hipFree(Ty_);
-  hipFree(Tyn_);
-  hipFree(Q_);
-  hipFree(mu_);
-  hipFree(sig_);
-  hipFree(wsh_);
-  hipFree(rm_);
-  hipFree(const_cast<float*>(mask_));
+  (void)hipFree(FI_);
+  (void)hipFree(FIn_);
+  (void)hipFree(maxc_);
+  (void)hipFree(argm_);
+  (void)hipFree(tmpl_);
+  (void)hipFree(t1_);
+  (void)hipFree(tbox_);
+  (void)hipFree(Ty_);
+  (void)hipFree(Tyn_);
+  (void)hipFree(Q_);
+  (void)hipFree(mu_);
+  (void)hipFree(sig_);
+  (void)hipFree(wsh_);
+  (void)hipFree(rm_);
+  (void)hipFree(const_cast<float*>(mask_));
   for (auto& e : evpool_) {
-    hipEventDestroy(e.a);
-    hipEventDestroy(e.b);
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
   }
 }
 
@@ -539,7 +621,7 @@ void Search::precompute_stats(const float* volume) {
   double ss = 0;
   for (double v : hp) ss += v;
   const double varg = ss / (double)N;
-  hipFree(part);
+  (void)hipFree(part);
 
   c64* A = nullptr;
   TOMHIP_CHECK(hipMalloc(&A, sizeof(c64) * N));
@@ -563,10 +645,10 @@ void Search::precompute_stats(const float* volume) {
                        1e-12 * varg, mu_, sig_);
     TOMHIP_CHECK(hipGetLastError());
     TOMHIP_CHECK(hipStreamSynchronize(s_));
-    hipFree(B);
+    (void)hipFree(B);
   }
   TOMHIP_CHECK(hipStreamSynchronize(s_));
-  hipFree(A);
+  (void)hipFree(A);
 }
 
 void Search::set_template(const float* tmpl, const float* wsh, double mu_t_, double sig_t_, bool remask) {
@@ -580,7 +662,7 @@ void Search::set_template(const float* tmpl, const float* wsh, double mu_t_, dou
     if (!wsh_) TOMHIP_CHECK(hipMalloc(&wsh_, sizeof(float) * n3));
     TOMHIP_CHECK(hipMemcpyAsync(wsh_, wsh, sizeof(float) * n3, hipMemcpyDefault, s_));
   } else if (wsh_) {
-    hipFree(wsh_);
+    (void)hipFree(wsh_);
     wsh_ = nullptr;
   }
   mu_t = mu_t_;
@@ -628,7 +710,7 @@ void Search::run(const float* rm_host, long count, long first_index) {
   if (count <= 0) return;
   TOMHIP_REQUIRE(first_index >= 0 && first_index + count <= INT32_MAX, ERR_WRONG_INPUT, "angle index range");
   if (count > rm_cap_) {
-    hipFree(rm_);
+    (void)hipFree(rm_);
     TOMHIP_CHECK(hipMalloc(&rm_, sizeof(float) * 9 * count));
     rm_cap_ = count;
   }
@@ -707,24 +789,27 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index) {
   timed(K_PASSA, [&] {
     dispatch_len(Z, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
-      hipLaunchKernelGGL((k_passA<L, 16>), dim3(Xh / 16, Y), dim3(NT), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
+      hipLaunchKernelGGL((k_passA<L, 16, true>), dim3(1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
                          (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+      if (Xh > 16)
+        hipLaunchKernelGGL((k_passA<L, 16, false>), dim3(Xh / 16 - 1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_,
+                           (const c32*)Tyn_, (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
     });
     TOMHIP_CHECK(hipGetLastError());
   });
   timed(K_PASSB, [&] {
     dispatch_len(Y, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
-      hipLaunchKernelGGL((k_passB<L, 16>), dim3(Xh / 16, Z, ka), dim3(NT), 0, s_, Q_, tw_.f32(L), Z, Xh);
+      hipLaunchKernelGGL((k_passB<L, 16>), dim3(Xh / 16, Z, ka), dim3(NT_B), 0, s_, Q_, tw_.f32(L), Z, Xh);
     });
     TOMHIP_CHECK(hipGetLastError());
   });
   timed(K_PASSC, [&] {
-    dispatch_len(Xh, [&](auto Lc) {
+    dispatch_xh(Xh, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
       constexpr int NL = passc_lines<L>();
       const long nl = (long)Y * Z;
-      hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const c32*)Q_, maxc_,
+      hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT_C), 0, s_, (const c32*)Q_, maxc_,
                          argm_, tw_.f32(L), tw_.f32(X), nl, (long)Z * Y * Xh, ka, (int)first_index);
     });
     TOMHIP_CHECK(hipGetLastError());

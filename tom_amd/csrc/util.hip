@@ -234,9 +234,9 @@ void prepare_template(Twiddles& tw, hipStream_t s, const float* T, int nx, int n
   TOMHIP_CHECK(hipStreamSynchronize(s));
   *mu_t = h[0];
   *sig_t = h[1];
-  hipFree(t64);
-  hipFree(c);
-  hipFree(res);
+  (void)hipFree(t64);
+  (void)hipFree(c);
+  (void)hipFree(res);
 }
 
 void template_stats(hipStream_t s, const float* T, int nx, int ny, int nz, const float* M, double* mu_t,
@@ -254,8 +254,8 @@ void template_stats(hipStream_t s, const float* T, int nx, int ny, int nz, const
   TOMHIP_CHECK(hipStreamSynchronize(s));
   *mu_t = h[0];
   *sig_t = h[1];
-  hipFree(c);
-  hipFree(res);
+  (void)hipFree(c);
+  (void)hipFree(res);
 }
 
 void launch_spheremask_os3(int nx, int ny, int nz, float* out, hipStream_t s) {
@@ -304,8 +304,8 @@ void global_max(const float* a, long n, float* val, long* idx, hipStream_t s) {
   }
   *val = bv;
   *idx = bi;
-  hipFree(pv);
-  hipFree(pi);
+  (void)hipFree(pv);
+  (void)hipFree(pi);
 }
 
 }  // namespace tomhip
