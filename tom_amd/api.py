@@ -341,6 +341,43 @@ def tom_os3_corr(img, template, mask=None, ctx: Optional[Context] = None):
     return out
 
 
+def _angle_list_1d(start: float, end: float, inc: float):
+    """One axis of tools/tom_os3_angleList.m:8-39: start:inc:end mod 360 when
+    start < end, else start+1:inc:end+359 mod 360 (wrap-around range)."""
+    if start < end:
+        return [float(np.mod(v, 360)) for v in _colon(start, inc, end)]
+    return [float(np.mod(v, 360)) for v in _colon(start + 1, inc, end + 360 - 1)]
+
+
+def _rotmatrix(phi, psi, theta):
+    """tom_angles2rotmatrix -> tom_sum_rotation (Sptrans/tom_sum_rotation.m:115-124):
+    Rz(psi) Rx(theta) Rz(phi) in double."""
+    f, p, t = np.deg2rad([phi, psi, theta])
+    rz_p = np.array([[np.cos(p), -np.sin(p), 0], [np.sin(p), np.cos(p), 0], [0, 0, 1.0]])
+    rx_t = np.array([[1.0, 0, 0], [0, np.cos(t), -np.sin(t)], [0, np.sin(t), np.cos(t)]])
+    rz_f = np.array([[np.cos(f), -np.sin(f), 0], [np.sin(f), np.cos(f), 0], [0, 0, 1.0]])
+    return (rz_p @ rx_t) @ rz_f
+
+
+def tom_os3_angleList(start, end, increment, dedup: bool = True) -> np.ndarray:
+    """Angle list of Oscar3/matlabPicker/tools/tom_os3_angleList.m:41-95 (3D):
+    phi outermost, psi, theta innermost; with dedup a triple is dropped when
+    floor(1e4 * its rotation matrix) equals that of an earlier kept triple.
+    Returns an (A, 3) float32 array of [phi psi theta] rows."""
+    axes = [_angle_list_1d(start[i], end[i], increment[i]) for i in range(3)]
+    out, seen = [], set()
+    for ph in axes[0]:
+        for ps in axes[1]:
+            for th in axes[2]:
+                if dedup:
+                    key = tuple(np.floor(_rotmatrix(ph, ps, th) * 10000).astype(np.int64).ravel())
+                    if key in seen:
+                        continue
+                    seen.add(key)
+                out.append((ph, ps, th))
+    return np.asarray(out, np.float32).reshape(-1, 3)
+
+
 def _colon(a: float, step: float, b: float) -> np.ndarray:
     """MATLAB a:step:b."""
     if step == 0 or (step > 0 and a > b) or (step < 0 and a < b):
