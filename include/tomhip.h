@@ -129,6 +129,30 @@ int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], c
 int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
                      const int64_t n[3], const float* angles, int64_t A, float* ccfmax, int32_t* angle);
 
+/* ---------------------------------------------------------------- subtomogram scoring */
+/* Batched, per-particle local angular search with per-particle wedges, in one
+ * persistent launch: the 'peak' mode of tom_mex_corr3d
+ * (Oscar3/c++Picker/src/mex/tom_mex_corr3d.cpp:53-806 -> tom::corr3d_batch,
+ * src/tom/corr/corr.cpp:969-1400, CorrelationHandlerPeaks
+ * src/tom/corr/correlation_handler.cpp:96-120; MATLAB av3_new/tom_av3_corr3d.m)
+ * with each particle scanning its own angle list (the local search of
+ * av3/alignment/av3_tomomatch.m:173-224).
+ *   box_edge[nbox]       box edges, each 32, 48 or 64
+ *   templates, masks     packed per box (box k at offset sum_{j<k} B_j^3);
+ *                        masks_cc likewise (peak-search masks in unshifted
+ *                        correlation coordinates) or NULL for everywhere
+ *   particle_box[P]      box index of each particle; particles packed with
+ *                        element offsets particle_offset[P]
+ *   wedge_semi_deg[P]    tom_wedge semi-angle of each particle's missing wedge
+ *                        (<= 0: none); SimpleWedge with cutoff (B-1)/2
+ *   angles[P][A][3]      [phi psi theta] degrees
+ * Outputs: best_angle[P] (0-based, -1 none), peak[P][3] (0-based, unshifted),
+ * ccc[P] (NaN when none). */
+int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
+                               const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
+                               const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
+                               const float* angles, int64_t A, int32_t* best_angle, int32_t* peak, float* ccc);
+
 #ifdef __cplusplus
 }
 #endif

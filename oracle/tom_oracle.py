@@ -421,3 +421,105 @@ def peak(a: np.ndarray, spline: bool = False):
         c[d] = s[d] - 1 + pos * 0.001
         vals.append(val)
     return c, float(max(vals))
+
+
+# ----------------------------------------------------------------------------
+# subtomogram scoring (corr3d_batch / corr3d_single, 'peak' handler)
+# ----------------------------------------------------------------------------
+def norm_mask(v: np.ndarray, M: np.ndarray, boolean: bool):
+    """tom::norm_mask(v, mask, NORM_NO_NORM, &variance, is_boolean_mask)
+    (Oscar3/c++Picker/src/tom/core/volume_fcn.cpp:168-247) in FP64:
+    mean over mask~=0; v2 = (v-mean)*mask (boolean: v-mean) inside, 0 outside;
+    variance = (sum v2^2 - n mean2^2) / numel; v2 -= mean2 inside."""
+    v = np.asarray(v, np.float64)
+    B = M != 0
+    n = int(B.sum())
+    mean = v[B].sum() / n
+    out = np.zeros_like(v)
+    out[B] = (v[B] - mean) if boolean else (v[B] - mean) * M[B]
+    s, s2 = out[B].sum(), (out[B] ** 2).sum()
+    mean2 = s / n
+    var = (s2 - n * mean2 * mean2) / v.size
+    if mean2 != 0:
+        out[B] -= mean2
+    return out, var
+
+
+def simple_wedge_half(shape, semi_angle_deg: float):
+    """tom::SimpleWedge(angle, cutoff=(min(size)-1)/2) weight on the full
+    unshifted frequency grid (src/tom/core/wedge.cpp:149-243, cutoff branch;
+    float arithmetic): keep |k| <= cutoff and (|kz| < 1e-4 or
+    tan(angle) <= |kx|/|kz|). Returned for every (kx, ky, kz) index; the
+    predicate is symmetric, so any half-spectrum slice of it is the reference's."""
+    f = np.float32
+    cutoff = f((min(shape) - 1) // 2)
+    tan_a = f(np.tan(semi_angle_deg * np.pi / 180.0))
+    ax = [np.fft.fftfreq(s, 1.0 / s).astype(np.float32) for s in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    X = np.where(X == -(shape[0] // 2), f(shape[0] // 2), X) if shape[0] % 2 == 0 else X
+    r = np.sqrt((X * X + Y * Y + Z * Z).astype(np.float32))
+    az = np.abs(Z)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        keep = (r <= cutoff) & ((az < f(1e-4)) | (tan_a <= (np.abs(X) / az).astype(np.float32)))
+    return keep.astype(np.float64)
+
+
+def subtomo_score(particle, template, M, mask_cc, semi_angle_deg, angles):
+    """One particle against one template over an angle list, the way
+    corr3d_batch -> corr3d_single (reapply_mask_after_wedge = true) ->
+    CorrelationHandlerPeaks do it (src/tom/corr/corr.cpp:969-1210, 1217-1400,
+    src/tom/corr/correlation_handler.cpp:96-120), FP64:
+      template -= mean(template); particle: norm_mask (soft), FFT, DC = 0
+      per angle: t = rotate(T, a); norm_mask(t); Tf = FFT(t), DC = 0
+                 particle wedge (semi_angle > 0): Tw = Tf.W, DC = 0,
+                 tw = c2r(Tw) (unnormalised), norm_mask_bool(tw) -> var_t,
+                 Tw = FFT(tw)
+                 C = conj(Tw).P, DC = 0; c = c2r(C)/(sqrt(var_p var_t) numel^2)
+                 peak of c over mask_cc != 0 (first in x-fastest order)
+      keep the strictly better value (first angle wins).
+    Returns (angle index or -1, peak [x y z] unshifted 0-based, value, maps)."""
+    n = particle.size
+    T = np.asarray(template, np.float64)
+    T = T - T.mean()
+    p, var_p = norm_mask(particle, M, False)
+    P = np.fft.fftn(p)
+    P.flat[0] = 0
+    W = simple_wedge_half(particle.shape, semi_angle_deg) if semi_angle_deg > 0 else None
+    best = (-1, np.zeros(3, int), 0.0)
+    vals = []
+    for ia, a in enumerate(np.asarray(angles).reshape(-1, 3)):
+        t = rotate(T.astype(np.float32), a).astype(np.float64)
+        t, var_t = norm_mask(t, M, False)
+        Tf = np.fft.fftn(t)
+        Tf.flat[0] = 0
+        if W is not None:
+            Tw = Tf * W
+            Tw.flat[0] = 0
+            tw = np.real(np.fft.ifftn(Tw)) * n  # FFTW c2r is unnormalised
+            tw, var_t = norm_mask(tw, M, True)
+            Tw = np.fft.fftn(tw)
+        else:
+            Tw = Tf
+        C = np.conj(Tw) * P
+        C.flat[0] = 0
+        c = np.real(np.fft.ifftn(C)) * n / (np.sqrt(var_p * var_t) * n * n)
+        sel = np.asfortranarray(mask_cc != 0)
+        cf = np.asfortranarray(c)
+        flat_c = cf.ravel(order="F")
+        flat_m = sel.ravel(order="F")
+        idxs = np.nonzero(flat_m)[0]
+        j = idxs[np.argmax(flat_c[idxs])]
+        val = flat_c[j]
+        pos = np.array([j % c.shape[0], (j // c.shape[0]) % c.shape[1], j // (c.shape[0] * c.shape[1])])
+        vals.append(val)
+        if ia == 0 or val > best[2]:
+            best = (ia, pos, val)
+    return best[0], best[1], best[2], np.asarray(vals)
+
+
+def mask_cc_sphere(shape, radius: float) -> np.ndarray:
+    """Peak-search mask in unshifted correlation coordinates: shifts s with
+    |s| <= radius (wrap-around), as a mask_cc for the 'peak' handler."""
+    ax = [np.fft.fftfreq(s, 1.0 / s) for s in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    return (np.sqrt(X * X + Y * Y + Z * Z) <= radius).astype(np.float32)

@@ -43,6 +43,8 @@ SIGNATURES = {
                                               vp]),
     "tomhip_os3_corr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "tomhip_oscar_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp]),
+    "tomhip_subtomo_score_batch": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp,
+                                                  vp, ctypes.c_int64, vp, vp, vp]),
 }
 
 _lib = None

@@ -378,6 +378,44 @@ def tom_os3_angleList(start, end, increment, dedup: bool = True) -> np.ndarray:
     return np.asarray(out, np.float32).reshape(-1, 3)
 
 
+def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, ctx: Optional[Context] = None):
+    """Batched local angular search of subtomograms (tom_av3_corr3d 'peak' /
+    corr3d_batch semantics, one persistent launch).
+
+    boxes: list of dicts {"template": (B,B,B), "mask": (B,B,B), "mask_cc":
+    (B,B,B) or None}, one per box edge B in {32, 48, 64} (MATLAB indexing).
+    particles: list of (B,B,B) arrays; particle_box[p] indexes boxes;
+    wedge_semi_deg[p] (<= 0: no wedge); angles: (P, A, 3) degrees.
+    Returns (best angle index 0-based (-1 none), peak (P,3) 0-based unshifted,
+    ccc (P,))."""
+    ctx = ctx or context()
+    edges = np.asarray([np.asarray(b["template"]).shape[0] for b in boxes], np.int32)
+    T = np.concatenate([np.asfortranarray(b["template"], np.float32).ravel(order="F") for b in boxes])
+    M = np.concatenate([np.asfortranarray(b["mask"], np.float32).ravel(order="F") for b in boxes])
+    if any(b.get("mask_cc") is not None for b in boxes):
+        Mcc = np.concatenate([np.asfortranarray(b["mask_cc"] if b.get("mask_cc") is not None
+                                                else np.ones(b["template"].shape), np.float32).ravel(order="F")
+                              for b in boxes])
+    else:
+        Mcc = None
+    pb = np.ascontiguousarray(particle_box, np.int32)
+    P = pb.shape[0]
+    sizes = [int(edges[i]) ** 3 for i in pb]
+    off = np.zeros(P, np.int64)
+    off[1:] = np.cumsum(sizes)[:-1]
+    packed = np.concatenate([np.asfortranarray(v, np.float32).ravel(order="F") for v in particles])
+    semi = np.ascontiguousarray(wedge_semi_deg, np.float32)
+    ang = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(P, -1, 3))
+    A = ang.shape[1]
+    best = np.zeros(P, np.int32)
+    peak = np.zeros((P, 3), np.int32)
+    ccc = np.zeros(P, np.float32)
+    ctx.check(ctx.lib.tomhip_subtomo_score_batch(ctx.h, len(boxes), _p(edges), _p(T), _p(M),
+                                                 None if Mcc is None else _p(Mcc), P, _p(pb), _p(packed), _p(off),
+                                                 _p(semi), _p(ang), A, _p(best), _p(peak), _p(ccc)))
+    return best, peak, ccc
+
+
 def _colon(a: float, step: float, b: float) -> np.ndarray:
     """MATLAB a:step:b."""
     if step == 0 or (step > 0 and a > b) or (step < 0 and a < b):

@@ -28,7 +28,7 @@ COMMON = [
 ]
 # per-file extra flags: the rotation must round exactly like the reference C
 EXTRA = {"rotate.hip": ["-ffp-contract=off"]}
-SOURCES = ["rotate.hip", "search.hip", "util.hip", "api.hip"]
+SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "util.hip", "api.hip"]
 
 
 def _compile(src: str) -> str:

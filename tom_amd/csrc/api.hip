@@ -1,4 +1,5 @@
 // C ABI of libtomhip.so (declared in include/tomhip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -8,6 +9,7 @@
 #include "../../include/tomhip.h"
 #include "rotate.h"
 #include "search.h"
+#include "subtomo.h"
 #include "util.h"
 
 using namespace tomhip;
@@ -539,6 +541,112 @@ int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], c
     sc.commit();
     an.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
+                               const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
+                               const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
+                               const float* angles, int64_t A, int32_t* best_angle, int32_t* peak, float* ccc) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && box_edge && templates && masks && particle_box && particles && particle_offset &&
+                       wedge_semi_deg && angles && best_angle && peak && ccc,
+                   ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(nbox >= 1 && nbox <= 64, ERR_WRONG_INPUT, "nbox must be in [1, 64]");
+    TOMHIP_REQUIRE(P >= 1 && P < (1L << 30) && A >= 1 && A <= 4096, ERR_WRONG_INPUT, "bad P or A");
+    hipStream_t s = ctx->stream;
+    // small index arrays on the host (copied from the device if needed)
+    auto host_copy = [&](const void* p, size_t bytes, void* dst) {
+      TOMHIP_CHECK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDefault, s));
+    };
+    std::vector<int32_t> edges(nbox), pb(P);
+    std::vector<int64_t> poff(P);
+    std::vector<float> semi(P);
+    host_copy(box_edge, sizeof(int32_t) * nbox, edges.data());
+    host_copy(particle_box, sizeof(int32_t) * P, pb.data());
+    host_copy(particle_offset, sizeof(int64_t) * P, poff.data());
+    host_copy(wedge_semi_deg, sizeof(float) * P, semi.data());
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    std::vector<long> toff(nbox + 1, 0);
+    for (int k = 0; k < nbox; ++k) {
+      TOMHIP_REQUIRE(edges[k] == 32 || edges[k] == 48 || edges[k] == 64, ERR_UNSUPPORTED,
+                     "subtomogram box edges must be 32, 48 or 64");
+      toff[k + 1] = toff[k] + (long)edges[k] * edges[k] * edges[k];
+    }
+    int64_t ptotal = 0;
+    for (int64_t i = 0; i < P; ++i) {
+      TOMHIP_REQUIRE(pb[i] >= 0 && pb[i] < nbox, ERR_WRONG_INPUT, "particle_box out of range");
+      TOMHIP_REQUIRE(poff[i] >= 0, ERR_WRONG_INPUT, "negative particle offset");
+      const int64_t e = (int64_t)edges[pb[i]];
+      ptotal = std::max<int64_t>(ptotal, poff[i] + e * e * e);
+    }
+    DevBuf dT(s, templates, sizeof(float) * toff[nbox], true, false);
+    DevBuf dM(s, masks, sizeof(float) * toff[nbox], true, false);
+    DevBuf dMcc(s, masks_cc, sizeof(float) * toff[nbox], true, false);
+    DevBuf dP(s, particles, sizeof(float) * ptotal, true, false);
+    DevBuf oA(s, best_angle, sizeof(int32_t) * P, false, true);
+    DevBuf oP(s, peak, sizeof(int32_t) * 3 * P, false, true);
+    DevBuf oC(s, ccc, sizeof(float) * P, false, true);
+    // mask statistics (host)
+    std::vector<float> hm(toff[nbox]);
+    TOMHIP_CHECK(hipMemcpyAsync(hm.data(), dM.get<float>(), sizeof(float) * toff[nbox], hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    std::vector<SubtomoBox> boxes(nbox);
+    for (int k = 0; k < nbox; ++k) {
+      SubtomoBox& b = boxes[k];
+      b.B = edges[k];
+      b.n = 0;
+      b.SM = b.SM2 = 0;
+      for (long i = toff[k]; i < toff[k + 1]; ++i) {
+        const double m = hm[i];
+        b.n += (m != 0);
+        b.SM += m;
+        b.SM2 += m * m;
+      }
+      TOMHIP_REQUIRE(b.n > 0, ERR_WRONG_INPUT, "a mask has no non-zero voxel");
+      b.M = dM.get<float>() + toff[k];
+      b.Mcc = masks_cc ? dMcc.get<float>() + toff[k] : nullptr;
+      b.T = nullptr;
+      b.FM = b.FB = nullptr;
+    }
+    // processing order: largest boxes first; wedge tangent; rotation matrices
+    std::vector<int32_t> order(P);
+    for (int64_t i = 0; i < P; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return edges[pb[a]] > edges[pb[b]]; });
+    std::vector<float> tanw(P);
+    const double kPi = 3.14159265358979323846264338327950288419716939937510;
+    for (int64_t i = 0; i < P; ++i) tanw[i] = semi[i] > 0 ? (float)tan(semi[i] * kPi / 180.0) : -1.f;
+    std::vector<float> ang((size_t)P * A * 3);
+    host_copy(angles, sizeof(float) * ang.size(), ang.data());
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    std::vector<float> rms = rotmats(ang.data(), P * A);
+    std::vector<long> poffl(poff.begin(), poff.end());
+    std::vector<void*> tmp;
+    auto up = [&](const void* h, size_t bytes) {
+      void* d = nullptr;
+      TOMHIP_CHECK(hipMalloc(&d, bytes));
+      tmp.push_back(d);
+      TOMHIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+      return d;
+    };
+    try {
+      const int32_t* d_pb = (const int32_t*)up(pb.data(), sizeof(int32_t) * P);
+      const int32_t* d_order = (const int32_t*)up(order.data(), sizeof(int32_t) * P);
+      const long* d_poff = (const long*)up(poffl.data(), sizeof(long) * P);
+      const float* d_tan = (const float*)up(tanw.data(), sizeof(float) * P);
+      const float* d_rm = (const float*)up(rms.data(), sizeof(float) * rms.size());
+      subtomo_run(*ctx->tw, s, boxes, dT.get<float>(), toff.data(), P, d_pb, d_order, dP.get<float>(), d_poff, d_tan,
+                  d_rm, A, oA.get<int32_t>(), oP.get<int32_t>(), oC.get<float>());
+      oA.commit();
+      oP.commit();
+      oC.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      for (void* p : tmp) (void)hipFree(p);
+      throw;
+    }
+    for (void* p : tmp) (void)hipFree(p);
   });
 }
 

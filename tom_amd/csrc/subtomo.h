@@ -1,0 +1,32 @@
+// Batched subtomogram scoring (host side of subtomo.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "fft_kernels.h"
+
+namespace tomhip {
+
+struct SubtomoBox {
+  int B;             // box edge (32, 48 or 64)
+  long n;            // #{M != 0}
+  double SM, SM2;    // sum of M, sum of M^2
+  const float* T;    // device: template minus its mean
+  const float* M;    // device: mask
+  const float* Mcc;  // device: peak-search mask (unshifted) or null
+  const c32* FM;     // device: half spectrum of M
+  const c32* FB;     // device: half spectrum of (M != 0)
+};
+
+int subtomo_workgroups();
+long subtomo_scratch_floats();
+// boxes_h[k].M / Mcc / B / n / SM / SM2 filled by the caller; T, FM, FB are
+// produced here. rms_dev: [P][A][9] rotation matrices.
+void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h, const float* templates_dev,
+                 const long* tmpl_off, int64_t P, const int32_t* pbox_dev, const int32_t* order_dev,
+                 const float* particles_dev, const long* poff_dev, const float* tan_dev, const float* rms_dev,
+                 int64_t A, int32_t* best_idx, int32_t* best_pos, float* best_val);
+
+}  // namespace tomhip
