@@ -153,6 +153,38 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
                                const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
                                const float* angles, int64_t A, int32_t* best_angle, int32_t* peak, float* ccc);
 
+/* ---------------------------------------------------------------- picklist */
+/* tom_os3_returnPicklist(peaks, angles, job, n) without binning
+ * (Oscar3/matlabPicker/results/tom_os3_returnPicklist.m:40-95): optional
+ * tom_os3_eraseBorders, then up to n x tom_peakc(peaks, ceil(tsize[0]/2))
+ * (Analysis/tom_peakc.m:39-86) until std(peaks) = 0. angle_map (int32, the
+ * search's best-angle map) may be NULL. out: [n][5] rows x y z (1-based)
+ * value angle; *count = rows written. The input map is not modified. */
+int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], const int32_t* angle_map,
+                        const int64_t tsize[3], int64_t n, int erase_borders, float* out, int64_t* count);
+
+/* ---------------------------------------------------------------- EM files */
+/* EM format header (Oscar3/c++Picker/include/tom/core/io.h:49-63). */
+typedef struct tomhip_em_header {
+  int8_t machine; /* 6 = PC little-endian; 0, 3, 5 big-endian */
+  int8_t byte2;
+  int8_t byte3;
+  int8_t type; /* 1 int8, 2 int16, 4 int32, 5 float, 8 complex, 9 double, 10 complex double */
+  uint32_t dims[3];
+  int8_t comment[80];
+  int32_t emdata[40];
+  int8_t userdata[256];
+} tomhip_em_header;
+/* tom_emread (IOfun/tom_emread.m:153-220): header, and the data of any real
+ * type converted to float32 (x fastest; capacity in elements). Host code; no
+ * context needed. */
+int tomhip_em_read_header(const char* path, tomhip_em_header* hdr);
+int tomhip_em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em_header* hdr_out);
+/* tom_emwrite: float32 PC-format file; hdr (optional) supplies comment,
+ * emdata and userdata. */
+int tomhip_em_write_f32(const char* path, const float* data, const int64_t dims[3], const tomhip_em_header* hdr);
+const char* tomhip_em_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

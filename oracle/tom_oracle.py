@@ -523,3 +523,41 @@ def mask_cc_sphere(shape, radius: float) -> np.ndarray:
     ax = [np.fft.fftfreq(s, 1.0 / s) for s in shape]
     X, Y, Z = np.meshgrid(*ax, indexing="ij")
     return (np.sqrt(X * X + Y * Y + Z * Z) <= radius).astype(np.float32)
+
+
+# ----------------------------------------------------------------------------
+# picklist (tom_os3_returnPicklist -> tom_peakc)
+# ----------------------------------------------------------------------------
+def erase_borders(volume, tsize):
+    """tom_os3_eraseBorders: keep the centred box of size N - (n+1)."""
+    inner = np.ones([s - (t + 1) for s, t in zip(volume.shape, tsize)])
+    return np.asarray(volume, np.float64) * paste_center(volume.shape, inner)
+
+
+def peakc(a, radius):
+    """tom_peakc(in, radius) (Analysis/tom_peakc.m:39-86) for interior peaks:
+    first maximum (x-fastest), then zero the voxels of the cut-out box
+    [c-r, c+r) closer than r to its centre (size/2+1, 1-based). Returns
+    (1-based position, value, modified copy)."""
+    v, p = find_max(np.asarray(a, np.float32))
+    a = np.array(a, np.float64, copy=True)
+    lo = [max(int(p[d]) + 1 - radius, 1) - 1 for d in range(3)]
+    hi = [min(lo[d] + 2 * radius, a.shape[d]) for d in range(3)]
+    box = a[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+    cen = [s / 2.0 + 1 for s in box.shape]
+    I, J, K = np.meshgrid(*[np.arange(s) for s in box.shape], indexing="ij")
+    dist = np.sqrt((I + 1 - cen[0]) ** 2 + (J + 1 - cen[1]) ** 2 + (K + 1 - cen[2]) ** 2)
+    box[dist < radius] = 0
+    return p + 1, v, a
+
+
+def return_picklist(peaks, angles, tsize, n):
+    """tom_os3_returnPicklist rows [x y z value angle] (no binning)."""
+    r = int(np.ceil(tsize[0] / 2))
+    pk = erase_borders(peaks, tsize) if tuple(peaks.shape) != tuple(tsize) else np.asarray(peaks, np.float64)
+    rows = []
+    while len(rows) < n and np.std(pk) > 0:
+        c, v, pk = peakc(pk.astype(np.float32), r)
+        ang = 0 if angles is None else angles[c[0] - 1, c[1] - 1, c[2] - 1]
+        rows.append([c[0], c[1], c[2], v, ang])
+    return np.asarray(rows, np.float64).reshape(-1, 5)

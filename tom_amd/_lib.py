@@ -43,6 +43,11 @@ SIGNATURES = {
                                               vp]),
     "tomhip_os3_corr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "tomhip_oscar_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp]),
+    "tomhip_picklist_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp, vp]),
+    "tomhip_em_read_header": (ctypes.c_int, [ctypes.c_char_p, vp]),
+    "tomhip_em_read_f32": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int64, vp]),
+    "tomhip_em_write_f32": (ctypes.c_int, [ctypes.c_char_p, vp, vp, vp]),
+    "tomhip_em_last_error": (ctypes.c_char_p, []),
     "tomhip_subtomo_score_batch": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp,
                                                   vp, ctypes.c_int64, vp, vp, vp]),
 }

@@ -416,6 +416,56 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
     return best, peak, ccc
 
 
+def tom_os3_returnPicklist(peaks, template_size, n, angles=None, erase_borders: bool = True,
+                           ctx: Optional[Context] = None) -> np.ndarray:
+    """Picklist rows [x y z (1-based) value angle] of tom_os3_returnPicklist
+    (no binning): iterated tom_peakc with radius ceil(template_size/2)."""
+    pb = _Buf(peaks)
+    ab = _Buf(angles, np.int32) if angles is not None else None
+    ctx = _prep_ctx(ctx, pb, ab)
+    N = _dims3(pb.dims)
+    t = _dims3(template_size if np.ndim(template_size) else (template_size,) * 3)
+    out = np.zeros((max(int(n), 1), 5), np.float32)
+    cnt = np.zeros(1, np.int64)
+    ctx.check(ctx.lib.tomhip_picklist_f32(ctx.h, pb.ptr, _p(N), None if ab is None else ab.ptr, _p(t), int(n),
+                                          int(erase_borders), _p(out), _p(cnt)))
+    return out[:int(cnt[0])]
+
+
+_EM_HEADER = np.dtype([("machine", "i1"), ("byte2", "i1"), ("byte3", "i1"), ("type", "i1"), ("dims", "<u4", 3),
+                       ("comment", "S80"), ("emdata", "<i4", 40), ("userdata", "S256")])
+
+
+def tom_emread(path: str):
+    """(volume (nx, ny, nz) float32 in MATLAB indexing, header record) of an
+    EM file (IOfun/tom_emread.m), read by the library's C++ reader."""
+    lib = load()
+    hdr = np.zeros(1, _EM_HEADER)
+    rc = lib.tomhip_em_read_header(path.encode(), _p(hdr))
+    if rc != 0:
+        raise TomHipError(rc, lib.tomhip_em_last_error().decode())
+    dims = tuple(int(d) for d in hdr["dims"][0])
+    out = np.zeros(dims, np.float32, order="F")
+    rc = lib.tomhip_em_read_f32(path.encode(), _p(out), out.size, _p(hdr))
+    if rc != 0:
+        raise TomHipError(rc, lib.tomhip_em_last_error().decode())
+    return out, hdr[0]
+
+
+def tom_emwrite(path: str, vol, header=None) -> None:
+    """Write a float32 PC-format EM file (tom_emwrite)."""
+    lib = load()
+    v = np.asfortranarray(vol, np.float32)
+    d = _dims3(v.shape)
+    h = None
+    if header is not None:
+        h = np.zeros(1, _EM_HEADER)
+        h[0] = header
+    rc = lib.tomhip_em_write_f32(path.encode(), _p(v), _p(d), None if h is None else _p(h))
+    if rc != 0:
+        raise TomHipError(rc, lib.tomhip_em_last_error().decode())
+
+
 def _colon(a: float, step: float, b: float) -> np.ndarray:
     """MATLAB a:step:b."""
     if step == 0 or (step > 0 and a > b) or (step < 0 and a < b):

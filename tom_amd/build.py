@@ -28,12 +28,16 @@ COMMON = [
 ]
 # per-file extra flags: the rotation must round exactly like the reference C
 EXTRA = {"rotate.hip": ["-ffp-contract=off"]}
-SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "util.hip", "api.hip"]
+SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "api.hip", "em_io.cpp"]
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, src + ".o")
-    cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):  # host-only code
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-c",
+               os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

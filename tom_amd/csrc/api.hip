@@ -8,6 +8,7 @@
 
 #include "../../include/tomhip.h"
 #include "rotate.h"
+#include "picks.h"
 #include "search.h"
 #include "subtomo.h"
 #include "util.h"
@@ -541,6 +542,33 @@ int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], c
     sc.commit();
     an.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], const int32_t* angle_map,
+                        const int64_t tsize[3], int64_t n, int erase_borders, float* out, int64_t* count) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && out && count, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "map");
+    check_dims3(tsize, "template");
+    TOMHIP_REQUIRE(n >= 0 && n <= 10000000, ERR_WRONG_INPUT, "bad n");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf m(s, map, sizeof(float) * NN, true, false);
+    DevBuf a(s, angle_map, sizeof(int32_t) * NN, true, false);
+    DevBuf o(s, out, sizeof(float) * 5 * (n > 0 ? n : 1), false, true);
+    float* work = nullptr;
+    TOMHIP_CHECK(hipMalloc(&work, sizeof(float) * NN));
+    try {
+      *count = picklist(m.get<float>(), (int)N[0], (int)N[1], (int)N[2], angle_map ? a.get<int32_t>() : nullptr,
+                        (int)tsize[0], (int)tsize[1], (int)tsize[2], n, erase_borders, work, o.get<float>(), s);
+      o.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(work);
+      throw;
+    }
+    (void)hipFree(work);
   });
 }
 
