@@ -109,6 +109,9 @@ int tomhip_search_finalize(tomhip_search* s, float* score, int32_t* angle);
  * ms/counts: 8 entries (rotate, box weight, template fwd, pass A, pass B,
  * pass C, -, -). */
 int tomhip_search_profile(tomhip_search* s, int enable, double* ms, int64_t* counts);
+/* Replay full angle batches through one captured hipGraph (1, default) or
+ * launch every kernel eagerly (0). Results are identical. */
+int tomhip_search_set_graph(tomhip_search* s, int enable);
 void tomhip_search_destroy(tomhip_search* s);
 
 /* ---------------------------------------------------------------- one-shot drivers */

@@ -191,7 +191,8 @@ def test_find_template_no_weight_binary_mask(api, oracle):
 
 
 def test_search_plan_batches_equal(api, oracle):
-    """Batch size and split runs do not change the result (ties included)."""
+    """Batch size, split runs and hipGraph replay versus eager launches do not
+    change the result (ties included)."""
     say("plan batches")
     rng = np.random.default_rng(8)
     V = rng.standard_normal((64, 32, 32)).astype(np.float32)
@@ -200,8 +201,9 @@ def test_search_plan_batches_equal(api, oracle):
     ang = oracle.angle_grid(60, 60, 60)
     tn, mu, sg = api.prepare_template(T, M)
     outs = []
-    for batch, split in [(1, 1), (4, 1), (7, 3)]:
+    for batch, split, graph in [(1, 1, False), (4, 1, True), (4, 1, False), (7, 3, True), (3, 1, True)]:
         with api.SearchPlan(V, T.shape, M, batch=batch) as p:
+            p.set_graph(graph)
             p.set_template(tn, None, mu, sg)
             step = int(np.ceil(len(ang) / split))
             for s0 in range(0, len(ang), step):

@@ -38,6 +38,7 @@ SIGNATURES = {
     "tomhip_search_run": (ctypes.c_int, [vp, vp, ctypes.c_int64, ctypes.c_int64]),
     "tomhip_search_finalize": (ctypes.c_int, [vp, vp, vp]),
     "tomhip_search_profile": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
+    "tomhip_search_set_graph": (ctypes.c_int, [vp, ctypes.c_int]),
     "tomhip_search_destroy": (None, [vp]),
     "tomhip_flcf_search_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp,
                                               vp]),

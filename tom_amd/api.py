@@ -286,6 +286,10 @@ class SearchPlan:
         self.ctx.check(self.ctx.lib.tomhip_search_finalize(self.h, _ptr(sc), _ptr(an)))
         return sc, an
 
+    def set_graph(self, enable: bool):
+        """Replay full angle batches through a captured hipGraph (default)."""
+        self.ctx.check(self.ctx.lib.tomhip_search_set_graph(self.h, int(enable)))
+
     def profile(self, enable: bool):
         """Enable/disable per-kernel event timing; returns (ms[8], counts[8])
         accumulated since the previous call."""

@@ -412,6 +412,14 @@ int tomhip_search_profile(tomhip_search* h, int enable, double* ms, int64_t* cou
   });
 }
 
+int tomhip_search_set_graph(tomhip_search* h, int enable) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h, ERR_WRONG_INPUT, "null search");
+    h->s->set_graph(enable != 0);
+  });
+}
+
 void tomhip_search_destroy(tomhip_search* h) {
   if (!h) return;
   if (h->ctx) {

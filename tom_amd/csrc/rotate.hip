@@ -34,6 +34,8 @@ struct RotParams {
   int sx, sy, sz;
   float px, py, pz;
   long dst_stride;      // elements between consecutive rotations in dst
+  const long* desc;     // optional graph batch descriptor {batch, first}: rm += batch*rm_step
+  long rm_step;
 };
 
 __device__ __forceinline__ float interp(const float* img, int b, int sx, int sxy, int ex, int ey,
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(NT_ROT) void k_rotate_brick(RotParams P) {
   const int bid = blockIdx.x;
   const int bx0 = (bid % nbx) * BR, by0 = ((bid / nbx) % nby) * BR, bz0 = (bid / (nbx * nby)) * BR;
   const int rot = blockIdx.y;
-  const float* rmp = P.rm + rot * 9;
+  const float* rmp = P.rm + rot * 9 + (P.desc ? P.desc[0] * P.rm_step : 0);
   const float rm00 = rmp[0], rm10 = rmp[1], rm20 = rmp[2];
   const float rm01 = rmp[3], rm11 = rmp[4], rm21 = rmp[5];
   const float rm02 = rmp[6], rm12 = rmp[7], rm22 = rmp[8];
@@ -197,7 +199,8 @@ void rotmat_ref(const float* euler, int neuler, float* rm) {
 }
 
 void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, const float* mask, int sx,
-                   int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream) {
+                   int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream,
+                   const long* desc, long rm_step) {
   RotParams P;
   P.src = src;
   P.dst = dst;
@@ -210,6 +213,8 @@ void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, 
   P.py = py;
   P.pz = pz;
   P.dst_stride = dst_stride;
+  P.desc = desc;
+  P.rm_step = rm_step;
   const int nb = ((sx + BR - 1) / BR) * ((sy + BR - 1) / BR) * ((sz + BR - 1) / BR);
   dim3 grid(nb, nrot);
   hipLaunchKernelGGL(k_rotate_brick, grid, dim3(NT_ROT), 0, stream, P);

@@ -44,6 +44,9 @@ class Search {
   void finalize(float* score, int32_t* angle);
 
   void set_profile(bool on) { profile_ = on; }
+  // Replay full angle batches through one captured hipGraph (default on;
+  // profiling runs eager so every launch can be timed).
+  void set_graph(bool on) { use_graph_ = on; }
   // Resolves the recorded event pairs (synchronises the stream) and returns
   // the accumulated per-kernel times.
   const KernelTimes& times();
@@ -58,7 +61,9 @@ class Search {
 
  private:
   void precompute_stats(const float* volume);
-  void run_batch(const float* rm_dev, int ka, long first_index);
+  void run_batch(const float* rm_dev, int ka, long first_index, const long* desc);
+  void capture_graph();
+  void drop_graph();
   template <class F>
   void timed(int kid, F&& f);
 
@@ -93,6 +98,12 @@ class Search {
   c32* Q_ = nullptr;         // [KA][Z][Y][Xh]
   bool started_ = false;
   bool remask_ = true;
+  // graph of one full batch, replayed with the device descriptor desc_
+  bool use_graph_ = true;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  long* desc_ = nullptr;  // {batch, first index}
+  bool graph_has_w_ = false, graph_remask_ = true;
 };
 
 }  // namespace tomhip

@@ -125,6 +125,11 @@ __global__ void k_stats(const c64* A, long n, double inv_nm, double g, double s,
   }
 }
 
+// last node of a captured angle batch: advance the batch descriptor
+__global__ void k_advance(long* desc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) desc[0] += 1;
+}
+
 __global__ void k_fill_max(float* maxc, int32_t* argm, long n) {
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
     maxc[i] = -FLT_MAX;
@@ -337,7 +342,8 @@ constexpr int NT_C = 512;
 template <int XH>
 __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void k_passC(const c32* Q, float* maxc, int32_t* argm, const c32* twHg,
                                                 const c32* twXg, long nlines, long qstride, int ka,
-                                                int first_index) {
+                                                int first_index, const long* desc) {
+  if (desc) first_index = (int)(desc[1] + desc[0] * ka);
   constexpr int NL = passc_lines<XH>();
   constexpr int S = plan_nstages(XH);
   constexpr int R0 = plan_radix(XH, 0);
@@ -597,6 +603,8 @@ Search::~Search() {
   (void)hipFree(wsh_);
   (void)hipFree(rm_);
   (void)hipFree(const_cast<float*>(mask_));
+  drop_graph();
+  (void)hipFree(desc_);
   for (auto& e : evpool_) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
@@ -659,11 +667,15 @@ void Search::set_template(const float* tmpl, const float* wsh, double mu_t_, dou
     TOMHIP_REQUIRE(mode == MODE_FLCF && remask, ERR_WRONG_INPUT, "a weight needs FLCF mode with remask");
     TOMHIP_REQUIRE(supported_len(nx) && supported_len(ny) && supported_len(nz), ERR_UNSUPPORTED,
                    "weighting needs template edges in the supported FFT lengths");
-    if (!wsh_) TOMHIP_CHECK(hipMalloc(&wsh_, sizeof(float) * n3));
+    if (!wsh_) {
+      TOMHIP_CHECK(hipMalloc(&wsh_, sizeof(float) * n3));
+      drop_graph();  // the captured batch reads wsh_
+    }
     TOMHIP_CHECK(hipMemcpyAsync(wsh_, wsh, sizeof(float) * n3, hipMemcpyDefault, s_));
   } else if (wsh_) {
     (void)hipFree(wsh_);
     wsh_ = nullptr;
+    drop_graph();
   }
   mu_t = mu_t_;
   sig_t = sig_t_;
@@ -713,22 +725,61 @@ void Search::run(const float* rm_host, long count, long first_index) {
     (void)hipFree(rm_);
     TOMHIP_CHECK(hipMalloc(&rm_, sizeof(float) * 9 * count));
     rm_cap_ = count;
+    drop_graph();  // captured pointers changed
   }
   TOMHIP_CHECK(hipMemcpyAsync(rm_, rm_host, sizeof(float) * 9 * count, hipMemcpyHostToDevice, s_));
-  for (long b = 0; b < count; b += KA_) {
+  const long nfull = count / KA_;
+  long done = 0;
+  if (use_graph_ && !profile_ && nfull >= 2) {
+    if (!graph_exec_ || graph_has_w_ != (wsh_ != nullptr) || graph_remask_ != remask_) capture_graph();
+    const long d[2] = {0, first_index};
+    TOMHIP_CHECK(hipMemcpyAsync(desc_, d, sizeof(d), hipMemcpyHostToDevice, s_));
+    for (long b = 0; b < nfull; ++b) TOMHIP_CHECK(hipGraphLaunch(graph_exec_, s_));
+    done = nfull * KA_;
+  }
+  for (long b = done; b < count; b += KA_) {
     const int ka = (int)((count - b < KA_) ? (count - b) : KA_);
-    run_batch(rm_ + 9 * b, ka, first_index + b);
+    run_batch(rm_ + 9 * b, ka, first_index + b, nullptr);
   }
 }
 
-void Search::run_batch(const float* rm_dev, int ka, long first_index) {
+void Search::capture_graph() {
+  drop_graph();
+  if (!desc_) TOMHIP_CHECK(hipMalloc(&desc_, sizeof(long) * 2));
+  hipGraph_t g = nullptr;
+  TOMHIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+  try {
+    run_batch(rm_, KA_, 0, desc_);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s_, desc_);
+  } catch (...) {
+    (void)hipStreamEndCapture(s_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  TOMHIP_CHECK(hipStreamEndCapture(s_, &g));
+  TOMHIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+  graph_ = g;
+  graph_has_w_ = (wsh_ != nullptr);
+  graph_remask_ = remask_;
+}
+
+void Search::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void Search::run_batch(const float* rm_dev, int ka, long first_index, const long* desc) {
   const long n3 = (long)nx * ny * nz;
   const float px = (float)((nx + 1) / 2), py = (float)((ny + 1) / 2), pz = (float)((nz + 1) / 2);
   // rotate; FLCF: times M (or M^2 when no weight follows, since the masked
   // template is masked again inside tom_os3_corr)
   const float* rmask = nullptr;
   if (mode == MODE_FLCF) rmask = (wsh_ || !remask_) ? mask_ : mask_ + n3;
-  timed(K_ROT, [&] { launch_rotate(tmpl_, t1_, rm_dev, ka, rmask, nx, ny, nz, px, py, pz, n3, s_); });
+  timed(K_ROT, [&] {
+    launch_rotate(tmpl_, t1_, rm_dev, ka, rmask, nx, ny, nz, px, py, pz, n3, s_, desc, 9L * ka);
+  });
 
   if (wsh_) {
     timed(K_BOX, [&] {
@@ -810,7 +861,7 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index) {
       constexpr int NL = passc_lines<L>();
       const long nl = (long)Y * Z;
       hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT_C), 0, s_, (const c32*)Q_, maxc_,
-                         argm_, tw_.f32(L), tw_.f32(X), nl, (long)Z * Y * Xh, ka, (int)first_index);
+                         argm_, tw_.f32(L), tw_.f32(X), nl, (long)Z * Y * Xh, ka, (int)first_index, desc);
     });
     TOMHIP_CHECK(hipGetLastError());
   });
