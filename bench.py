@@ -58,52 +58,13 @@ def angle_grid(step):
     return np.asarray([(a, b, c) for a in phi for b in psi for c in the], np.float32)
 
 
-def phantom(rng, n, k, sigma, R):
-    x = np.arange(n, dtype=np.float64) - n // 2
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    T = np.zeros((n, n, n))
-    for _ in range(k):
-        while True:
-            c = rng.uniform(-R, R, 3)
-            if np.linalg.norm(c) <= R:
-                break
-        T += rng.uniform(0.5, 1.0) * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * sigma ** 2))
-    return T.astype(np.float32)
-
-
-def place(rng, N, n, P, spacing):
-    pos = []
-    lo = n // 2
-    tries = 0
-    while len(pos) < P:
-        tries += 1
-        if tries > 200000:
-            raise RuntimeError("could not place particles")
-        c = np.array([rng.integers(lo, N[d] - n + lo + 1) for d in range(3)])
-        if all(np.linalg.norm(c - q) >= spacing for q in pos):
-            pos.append(c)
-    return np.asarray(pos)
-
-
 def make_tomogram(cfg, api, torch, dev):
     """Seeded synthetic tomogram on the GPU (rotations by the library itself)."""
+    from tom_amd import synth
     rng = np.random.default_rng(cfg["seed"])
-    N, n = cfg["N"], cfg["n"]
-    T = phantom(rng, n, cfg["k"], cfg["sigma"], cfg["R"])
-    pos = place(rng, N, n, cfg["P"], cfg["spacing"])
-    eul = np.stack([rng.uniform(0, 360, cfg["P"]), rng.uniform(0, 360, cfg["P"]),
-                    np.degrees(np.arccos(rng.uniform(-1, 1, cfg["P"])))], 1).astype(np.float32)
-    Td = torch.from_numpy(np.ascontiguousarray(T.transpose(2, 1, 0))).to(dev)
-    rot = api.tom_rotate_batch(Td, eul)  # (P, n, n, n) as (z, y, x)
-    S = torch.zeros((N[2], N[1], N[0]), dtype=torch.float32, device=dev)
-    h = n // 2
-    for p in range(cfg["P"]):
-        x, y, z = (int(v) - h for v in pos[p])
-        S[z:z + n, y:y + n, x:x + n] += rot[p]
-    S *= math.sqrt(cfg["snr"] / float(S.double().var()))
-    g = torch.Generator(device=dev)
-    g.manual_seed(cfg["seed"])
-    V = S + torch.randn(S.shape, generator=g, device=dev, dtype=torch.float32)
+    T = synth.phantom(rng, cfg["n"], cfg["k"], cfg["sigma"], cfg["R"])
+    V, pos, eul, _ = synth.make_tomogram(cfg["N"], [T], cfg["P"], cfg["spacing"], cfg["snr"], rng, cfg["seed"],
+                                         api, torch, dev)
     return V, T, pos, eul
 
 
@@ -235,6 +196,19 @@ def main():
     log(f"[bench] {ntimed} orientations in {dt:.3f} s -> {value:.4e} voxel-orientations/s "
         f"({dt / ntimed * 1e3:.3f} ms/orientation)")
 
+    # ---- the same orientations again through hipGraph replays (no per-kernel
+    # events): the default production path
+    graph = None
+    if not args.no_profile:
+        plan.set_template(tn, W, mu_t, sig_t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(angles[:ntimed], 0)
+        torch.cuda.synchronize()
+        dtg = time.perf_counter() - t0
+        graph = {"value": N * ntimed / dtg, "ms_per_step": dtg / max(1, math.ceil(ntimed / KA)) * 1e3}
+        log(f"[bench] hipGraph replay: {ntimed} orientations in {dtg:.3f} s -> {graph['value']:.4e}")
+
     # ---- finalize + sanity: planted particles are found
     t0 = time.perf_counter()
     score, ang = plan.finalize()
@@ -303,7 +277,9 @@ def main():
         "config": {"workload": cfg["desc"], "volume": list(cfg["N"]), "template": cfg["n"],
                    "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA,
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
-                   "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel},
+                   "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
+                   "timed_path": "eager launches with per-kernel HIP events",
+                   "graph_replay": graph},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

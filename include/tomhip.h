@@ -166,6 +166,57 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
 int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], const int32_t* angle_map,
                         const int64_t tsize[3], int64_t n, int erase_borders, float* out, int64_t* count);
 
+/* Local exact refinement of candidates (no reference counterpart as an entry
+ * point; it evaluates the same score as tomhip_search_run/finalize, written
+ * as a direct sum): for candidate p with group g = cand[p][3], every
+ * orientation a of euler[a][g][0..2] (A per group, degrees) is scored at
+ * cand[p][0..2] (0-based voxel) and its 26 neighbours. Outputs per candidate:
+ * best score, best a (first wins, strict '>'), best voxel as x y z.
+ * Needs an FLCF plan with its template set. volume: the searched volume. */
+int tomhip_search_refine(tomhip_search* h, const float* volume, const float* euler, int G, int A,
+                         const int32_t* cand, int64_t P, float* score, int32_t* angle, int32_t* pos);
+
+/* ---------------------------------------------------------------- learned proposal */
+/* No counterpart in the reference: a 3-level 3D U-Net (16/32/64 channels, two
+ * 3x3x3 convolutions + ReLU per level, 2x2x2 max-pool, trilinear x2
+ * upsampling, skip concatenation) with a 1x1x1 head giving a particle logit
+ * and nbin orientation-bin logits per voxel. BF16 MFMA convolutions with FP32
+ * accumulation, FP32 master weights, Adam. It proposes candidate positions and
+ * coarse orientations; the exact FLCF then refines them locally.
+ * Volumes: float32 x fastest, edges multiples of 4 and >= 8. */
+typedef struct tomhip_unet tomhip_unet;
+int tomhip_unet_create(tomhip_ctx* ctx, const int64_t dims[3], int nbin, uint64_t seed, int train,
+                       tomhip_unet** out);
+void tomhip_unet_destroy(tomhip_unet* h);
+int tomhip_unet_num_params(tomhip_unet* h, int64_t* n);
+/* parameters in layer order (encoder 1a 1b 2a 2b, bottom a b, decoder 2a 2b
+ * 1a 1b, head), each layer W[taps][Cout][Cin] then bias[Cout] */
+int tomhip_unet_get_params(tomhip_unet* h, float* out);
+int tomhip_unet_set_params(tomhip_unet* h, const float* in);
+int tomhip_unet_forward(tomhip_unet* h, const float* volume);
+/* logits [V][80]: channel 0 particle, 1..nbin orientation bins */
+int tomhip_unet_get_logits(tomhip_unet* h, float* out);
+/* loss: weighted BCE (positive weight wpos) of channel 0 against tprob[V],
+ * averaged over voxels, plus softmax cross-entropy of the bins against
+ * tbin[V] (-1 = no label) averaged over labelled voxels. Computes all
+ * parameter gradients; loss (optional) = {bce, ce}. */
+int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, float wpos, double* loss);
+int tomhip_unet_get_grads(tomhip_unet* h, float* out);
+int tomhip_unet_adam(tomhip_unet* h, float lr);
+/* prob[V] = sigmoid(logit 0), bin[V] = argmax of the bin logits (0-based) */
+int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin);
+/* Whole-volume inference in overlapping tiles of the network's size (halo
+ * voxels of context on each side, tile origins on multiples of 4). */
+int tomhip_unet_infer(tomhip_unet* h, const float* volume, const int64_t N[3], int halo, float* prob,
+                      int32_t* bin);
+
+/* Candidate extraction for the learned proposal (no reference counterpart):
+ * local maxima of map (x fastest) above threshold within the cube of
+ * half-edge radius (ties to the lowest linear index), as rows x y z (0-based)
+ * value, unordered. *count = number found; at most capacity rows written. */
+int tomhip_local_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, float threshold,
+                            int64_t capacity, float* out, int64_t* count);
+
 /* ---------------------------------------------------------------- EM files */
 /* EM format header (Oscar3/c++Picker/include/tom/core/io.h:49-63). */
 typedef struct tomhip_em_header {

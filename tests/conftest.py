@@ -17,3 +17,10 @@ def oracle():
     from oracle import tom_oracle
     tom_oracle.lib()
     return tom_oracle
+
+
+@pytest.fixture(scope="session")
+def api():
+    from tom_amd import api as _api
+    _api.context()
+    return _api

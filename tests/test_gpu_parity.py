@@ -209,6 +209,14 @@ def test_search_plan_batches_equal(api, oracle):
             for s0 in range(0, len(ang), step):
                 p.run(ang[s0:s0 + step], s0)
             outs.append(p.finalize())
+    # a fresh context whose first launches are a graph capture (twiddle
+    # tables must exist before the capture starts)
+    fresh = api.Context(0)
+    with api.SearchPlan(V, T.shape, M, batch=4, ctx=fresh) as p:
+        p.set_graph(True)
+        p.set_template(tn, None, mu, sg)
+        p.run(ang, 0)
+        outs.append(p.finalize())
     for sc, an in outs[1:]:
         np.testing.assert_array_equal(sc, outs[0][0])
         np.testing.assert_array_equal(an, outs[0][1])
@@ -233,3 +241,44 @@ def test_oscar(api, oracle):
     sel = (srt[-1] - srt[-2]) > 1e-4 * scale
     np.testing.assert_array_equal(phi[sel], ang[idx, 0][sel].astype(np.int16))
     np.testing.assert_array_equal(the[sel], ang[idx, 2][sel].astype(np.int16))
+
+
+def test_refine_matches_exhaustive_maps(api, oracle):
+    """tomhip_search_refine: for each candidate, the best FP64 score over its
+    3x3x3 neighbourhood (wrapping at the borders) and its group's
+    orientations; the orientation and voxel agree wherever the oracle's margin
+    between the best and second-best (voxel, orientation) pair exceeds 1e-4."""
+    say("refine")
+    rng = np.random.default_rng(21)
+    N = (64, 48, 32)
+    n = 16
+    V = rng.standard_normal(N).astype(np.float32)
+    T = _phantom(rng, n, 4, 1.5, 5)
+    V[20:36, 10:26, 8:24] += 2 * oracle.rotate(T, [30, 60, 90])
+    M = oracle.sphere_mask_gauss(n, 6, 1.5)
+    W = (oracle.wedge((n, n, n), 30) * oracle.bandpass_weight((n, n, n), 1, 7)).astype(np.float32)
+    angles = np.concatenate([oracle.angle_grid(60, 60, 60)[::7], [[30, 60, 90]]])[:20]
+    G, A = 2, 10
+    ga = angles.reshape(G, A, 3).transpose(1, 0, 2)  # (A, G, 3)
+    _, _, allm = oracle.find_template(V, T, M, angles, W=W, return_all=True)
+    cand = np.array([[28, 18, 16, 0], [28, 18, 16, 1], [0, 0, 0, 0], [63, 47, 31, 1], [5, 40, 30, 0],
+                     [40, 3, 1, 1]], np.int32)
+    tn, mu, sg = api.prepare_template(T, M, W)
+    with api.SearchPlan(V, T.shape, M, batch=4) as p:
+        p.set_template(tn, W, mu, sg)
+        sc, an, pos = p.refine(V, ga, cand)
+    for i, (x, y, z, g) in enumerate(cand):
+        vals = []
+        for a in range(A):
+            for q in range(27):
+                dx, dy, dz = q % 3 - 1, (q // 3) % 3 - 1, q // 9 - 1
+                px, py, pz = (x + dx) % N[0], (y + dy) % N[1], (z + dz) % N[2]
+                vals.append((allm[g * A + a][px, py, pz], a, (px, py, pz)))
+        best = max(v[0] for v in vals)
+        srt = sorted((v[0] for v in vals), reverse=True)
+        err = abs(sc[i] - best)
+        say(f"  cand {i}: gpu {sc[i]:.6f} fp64 {best:.6f} |d| {err:.2e}")
+        assert err < SCORE_TOL
+        if srt[0] - srt[1] > MARGIN:
+            ref = next(v for v in vals if v[0] == best)
+            assert an[i] == ref[1] and tuple(pos[i]) == ref[2]

@@ -1,0 +1,199 @@
+"""Learned-proposal U-Net: the CPU restatement is pinned (tap convention by a
+brute-force loop, losses by their closed forms, gradients by FP64 central
+finite differences), then the HIP network is compared with it.
+
+GPU tolerances (BF16 operands, FP32 accumulation, BF16 stored activations and
+pre-activation gradients, as in the restatement with bf16=True):
+  logits: max |d| <= 3e-2 * max |logit| and mean |d| <= 2e-3 * max |logit|;
+  gradients: per layer cosine >= 0.995 and relative L2 error <= 0.1;
+  losses: relative 1e-3."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import unet_ref as U
+
+NBIN = 64
+
+
+def _theta(rng, scale=1.0):
+    """He-scaled random parameters laid out as the HIP network stores them
+    (zero for the 15 padding input channels of the first layer and for the
+    unused head channels)."""
+    out = []
+    for li, (ci, co, t, _) in enumerate(U.SPEC):
+        fan = t * (1 if li == 0 else ci)
+        W = rng.standard_normal((t, co, ci)) * np.sqrt(2.0 / fan) * scale
+        if li == 0:
+            W[:, :, 1:] = 0
+        if t == 1:
+            W[:, 1 + NBIN:, :] = 0
+        b = rng.standard_normal(co) * 0.05
+        if t == 1:
+            b[1 + NBIN:] = 0
+        out += [W.reshape(-1), b]
+    return np.concatenate(out)
+
+
+def _targets(rng, shape):
+    tprob = (rng.random(shape) < 0.05).astype(np.float32)
+    tbin = np.where(tprob > 0, rng.integers(0, NBIN, shape), -1).astype(np.int32)
+    return tprob, tbin
+
+
+# ---------------------------------------------------------------- CPU pins
+def test_param_count():
+    assert U.num_params() == sum(t * co * ci + co for ci, co, t, _ in U.SPEC)
+
+
+def test_tap_convention_brute_force():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((4, 5, 6, 16))
+    W = rng.standard_normal((27, 16, 16))
+    b = rng.standard_normal(16)
+    ref = U.conv_direct(x, W, b, True)
+    w = torch.as_tensor(W).permute(1, 2, 0).reshape(16, 16, 3, 3, 3)
+    y = torch.nn.functional.conv3d(torch.as_tensor(x).permute(3, 0, 1, 2)[None], w, torch.as_tensor(b), padding=1)
+    y = torch.relu(y)[0].permute(1, 2, 3, 0).numpy()
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_losses_closed_form():
+    rng = np.random.default_rng(1)
+    L = torch.as_tensor(rng.standard_normal((3, 4, 5, 80)) * 3)
+    tprob, tbin = _targets(rng, (3, 4, 5))
+    tprob[0, 0, 0], tbin[0, 0, 0] = 1.0, 5
+    bce, ce = U.loss(L, tprob, tbin, NBIN, 7.0)
+    z = L[..., 0].numpy().ravel()
+    t = tprob.ravel()
+    p = 1 / (1 + np.exp(-z))
+    ref_bce = np.mean(-(7.0 * t * np.log(p) + (1 - t) * np.log(1 - p)))
+    Lb = L.numpy().reshape(-1, 80)[:, 1:1 + NBIN]
+    sel = tbin.ravel() >= 0
+    lse = np.log(np.exp(Lb[sel]).sum(1))
+    ref_ce = np.mean(lse - Lb[sel][np.arange(sel.sum()), tbin.ravel()[sel]])
+    assert abs(float(bce) - ref_bce) < 1e-12 and abs(float(ce) - ref_ce) < 1e-12
+
+
+def test_restatement_gradients_finite_differences():
+    """FP64 central differences (no bf16 rounding) at parameters of every layer."""
+    rng = np.random.default_rng(2)
+    shape = (8, 8, 8)
+    vol = rng.standard_normal(shape)
+    theta = _theta(rng)
+    tprob, tbin = _targets(rng, shape)
+    tprob[4, 4, 4], tbin[4, 4, 4] = 1.0, 3
+    _, _, g = U.gradients(vol, theta, tprob, tbin, NBIN, 5.0, bf16=False)
+
+    def L(th):
+        logits, _ = U.forward(vol, th, bf16=False)
+        b, c = U.loss(logits, tprob, tbin, NBIN, 5.0)
+        return float(b + c)
+
+    offs = np.cumsum([0] + [t * co * ci + co for ci, co, t, _ in U.SPEC])
+    idx = []
+    for li in range(len(U.SPEC)):
+        ci, co, t, _ = U.SPEC[li]
+        nw = t * co * ci
+        # a weight of a live input channel and the bias of channel 0
+        idx.append(offs[li] + (rng.integers(0, nw) if li else 0))
+        idx.append(offs[li] + nw)
+    eps = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        fd = (L(tp) - L(tm)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, (i, fd, g[i])
+
+
+def test_bf16_round_nearest_even():
+    v = torch.tensor([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 0.0], dtype=torch.float64)
+    r = U.bf16_round(v)
+    assert r.tolist() == [1.0, 1.0, 1.0 + 4 * 2 ** -8, -2.5, 0.0]
+
+
+# ---------------------------------------------------------------- GPU parity
+def _gpu_net(api, shape_zyx, theta):
+    nz, ny, nx = shape_zyx
+    net = api.Unet((nx, ny, nz), nbin=NBIN, seed=0, train=True)
+    net.set_params(theta.astype(np.float32))
+    return net
+
+
+def _as_f(a):
+    """(nz,ny,nx) C-order array -> F-order (nx,ny,nz) view of the same memory."""
+    return np.asfortranarray(np.ascontiguousarray(a).transpose(2, 1, 0))
+
+
+@pytest.mark.gpu
+def test_unet_forward_matches_restatement(api):
+    print("unet forward", flush=True)
+    rng = np.random.default_rng(10)
+    shape = (16, 16, 24)
+    vol = rng.standard_normal(shape).astype(np.float32)
+    theta = _theta(rng).astype(np.float32)
+    with _gpu_net(api, shape, theta) as net:
+        net.forward(_as_f(vol))
+        got = net.logits()
+        back = net.get_params()
+    np.testing.assert_array_equal(back, theta)
+    ref, _ = U.forward(vol, theta.astype(np.float64), bf16=True)
+    ref = ref.detach().numpy()
+    scale = np.abs(ref).max()
+    d = np.abs(got - ref)
+    print(f"  logits max|d|/max|ref| = {d.max() / scale:.2e}, mean = {d.mean() / scale:.2e}", flush=True)
+    assert d.max() <= 3e-2 * scale and d.mean() <= 2e-3 * scale
+
+
+@pytest.mark.gpu
+def test_unet_backward_matches_restatement(api):
+    print("unet backward", flush=True)
+    rng = np.random.default_rng(11)
+    shape = (16, 16, 24)
+    vol = rng.standard_normal(shape).astype(np.float32)
+    theta = _theta(rng).astype(np.float32)
+    tprob, tbin = _targets(rng, shape)
+    with _gpu_net(api, shape, theta) as net:
+        net.forward(_as_f(vol))
+        bce, ce = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
+        g = net.get_grads()
+    rb, rc, rg = U.gradients(vol, theta.astype(np.float64), tprob, tbin, NBIN, 4.0, bf16=True)
+    print(f"  loss gpu ({bce:.6f}, {ce:.6f}) ref ({rb:.6f}, {rc:.6f})", flush=True)
+    assert abs(bce - rb) <= 1e-3 * abs(rb) and abs(ce - rc) <= 1e-3 * abs(rc)
+    off = 0
+    for li, (ci, co, t, _) in enumerate(U.SPEC):
+        n = t * co * ci + co
+        a, b = g[off:off + n].astype(np.float64), rg[off:off + n]
+        off += n
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-300)
+        print(f"  layer {li}: cos {cos:.5f} rel {rel:.3e}", flush=True)
+        assert cos >= 0.995 and rel <= 0.1, li
+
+
+@pytest.mark.gpu
+def test_unet_adam_descends_and_infer_tiles(api):
+    """A few Adam steps on one volume lower the loss; tiled inference with a
+    tile as large as the volume equals predict() after forward() of the
+    globally normalised volume."""
+    print("unet adam + infer", flush=True)
+    rng = np.random.default_rng(12)
+    shape = (16, 16, 16)
+    vol = rng.standard_normal(shape).astype(np.float32)
+    tprob, tbin = _targets(rng, shape)
+    with api.Unet((16, 16, 16), nbin=NBIN, seed=3, train=True) as net:
+        losses = []
+        for _ in range(12):
+            net.forward(_as_f(vol))
+            b, c = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
+            losses.append(b + c)
+            net.adam(1e-3)
+        print(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+        assert losses[-1] < 0.9 * losses[0]
+        vn = (vol - vol.astype(np.float64).mean()) / vol.astype(np.float64).std()
+        net.forward(_as_f(vn.astype(np.float32)))
+        p0, b0 = net.predict()
+        p1, b1 = net.infer(_as_f(vol), halo=0)
+    np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-6)
+    assert (b1 == b0).mean() > 0.999

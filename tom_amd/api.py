@@ -286,6 +286,26 @@ class SearchPlan:
         self.ctx.check(self.ctx.lib.tomhip_search_finalize(self.h, _ptr(sc), _ptr(an)))
         return sc, an
 
+    def refine(self, volume, group_angles, cand):
+        """Exact FLCF around candidates: group_angles (A, G, 3) [phi psi theta]
+        (A orientations for each of G groups), cand (P, 4) int rows
+        x y z group (0-based). Returns (score[P], angle index[P] into the
+        group's list, position[P, 3] x y z 0-based)."""
+        vb = _Buf(volume)
+        _prep_ctx(self.ctx, vb)
+        ga = np.ascontiguousarray(np.asarray(group_angles, np.float32))
+        if ga.ndim != 3 or ga.shape[2] != 3:
+            raise ValueError("group_angles must be (A, G, 3)")
+        A, G = ga.shape[0], ga.shape[1]
+        c = np.ascontiguousarray(np.asarray(cand, np.int32).reshape(-1, 4))
+        P = c.shape[0]
+        sc = np.zeros(P, np.float32)
+        an = np.zeros(P, np.int32)
+        pos = np.zeros((P, 3), np.int32)
+        self.ctx.check(self.ctx.lib.tomhip_search_refine(self.h, vb.ptr, _p(ga), G, A, _p(c), P, _p(sc), _p(an),
+                                                         _p(pos)))
+        return sc, an, pos
+
     def set_graph(self, enable: bool):
         """Replay full angle batches through a captured hipGraph (default)."""
         self.ctx.check(self.ctx.lib.tomhip_search_set_graph(self.h, int(enable)))
@@ -500,3 +520,130 @@ def tom_oscar(vol, template, phi_start, phi_end, phi_incr, psi_start, psi_end, p
     maps = [np.where(never, -1, ang[safe, k]).astype(np.int16) for k in range(3)]
     ccf = np.where(never, -1000.0, ccf)
     return ccf, maps[0], maps[1], maps[2]
+
+
+# ---------------------------------------------------------------- learned proposal
+class Unet:
+    """3D U-Net proposal network (tomhip_unet_*): per-voxel particle
+    probability and coarse-orientation bin. `dims` = (nx, ny, nz) of the
+    network input (multiples of 4, >= 8); `train=False` allocates no gradient
+    buffers. Inputs may be numpy (F-order (nx,ny,nz)) or torch ((nz,ny,nx))."""
+
+    HEAD = 80
+
+    def __init__(self, dims, nbin: int = 64, seed: int = 0, train: bool = True, ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        self.dims = _dims3(dims)
+        self.nbin = int(nbin)
+        self.V = int(np.prod(self.dims))
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.tomhip_unet_create(self.ctx.h, _p(self.dims), self.nbin, int(seed), int(train),
+                                                       ctypes.byref(h)))
+        self.h = h
+
+    def num_params(self) -> int:
+        n = np.zeros(1, np.int64)
+        self.ctx.check(self.ctx.lib.tomhip_unet_num_params(self.h, _p(n)))
+        return int(n[0])
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), np.float32)
+        self.ctx.check(self.ctx.lib.tomhip_unet_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(np.asarray(theta, np.float32))
+        if t.size != self.num_params():
+            raise ValueError("parameter vector has the wrong length")
+        self.ctx.check(self.ctx.lib.tomhip_unet_set_params(self.h, _p(t)))
+
+    def forward(self, vol):
+        vb = _Buf(vol)
+        if tuple(int(x) for x in vb.dims) != tuple(int(x) for x in self.dims):
+            raise ValueError("volume does not match the network input size")
+        _prep_ctx(self.ctx, vb)
+        self.ctx.check(self.ctx.lib.tomhip_unet_forward(self.h, vb.ptr))
+
+    def logits(self) -> np.ndarray:
+        """float32 (nz, ny, nx, 80)."""
+        out = np.zeros((self.V, self.HEAD), np.float32)
+        self.ctx.check(self.ctx.lib.tomhip_unet_get_logits(self.h, _p(out)))
+        nx, ny, nz = (int(x) for x in self.dims)
+        return out.reshape(nz, ny, nx, self.HEAD)
+
+    def backward(self, tprob, tbin, wpos: float = 1.0):
+        """Gradients of BCE(tprob) + CE(tbin); returns (bce, ce)."""
+        pb, bb = _Buf(tprob), _Buf(tbin, np.int32)
+        _prep_ctx(self.ctx, pb, bb)
+        loss = np.zeros(2)
+        self.ctx.check(self.ctx.lib.tomhip_unet_backward(self.h, pb.ptr, bb.ptr, float(wpos), _p(loss)))
+        return float(loss[0]), float(loss[1])
+
+    def get_grads(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), np.float32)
+        self.ctx.check(self.ctx.lib.tomhip_unet_get_grads(self.h, _p(out)))
+        return out
+
+    def adam(self, lr: float):
+        self.ctx.check(self.ctx.lib.tomhip_unet_adam(self.h, float(lr)))
+
+    def predict(self, like=None):
+        """(prob, bin) of the last forward pass, shaped like the input."""
+        ref = _Buf(like) if like is not None else None
+        dims = tuple(int(x) for x in self.dims)
+        if ref is not None and ref.torch:
+            prob = torch.empty(tuple(reversed(dims)), dtype=torch.float32, device=ref.obj.device)
+            b = torch.empty(tuple(reversed(dims)), dtype=torch.int32, device=ref.obj.device)
+        else:
+            prob = np.zeros(dims, np.float32, order="F")
+            b = np.zeros(dims, np.int32, order="F")
+        self.ctx.check(self.ctx.lib.tomhip_unet_predict(self.h, _ptr(prob), _ptr(b)))
+        return prob, b
+
+    def infer(self, vol, halo: int = 24):
+        """Whole-volume (prob, bin) maps in overlapping tiles of the network
+        size; the volume is normalised by its global mean and std."""
+        vb = _Buf(vol)
+        ctx = _prep_ctx(self.ctx, vb)
+        N = _dims3(vb.dims)
+        prob = _empty_like(vb, vb.dims)
+        b = _empty_like(vb, vb.dims, np.int32)
+        ctx.check(ctx.lib.tomhip_unet_infer(self.h, vb.ptr, _p(N), int(halo), _ptr(prob), _ptr(b)))
+        return prob, b
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.tomhip_unet_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def local_maxima(peaks, radius: int, threshold: float, capacity: int = 1 << 20, ctx: Optional[Context] = None):
+    """Local maxima of a map above threshold (cube of half-edge radius, ties
+    to the lowest index): (P, 4) float32 rows x y z (0-based) value, sorted by
+    value (descending, then index). Raises if more than capacity are found."""
+    mb = _Buf(peaks)
+    ctx = _prep_ctx(ctx, mb)
+    N = _dims3(mb.dims)
+    out = np.zeros((int(capacity), 4), np.float32)
+    cnt = np.zeros(1, np.int64)
+    ctx.check(ctx.lib.tomhip_local_maxima_f32(ctx.h, mb.ptr, _p(N), int(radius), float(threshold), int(capacity),
+                                              _p(out), _p(cnt)))
+    k = int(cnt[0])
+    if k > capacity:
+        raise RuntimeError(f"{k} local maxima exceed the capacity {capacity}; raise the threshold or the capacity")
+    out = out[:k]
+    lin = (out[:, 2].astype(np.int64) * int(N[1]) + out[:, 1].astype(np.int64)) * int(N[0]) + out[:, 0].astype(np.int64)
+    order = np.lexsort((lin, -out[:, 3]))
+    return out[order]

@@ -28,7 +28,7 @@ COMMON = [
 ]
 # per-file extra flags: the rotation must round exactly like the reference C
 EXTRA = {"rotate.hip": ["-ffp-contract=off"]}
-SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "api.hip", "em_io.cpp"]
+SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "unet.hip", "api.hip", "em_io.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -11,6 +11,7 @@
 #include "picks.h"
 #include "search.h"
 #include "subtomo.h"
+#include "unet.h"
 #include "util.h"
 
 using namespace tomhip;
@@ -21,6 +22,12 @@ struct tomhip_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::unique_ptr<Twiddles> tw;
+};
+
+struct tomhip_unet {
+  tomhip_ctx* ctx = nullptr;
+  std::unique_ptr<Unet> u;
+  long V = 0;
 };
 
 struct tomhip_search {
@@ -429,6 +436,65 @@ void tomhip_search_destroy(tomhip_search* h) {
   delete h;
 }
 
+int tomhip_search_refine(tomhip_search* h, const float* volume, const float* euler, int G, int A,
+                         const int32_t* cand, int64_t P, float* score, int32_t* angle, int32_t* pos) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h && volume && euler && cand && score && angle && pos, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(G >= 1 && A >= 1 && P >= 1 && P < (1L << 30) && (long)G * A <= (1L << 24), ERR_WRONG_INPUT,
+                   "bad refinement sizes");
+    Search& S = *h->s;
+    hipStream_t s = ctx->stream;
+    std::vector<float> e((size_t)G * A * 3);
+    TOMHIP_CHECK(hipMemcpyAsync(e.data(), euler, sizeof(float) * e.size(), hipMemcpyDefault, s));
+    std::vector<int32_t> c((size_t)P * 4);
+    TOMHIP_CHECK(hipMemcpyAsync(c.data(), cand, sizeof(int32_t) * c.size(), hipMemcpyDefault, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < P; ++i) {
+      const int32_t* q = c.data() + 4 * i;
+      TOMHIP_REQUIRE(q[0] >= 0 && q[0] < S.X && q[1] >= 0 && q[1] < S.Y && q[2] >= 0 && q[2] < S.Z, ERR_WRONG_INPUT,
+                     "candidate outside the volume");
+      TOMHIP_REQUIRE(q[3] >= 0 && q[3] < G, ERR_WRONG_INPUT, "candidate group out of range");
+    }
+    std::vector<float> rm = rotmats(e.data(), (int64_t)G * A);  // [A][G][9]
+    DevBuf v(s, volume, sizeof(float) * S.N, true, false);
+    DevBuf dc(s, c.data(), sizeof(int32_t) * c.size(), true, false);
+    float* bs = nullptr;
+    int32_t *ba = nullptr, *bp = nullptr;
+    TOMHIP_CHECK(hipMalloc(&bs, sizeof(float) * P));
+    TOMHIP_CHECK(hipMalloc(&ba, sizeof(int32_t) * P));
+    TOMHIP_CHECK(hipMalloc(&bp, sizeof(int32_t) * P));
+    std::vector<float> hs(P);
+    std::vector<int32_t> ha(P), hp(P);
+    try {
+      S.refine(v.get<float>(), rm.data(), G, A, dc.get<int32_t>(), P, bs, ba, bp);
+      TOMHIP_CHECK(hipMemcpyAsync(hs.data(), bs, sizeof(float) * P, hipMemcpyDeviceToHost, s));
+      TOMHIP_CHECK(hipMemcpyAsync(ha.data(), ba, sizeof(int32_t) * P, hipMemcpyDeviceToHost, s));
+      TOMHIP_CHECK(hipMemcpyAsync(hp.data(), bp, sizeof(int32_t) * P, hipMemcpyDeviceToHost, s));
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(bs);
+      (void)hipFree(ba);
+      (void)hipFree(bp);
+      throw;
+    }
+    (void)hipFree(bs);
+    (void)hipFree(ba);
+    (void)hipFree(bp);
+    std::vector<int32_t> xyz((size_t)P * 3);
+    for (int64_t i = 0; i < P; ++i) {
+      const long k = hp[i] < 0 ? 0 : hp[i];
+      xyz[3 * i] = (int32_t)(k % S.X);
+      xyz[3 * i + 1] = (int32_t)((k / S.X) % S.Y);
+      xyz[3 * i + 2] = (int32_t)(k / ((long)S.X * S.Y));
+    }
+    TOMHIP_CHECK(hipMemcpyAsync(score, hs.data(), sizeof(float) * P, hipMemcpyDefault, s));
+    TOMHIP_CHECK(hipMemcpyAsync(angle, ha.data(), sizeof(int32_t) * P, hipMemcpyDefault, s));
+    TOMHIP_CHECK(hipMemcpyAsync(pos, xyz.data(), sizeof(int32_t) * xyz.size(), hipMemcpyDefault, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int tomhip_flcf_search_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
                            const int64_t n[3], const float* mask, const float* W, const float* angles, int64_t A,
                            int batch, float* score, int32_t* angle) {
@@ -580,6 +646,24 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
   });
 }
 
+int tomhip_local_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, float threshold,
+                            int64_t capacity, float* out, int64_t* count) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && out && count, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "map");
+    TOMHIP_REQUIRE(radius >= 0 && radius <= 64, ERR_WRONG_INPUT, "radius must be in [0, 64]");
+    TOMHIP_REQUIRE(capacity >= 1 && capacity <= (1L << 30), ERR_WRONG_INPUT, "bad capacity");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf m(s, map, sizeof(float) * NN, true, false);
+    DevBuf o(s, out, sizeof(float) * 4 * capacity, false, true);
+    *count = local_maxima(m.get<float>(), (int)N[0], (int)N[1], (int)N[2], radius, threshold, capacity,
+                          o.get<float>(), s);
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
                                const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
                                const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
@@ -683,6 +767,136 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
       throw;
     }
     for (void* p : tmp) (void)hipFree(p);
+  });
+}
+
+// ------------------------------------------------------------------ U-Net
+int tomhip_unet_create(tomhip_ctx* ctx, const int64_t dims[3], int nbin, uint64_t seed, int train,
+                       tomhip_unet** out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(dims, "U-Net input");
+    auto h = std::make_unique<tomhip_unet>();
+    h->ctx = ctx;
+    h->u = std::make_unique<Unet>(ctx->stream, (int)dims[2], (int)dims[1], (int)dims[0], nbin, seed, train != 0);
+    h->V = dims[0] * dims[1] * dims[2];
+    *out = h.release();
+  });
+}
+
+void tomhip_unet_destroy(tomhip_unet* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  delete h;
+}
+
+int tomhip_unet_num_params(tomhip_unet* h, int64_t* n) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && n, ERR_WRONG_INPUT, "null pointer");
+    *n = h->u->num_params();
+  });
+}
+
+int tomhip_unet_get_params(tomhip_unet* h, float* out) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && out, ERR_WRONG_INPUT, "null pointer");
+    h->u->params(out, true);
+  });
+}
+
+int tomhip_unet_set_params(tomhip_unet* h, const float* in) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && in, ERR_WRONG_INPUT, "null pointer");
+    std::vector<float> tmp(in, in + h->u->num_params());
+    h->u->params(tmp.data(), false);
+  });
+}
+
+int tomhip_unet_forward(tomhip_unet* h, const float* volume) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && volume, ERR_WRONG_INPUT, "null pointer");
+    hipStream_t s = h->ctx->stream;
+    DevBuf v(s, volume, sizeof(float) * h->V, true, false);
+    h->u->forward(v.get<float>());
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_unet_get_logits(tomhip_unet* h, float* out) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && out, ERR_WRONG_INPUT, "null pointer");
+    hipStream_t s = h->ctx->stream;
+    TOMHIP_CHECK(hipMemcpyAsync(out, h->u->logits(), sizeof(float) * 80 * h->V, hipMemcpyDefault, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, float wpos, double* loss) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && tprob && tbin, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(std::isfinite(wpos) && wpos > 0, ERR_WRONG_INPUT, "wpos must be positive");
+    hipStream_t s = h->ctx->stream;
+    DevBuf p(s, tprob, sizeof(float) * h->V, true, false);
+    DevBuf b(s, tbin, sizeof(int32_t) * h->V, true, false);
+    std::vector<int32_t> hb(h->V);
+    TOMHIP_CHECK(hipMemcpyAsync(hb.data(), b.get<int32_t>(), sizeof(int32_t) * h->V, hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    long npos = 0;
+    for (int32_t v : hb) npos += (v >= 0);
+    double l[2];
+    h->u->backward(p.get<float>(), b.get<int32_t>(), wpos, npos, l);
+    if (loss) {
+      loss[0] = l[0];
+      loss[1] = l[1];
+    }
+  });
+}
+
+int tomhip_unet_get_grads(tomhip_unet* h, float* out) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && out, ERR_WRONG_INPUT, "null pointer");
+    h->u->grads(out);
+  });
+}
+
+int tomhip_unet_adam(tomhip_unet* h, float lr) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(std::isfinite(lr) && lr > 0, ERR_WRONG_INPUT, "lr must be positive");
+    h->u->adam(lr);
+    TOMHIP_CHECK(hipStreamSynchronize(h->ctx->stream));
+  });
+}
+
+int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && (prob || bin), ERR_WRONG_INPUT, "null pointer");
+    hipStream_t s = h->ctx->stream;
+    DevBuf p(s, prob, sizeof(float) * h->V, false, true);
+    DevBuf b(s, bin, sizeof(int32_t) * h->V, false, true);
+    h->u->predict(prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr);
+    p.commit();
+    b.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_unet_infer(tomhip_unet* h, const float* volume, const int64_t N[3], int halo, float* prob,
+                      int32_t* bin) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && volume && (prob || bin), ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "volume");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = h->ctx->stream;
+    DevBuf v(s, volume, sizeof(float) * NN, true, false);
+    DevBuf p(s, prob, sizeof(float) * NN, false, true);
+    DevBuf b(s, bin, sizeof(int32_t) * NN, false, true);
+    h->u->infer(v.get<float>(), (int)N[0], (int)N[1], (int)N[2], halo, prob ? p.get<float>() : nullptr,
+                bin ? b.get<int32_t>() : nullptr);
+    p.commit();
+    b.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

@@ -182,6 +182,43 @@ __global__ void k_erase(float* a, int X, int Y, int Z, int r, PickState* st) {
   }
 }
 
+// Local maxima above a threshold: voxel v is kept when no voxel of the cube
+// of half-edge r around it (clipped to the volume) is larger, with ties going
+// to the lowest linear index. Survivors are appended as {x, y, z, value}.
+__global__ void k_local_max(const float* __restrict__ a, int X, int Y, int Z, int r, float thr, long cap,
+                            float4* out, unsigned long long* counter) {
+  const long n = (long)X * Y * Z;
+  for (long i = (long)blockIdx.x * NTP + threadIdx.x; i < n; i += (long)gridDim.x * NTP) {
+    const float v = a[i];
+    if (!(v > thr)) continue;
+    const int x = (int)(i % X), y = (int)((i / X) % Y), z = (int)(i / ((long)X * Y));
+    bool keep = true;
+    for (int dz = -r; dz <= r && keep; ++dz) {
+      const int zz = z + dz;
+      if (zz < 0 || zz >= Z) continue;
+      for (int dy = -r; dy <= r && keep; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= Y) continue;
+        const float* row = a + ((long)zz * Y + yy) * X;
+        const long base = ((long)zz * Y + yy) * X;
+        for (int dx = -r; dx <= r; ++dx) {
+          const int xx = x + dx;
+          if (xx < 0 || xx >= X) continue;
+          const float w = row[xx];
+          if (w > v || (w == v && base + xx < i)) {
+            keep = false;
+            break;
+          }
+        }
+      }
+    }
+    if (keep) {
+      const unsigned long long slot = atomicAdd(counter, 1ull);
+      if ((long)slot < cap) out[slot] = make_float4((float)x, (float)y, (float)z, v);
+    }
+  }
+}
+
 }  // namespace
 
 long picklist(const float* map, int X, int Y, int Z, const int32_t* angles, int tx, int ty, int tz, long n,
@@ -244,6 +281,28 @@ long picklist(const float* map, int X, int Y, int Z, const int32_t* angles, int 
   (void)hipFree(part);
   (void)hipFree(st);
   return count;
+}
+
+long local_maxima(const float* map, int X, int Y, int Z, int r, float thr, long cap, float* out, hipStream_t s) {
+  unsigned long long* counter = nullptr;
+  TOMHIP_CHECK(hipMalloc(&counter, sizeof(unsigned long long)));
+  unsigned long long total = 0;
+  try {
+    TOMHIP_CHECK(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s));
+    const long n = (long)X * Y * Z;
+    unsigned g = (unsigned)((n + NTP - 1) / NTP);
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_local_max, dim3(g), dim3(NTP), 0, s, map, X, Y, Z, r, thr, cap,
+                       reinterpret_cast<float4*>(out), counter);
+    TOMHIP_CHECK(hipGetLastError());
+    TOMHIP_CHECK(hipMemcpyAsync(&total, counter, sizeof(total), hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(counter);
+    throw;
+  }
+  (void)hipFree(counter);
+  return (long)total;
 }
 
 }  // namespace tomhip

@@ -43,6 +43,15 @@ class Search {
   // Writes score (float) and 0-based best-angle index maps (device or host).
   void finalize(float* score, int32_t* angle);
 
+  // Exact FLCF of a local orientation set around candidates (FLCF mode, after
+  // set_template): rm_host [A][G][9] holds A orientations for each of G
+  // groups; cand_dev (device) [P] of {x, y, z, group}; each candidate is
+  // scored at its position and 26 neighbours for the A orientations of its
+  // group. Outputs (device, [P]): best score, best orientation index (0..A-1,
+  // first wins), best voxel (linear index). volume: the searched volume.
+  void refine(const float* volume, const float* rm_host, int G, int A, const int32_t* cand_dev, long P,
+              float* bscore, int32_t* bangle, int32_t* bpos);
+
   void set_profile(bool on) { profile_ = on; }
   // Replay full angle batches through one captured hipGraph (default on;
   // profiling runs eager so every launch can be timed).
@@ -62,6 +71,8 @@ class Search {
  private:
   void precompute_stats(const float* volume);
   void run_batch(const float* rm_dev, int ka, long first_index, const long* desc);
+  void rotate_templates(const float* rm_dev, int count, float* tout, const long* desc, long rm_step);
+  void box_filter(int count, float* t, c32* work);
   void capture_graph();
   void drop_graph();
   template <class F>

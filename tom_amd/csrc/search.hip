@@ -29,6 +29,7 @@
 // unrotated filtered template, tom_os3_findTemplate.m:233-234) and sig_I > 0,
 // the score is a strictly increasing function of c at every voxel, so the
 // max/argmax runs on c and the normalisation is applied once in finalize().
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -500,6 +501,115 @@ __global__ void k_finalize_xcf(const float* maxc, const int32_t* argm, float* sc
   }
 }
 
+// ---------------------------------------------------------------- refinement
+// Exact FLCF of a local orientation set at candidate positions and their
+// neighbours (offsets in {-1, 0, 1}^3): c[k] = sum_u t[u] V[k + u + s - h]
+// (indices modulo the volume, as the FFT path wraps), then the score of
+// k_finalize_flcf. One block per candidate; thread rows (uy, uz) of the
+// template; along ux a sliding window of volume samples serves the three
+// x offsets of each of the 9 (y, z) offset rows.
+constexpr int NT_R = 256;
+
+__device__ __forceinline__ int wrapi(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
+template <int KS>
+__global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, int X, int Y, int Z,
+                                                 const float* __restrict__ t, int G, int nx, int ny, int nz, int sx,
+                                                 int sy, int sz, const int4* __restrict__ cand,
+                                                 const double* __restrict__ mu, const double* __restrict__ sig,
+                                                 double inv_m, double mu_t, double sig_t, int a0, int nvalid,
+                                                 float* bscore, int32_t* bangle, int32_t* bpos) {
+  const int4 c = cand[blockIdx.x];
+  const long n3 = (long)nx * ny * nz;
+  float acc[KS][27];
+#pragma unroll
+  for (int k = 0; k < KS; ++k)
+#pragma unroll
+    for (int q = 0; q < 27; ++q) acc[k][q] = 0.f;
+  const int rows = ny * nz;
+  for (int row = threadIdx.x; row < rows; row += NT_R) {
+    const int uy = row % ny, uz = row / ny;
+    const float* vr[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int yy = wrapi(c.y + (r % 3 - 1) + uy + sy, Y);
+      const int zz = wrapi(c.z + (r / 3 - 1) + uz + sz, Z);
+      vr[r] = V + ((long)zz * Y + yy) * X;
+    }
+    int xm = wrapi(c.x - 1 + sx, X);
+    int x0 = xm + 1 == X ? 0 : xm + 1;
+    int xi = x0 + 1 == X ? 0 : x0 + 1;
+    float w0[9], w1[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      w0[r] = vr[r][xm];
+      w1[r] = vr[r][x0];
+    }
+    const float* tr = t + (long)c.w * n3 + (long)row * nx;
+    for (int ux = 0; ux < nx; ++ux) {
+      float tv[KS];
+#pragma unroll
+      for (int k = 0; k < KS; ++k) tv[k] = tr[(long)k * G * n3 + ux];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const float w2 = vr[r][xi];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+          acc[k][3 * r] += tv[k] * w0[r];
+          acc[k][3 * r + 1] += tv[k] * w1[r];
+          acc[k][3 * r + 2] += tv[k] * w2;
+        }
+        w0[r] = w1[r];
+        w1[r] = w2;
+      }
+      xi = xi + 1 == X ? 0 : xi + 1;
+    }
+  }
+  __shared__ float red[NT_R / 64][KS * 27];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < KS; ++k)
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      float v = acc[k][q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wv][k * 27 + q] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float best = bscore[blockIdx.x];
+    int32_t ba = bangle[blockIdx.x], bp = bpos[blockIdx.x];
+    for (int k = 0; k < nvalid; ++k)
+      for (int q = 0; q < 27; ++q) {
+        float cc = 0.f;
+        for (int w = 0; w < NT_R / 64; ++w) cc += red[w][k * 27 + q];
+        const int x = wrapi(c.x + q % 3 - 1, X), y = wrapi(c.y + (q / 3) % 3 - 1, Y), z = wrapi(c.z + q / 9 - 1, Z);
+        const long idx = ((long)z * Y + y) * X + x;
+        const double den = sig_t * sig[idx];
+        const float sc = den > 0 ? (float)(((double)cc * inv_m - mu[idx] * mu_t) / den) : 0.f;
+        if (sc > best) {
+          best = sc;
+          ba = a0 + k;
+          bp = (int32_t)idx;
+        }
+      }
+    bscore[blockIdx.x] = best;
+    bangle[blockIdx.x] = ba;
+    bpos[blockIdx.x] = bp;
+  }
+}
+
+__global__ void k_refine_init(float* s, int32_t* a, int32_t* p, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    s[i] = -FLT_MAX;
+    a[i] = -1;
+    p[i] = -1;
+  }
+}
+
 // Half-lengths X/2 the c2r pass supports (X a multiple of 32).
 template <class F>
 void dispatch_xh(int L, F&& f) {
@@ -746,6 +856,8 @@ void Search::run(const float* rm_host, long count, long first_index) {
 void Search::capture_graph() {
   drop_graph();
   if (!desc_) TOMHIP_CHECK(hipMalloc(&desc_, sizeof(long) * 2));
+  // twiddle tables are allocated on first use, which a capture does not allow
+  for (int L : {nx, ny, nz, X, Y, Z, Xh}) (void)tw_.f32(L);
   hipGraph_t g = nullptr;
   TOMHIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
   try {
@@ -770,59 +882,62 @@ void Search::drop_graph() {
   graph_ = nullptr;
 }
 
-void Search::run_batch(const float* rm_dev, int ka, long first_index, const long* desc) {
+void Search::rotate_templates(const float* rm_dev, int count, float* tout, const long* desc, long rm_step) {
   const long n3 = (long)nx * ny * nz;
   const float px = (float)((nx + 1) / 2), py = (float)((ny + 1) / 2), pz = (float)((nz + 1) / 2);
-  // rotate; FLCF: times M (or M^2 when no weight follows, since the masked
-  // template is masked again inside tom_os3_corr)
+  // FLCF: times M (or M^2 when no weight follows, since the masked template
+  // is masked again inside tom_os3_corr)
   const float* rmask = nullptr;
   if (mode == MODE_FLCF) rmask = (wsh_ || !remask_) ? mask_ : mask_ + n3;
-  timed(K_ROT, [&] {
-    launch_rotate(tmpl_, t1_, rm_dev, ka, rmask, nx, ny, nz, px, py, pz, n3, s_, desc, 9L * ka);
-  });
+  launch_rotate(tmpl_, tout, rm_dev, count, rmask, nx, ny, nz, px, py, pz, n3, s_, desc, rm_step);
+}
 
-  if (wsh_) {
-    timed(K_BOX, [&] {
-      const long nl = (long)ka * ny * nz;
-      dispatch_len(nx, [&](auto Lc) {
-        constexpr int L = decltype(Lc)::value;
-        constexpr int NL = contig_lines<float, L>();
-        hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, t1_, tbox_, nl,
-                           tw_.f32(L));
-      });
-      TOMHIP_CHECK(hipGetLastError());
-      const long XY = (long)nx * ny;
-      dispatch_len(ny, [&](auto Lc) {
-        constexpr int L = decltype(Lc)::value;
-        constexpr int TW = strided_tw<float, L>();
-        hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, nz, ka), dim3(NT), 0, s_,
-                           tbox_, nx, (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
-      });
-      TOMHIP_CHECK(hipGetLastError());
-      dispatch_len(nz, [&](auto Lc) {
-        constexpr int L = decltype(Lc)::value;
-        constexpr int TW = strided_tw<float, L>();
-        hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, ny, ka), dim3(NT), 0, s_,
-                           tbox_, nx, XY, (long)nx, n3, tw_.f32(L), (const float*)wsh_);
-      });
-      TOMHIP_CHECK(hipGetLastError());
-      dispatch_len(ny, [&](auto Lc) {
-        constexpr int L = decltype(Lc)::value;
-        constexpr int TW = strided_tw<float, L>();
-        hipLaunchKernelGGL((k_c2c_strided<float, L, 1>), dim3((nx + TW - 1) / TW, nz, ka), dim3(NT), 0, s_,
-                           tbox_, nx, (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
-      });
-      TOMHIP_CHECK(hipGetLastError());
-      dispatch_len(nx, [&](auto Lc) {
-        constexpr int L = decltype(Lc)::value;
-        constexpr int NL = contig_lines<float, L>();
-        hipLaunchKernelGGL(k_box_x_inv<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_,
-                           (const c32*)tbox_, t1_, mask_, nl, (long)ny * nz, (float)(1.0 / (double)n3),
-                           tw_.f32(L));
-      });
-      TOMHIP_CHECK(hipGetLastError());
-    });
-  }
+void Search::box_filter(int count, float* t, c32* work) {
+  // t <- M .* ifftn(fftn(t) .* W) per template (tom_os3_findTemplate.m:264-267
+  // followed by the masking inside tom_os3_corr)
+  const long n3 = (long)nx * ny * nz;
+  const long nl = (long)count * ny * nz;
+  dispatch_len(nx, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int NL = contig_lines<float, L>();
+    hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const float*)t, work,
+                       nl, tw_.f32(L));
+  });
+  TOMHIP_CHECK(hipGetLastError());
+  const long XY = (long)nx * ny;
+  dispatch_len(ny, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int TW = strided_tw<float, L>();
+    hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, nz, count), dim3(NT), 0, s_, work, nx,
+                       (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
+  });
+  TOMHIP_CHECK(hipGetLastError());
+  dispatch_len(nz, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int TW = strided_tw<float, L>();
+    hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, ny, count), dim3(NT), 0, s_, work, nx,
+                       XY, (long)nx, n3, tw_.f32(L), (const float*)wsh_);
+  });
+  TOMHIP_CHECK(hipGetLastError());
+  dispatch_len(ny, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int TW = strided_tw<float, L>();
+    hipLaunchKernelGGL((k_c2c_strided<float, L, 1>), dim3((nx + TW - 1) / TW, nz, count), dim3(NT), 0, s_, work, nx,
+                       (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
+  });
+  TOMHIP_CHECK(hipGetLastError());
+  dispatch_len(nx, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int NL = contig_lines<float, L>();
+    hipLaunchKernelGGL(k_box_x_inv<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const c32*)work, t,
+                       mask_, nl, (long)ny * nz, (float)(1.0 / (double)n3), tw_.f32(L));
+  });
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+void Search::run_batch(const float* rm_dev, int ka, long first_index, const long* desc) {
+  timed(K_ROT, [&] { rotate_templates(rm_dev, ka, t1_, desc, 9L * ka); });
+  if (wsh_) timed(K_BOX, [&] { box_filter(ka, t1_, tbox_); });
 
   timed(K_TFWD, [&] {
     dispatch_len(Y, [&](auto Lc) {
@@ -878,6 +993,50 @@ void Search::finalize(float* score, int32_t* angle) {
                        (const int32_t*)argm_, score, angle, X, Y, Z, 1.0 / (double)N);
   }
   TOMHIP_CHECK(hipGetLastError());
+}
+
+void Search::refine(const float* volume, const float* rm_host, int G, int A, const int32_t* cand_dev, long P,
+                    float* bscore, int32_t* bangle, int32_t* bpos) {
+  TOMHIP_REQUIRE(mode == MODE_FLCF, ERR_WRONG_INPUT, "refinement needs an FLCF plan");
+  TOMHIP_REQUIRE(started_, ERR_WRONG_INPUT, "set the template before refining");
+  TOMHIP_REQUIRE(G >= 1 && A >= 1 && P >= 1, ERR_WRONG_INPUT, "empty refinement");
+  TOMHIP_REQUIRE((long)G * A <= (1L << 24) && P <= (1L << 30), ERR_WRONG_INPUT, "refinement too large");
+  constexpr int KS = 2;
+  const long n3 = (long)nx * ny * nz;
+  const long ntp = (long)KS * G;
+  float* rm = nullptr;
+  float* tt = nullptr;
+  c32* work = nullptr;
+  auto release = [&] {
+    (void)hipFree(rm);
+    (void)hipFree(tt);
+    (void)hipFree(work);
+  };
+  try {
+    TOMHIP_CHECK(hipMalloc(&rm, sizeof(float) * 9 * (long)G * A));
+    TOMHIP_CHECK(hipMalloc(&tt, sizeof(float) * n3 * ntp));
+    if (wsh_) TOMHIP_CHECK(hipMalloc(&work, sizeof(c32) * n3 * ntp));
+    TOMHIP_CHECK(hipMemcpyAsync(rm, rm_host, sizeof(float) * 9 * (long)G * A, hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(k_refine_init, dim3(grid_for(P)), dim3(NT), 0, s_, bscore, bangle, bpos, P);
+    TOMHIP_CHECK(hipGetLastError());
+    // per-axis s - h of c[k] = sum_u t[u] V[k + u + s - h]
+    const int sx = 2 * (X / 2) - X - nx / 2, sy = 2 * (Y / 2) - Y - ny / 2, sz = 2 * (Z / 2) - Z - nz / 2;
+    const double inv_m = 1.0 / m;
+    for (int a0 = 0; a0 < A; a0 += KS) {
+      const int cnt = std::min(KS, A - a0);
+      rotate_templates(rm + 9L * a0 * G, cnt * G, tt, nullptr, 0);
+      if (wsh_) box_filter(cnt * G, tt, work);
+      hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P), dim3(NT_R), 0, s_, volume, X, Y, Z, (const float*)tt, G,
+                         nx, ny, nz, sx, sy, sz, reinterpret_cast<const int4*>(cand_dev), (const double*)mu_,
+                         (const double*)sig_, inv_m, mu_t, sig_t, a0, cnt, bscore, bangle, bpos);
+      TOMHIP_CHECK(hipGetLastError());
+    }
+    TOMHIP_CHECK(hipStreamSynchronize(s_));
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
 }
 
 }  // namespace tomhip

@@ -1,0 +1,55 @@
+// Learned proposal network (host side of unet.hip): a 3-level 3D U-Net on
+// BF16 MFMA convolutions with FP32 accumulation and FP32 master weights.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <memory>
+
+namespace tomhip {
+
+// Channels-last implicit-GEMM 3D convolution (zero padding, stride 1).
+// X: bf16 [D][H][W][xs], channels [xo, xo + Cin); Wt: bf16 [taps][Cout][Cin]
+// (taps 27 = 3x3x3 with tap t = 9(dz+1) + 3(dy+1) + (dx+1), or 1);
+// output Yb (bf16, stride ys, offset yo) and/or Yf (f32 [V][Cout]).
+void conv_forward(const uint16_t* X, int xs, int xo, int Cin, int D, int H, int W, const uint16_t* Wt,
+                  const float* bias, int Cout, int taps, int relu, uint16_t* Yb, int ys, int yo, float* Yf,
+                  hipStream_t s);
+// dW[t][co][ci] += sum_v dY[v][co] X[v + d_t][ci]; dY bf16 [V][Cout].
+void conv_weight_grad(const uint16_t* X, int xs, int xo, int Cin, const uint16_t* dY, int Cout, int D, int H, int W,
+                      int taps, float* dW, hipStream_t s);
+
+class Unet {
+ public:
+  // D, H, W: input edges (multiples of 4); nbin orientation bins (<= 79).
+  Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool train);
+  ~Unet();
+  long num_params() const;
+  // all parameters in layer order, each layer W [taps][Cout][Cin] then bias
+  void params(float* host, bool get);
+  void grads(float* host);
+  // vol: device f32 [D][H][W] (x fastest), already normalised
+  void forward(const float* vol);
+  double forward_flops() const;
+  // tprob: device f32 [V] particle targets in [0, 1]; tbin: device int32 [V]
+  // bin index or -1; npos = #{tbin >= 0}. loss_out (host, optional) gets
+  // {mean BCE, mean CE}.
+  void backward(const float* tprob, const int32_t* tbin, float wpos, long npos, double* loss_out);
+  void adam(float lr);
+  void predict(float* prob, int32_t* bin);
+  // whole volume vol (device f32 X x Y x Z), normalised here by its global
+  // mean and standard deviation, in tiles of the network size with halo
+  // voxels of context on each side; prob / bin (device) receive every voxel
+  void infer(const float* vol, int X, int Y, int Z, int halo, float* prob, int32_t* bin);
+  const float* logits() const;  // device f32 [V][80]
+  int D() const;
+  int H() const;
+  int W() const;
+
+ private:
+  void refresh_weights();
+  struct Impl;
+  std::unique_ptr<Impl> d_;
+};
+
+}  // namespace tomhip

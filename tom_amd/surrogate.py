@@ -1,0 +1,182 @@
+"""Learned-proposal template matching (no reference counterpart): a 3D U-Net
+proposes particle positions and one of 64 coarse orientation bins; the exact
+FLCF (the same score as the exhaustive search) then refines each candidate over
++-15 degrees in 5-degree steps on each Euler angle around its bin centre and
+over the 27 voxels around its position.
+
+Training labels come from the exhaustive search itself: peaks of the FLCF map
+(tom_os3_returnPicklist semantics) above a score threshold mark particles, and
+the best orientation there gives the bin.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import api, orient
+
+
+# ---------------------------------------------------------------- labels
+def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_peaks: int = 100000):
+    """Picklist (tom_os3_returnPicklist: border erase, tom_peakc radius
+    ceil(n/2)) of an exhaustive score map, keeping rows with value > threshold.
+    Returns (P, 5) rows x y z (0-based) value angle-index."""
+    pk = api.tom_os3_returnPicklist(score, (template_size,) * 3, max_peaks, angles=angle, erase_borders=True)
+    pk = np.asarray(pk, np.float64).reshape(-1, 5)
+    pk = pk[pk[:, 3] > threshold]
+    pk[:, :3] -= 1
+    return pk
+
+
+def label_volumes(shape_zyx, peaks, angles, bins, radius: float = 2.0):
+    """Targets for the proposal network: tprob = 1 within `radius` of each
+    peak, tbin = bin of the peak's best orientation there (-1 elsewhere).
+    peaks: (P, 5) rows x y z value angle-index; angles: the search's list."""
+    tprob = np.zeros(shape_zyx, np.float32)
+    tbin = np.full(shape_zyx, -1, np.int32)
+    if len(peaks) == 0:
+        return tprob, tbin
+    b = orient.assign_bins(angles[peaks[:, 4].astype(np.int64)], bins)
+    r = int(np.ceil(radius))
+    off = np.array([(dz, dy, dx) for dz in range(-r, r + 1) for dy in range(-r, r + 1) for dx in range(-r, r + 1)
+                    if dx * dx + dy * dy + dz * dz <= radius * radius])
+    Z, Y, X = shape_zyx
+    for (x, y, z), bb in zip(peaks[:, :3].astype(np.int64), b):
+        zz, yy, xx = z + off[:, 0], y + off[:, 1], x + off[:, 2]
+        ok = (zz >= 0) & (zz < Z) & (yy >= 0) & (yy < Y) & (xx >= 0) & (xx < X)
+        tprob[zz[ok], yy[ok], xx[ok]] = 1.0
+        tbin[zz[ok], yy[ok], xx[ok]] = bb
+    return tprob, tbin
+
+
+# ---------------------------------------------------------------- training
+class Trainer:
+    """Adam on random crops (edge `crop`, a multiple of 4) of labelled
+    tomograms. Volumes are normalised by their global mean and std, as
+    Unet.infer does."""
+
+    def __init__(self, crop=64, nbin=64, seed=0, lr=1e-3, wpos=20.0):
+        self.crop = int(crop)
+        self.net = api.Unet((crop, crop, crop), nbin=nbin, seed=seed, train=True)
+        self.rng = np.random.default_rng(seed)
+        self.lr = lr
+        self.wpos = wpos
+        self.data = []
+        self.history = []
+
+    def add(self, volume_zyx, tprob, tbin):
+        """volume_zyx: numpy (z, y, x) float32; targets of the same shape."""
+        v = np.asarray(volume_zyx, np.float64)
+        vn = ((v - v.mean()) / v.std()).astype(np.float32)
+        pos = np.argwhere(tprob > 0)
+        self.data.append((vn, tprob, tbin, pos))
+
+    def _crop(self):
+        vn, tp, tb, pos = self.data[self.rng.integers(len(self.data))]
+        c = self.crop
+        Z, Y, X = vn.shape
+        if len(pos) and self.rng.random() < 0.7:
+            z, y, x = pos[self.rng.integers(len(pos))] - self.rng.integers(4, c - 4, 3)
+        else:
+            z, y, x = self.rng.integers(0, Z - c + 1), self.rng.integers(0, Y - c + 1), self.rng.integers(0, X - c + 1)
+        z, y, x = (int(np.clip(z, 0, Z - c)), int(np.clip(y, 0, Y - c)), int(np.clip(x, 0, X - c)))
+        sl = (slice(z, z + c), slice(y, y + c), slice(x, x + c))
+        f = lambda a: np.asfortranarray(np.ascontiguousarray(a[sl]).transpose(2, 1, 0))  # noqa: E731
+        return f(vn), f(tp), f(tb)
+
+    def step(self):
+        v, tp, tb = self._crop()
+        self.net.forward(v)
+        bce, ce = self.net.backward(tp, tb, self.wpos)
+        self.net.adam(self.lr)
+        self.history.append((bce, ce))
+        return bce, ce
+
+    def train(self, steps, log=None, every=100):
+        t0 = time.perf_counter()
+        for i in range(steps):
+            bce, ce = self.step()
+            if log and (i % every == 0 or i == steps - 1):
+                log(f"[train] step {i}: bce {bce:.4f} ce {ce:.4f} ({time.perf_counter() - t0:.1f} s)")
+        return time.perf_counter() - t0
+
+
+# ---------------------------------------------------------------- proposal + refinement
+def greedy_nms(points, values, radius, k):
+    """Indices of at most k points taken in descending value order, skipping
+    any within `radius` of one already taken."""
+    from scipy.spatial import cKDTree
+    order = np.argsort(-values, kind="stable")
+    tree = cKDTree(points)
+    taken = []
+    dead = np.zeros(len(points), bool)
+    for i in order:
+        if dead[i]:
+            continue
+        taken.append(i)
+        if len(taken) >= k:
+            break
+        dead[tree.query_ball_point(points[i], radius - 1e-9)] = True
+    return np.asarray(taken, np.int64)
+
+
+class Surrogate:
+    """Proposal network (inference tiles of edge `tile`) + exact refinement."""
+
+    def __init__(self, params, bins, tile=128, halo=24, half=15.0, step=5.0, nbin=64):
+        self.net = api.Unet((tile, tile, tile), nbin=nbin, seed=0, train=False)
+        self.net.set_params(params)
+        self.bins = np.asarray(bins, np.float64)
+        self.halo = halo
+        self.half, self.step = half, step
+        self.timings = {}
+
+    def propose(self, volume, K=2000, nms_radius=12.0, threshold=0.05):
+        """Top-K candidates (K, 5): x y z bin prob."""
+        t0 = time.perf_counter()
+        prob, b = self.net.infer(volume, self.halo)
+        _sync(prob)
+        t1 = time.perf_counter()
+        lm = api.local_maxima(prob, 2, threshold)
+        t2 = time.perf_counter()
+        keep = greedy_nms(lm[:, :3], lm[:, 3], nms_radius, K) if len(lm) else np.zeros(0, np.int64)
+        lm = lm[keep]
+        bh = b.cpu().numpy() if hasattr(b, "cpu") else np.asarray(b)
+        xyz = lm[:, :3].astype(np.int64)
+        if hasattr(b, "cpu"):  # torch (z, y, x)
+            bins = bh[xyz[:, 2], xyz[:, 1], xyz[:, 0]]
+        else:  # numpy (x, y, z)
+            bins = bh[xyz[:, 0], xyz[:, 1], xyz[:, 2]]
+        t3 = time.perf_counter()
+        self.timings.update(infer_s=t1 - t0, local_max_s=t2 - t1, nms_s=t3 - t2)
+        return np.concatenate([lm[:, :3], bins[:, None], lm[:, 3:4]], 1)
+
+    def refine(self, plan, volume, cand):
+        """Exact FLCF over the local grid of each candidate's bin and its 27
+        voxels. Returns (score[P], euler[P, 3], pos[P, 3] x y z)."""
+        t0 = time.perf_counter()
+        ub, grp = np.unique(cand[:, 3].astype(np.int64), return_inverse=True)
+        grids = np.stack([orient.local_grid(self.bins[b], self.half, self.step) for b in ub], 1)  # (A, G, 3)
+        c4 = np.concatenate([cand[:, :3].astype(np.int32), grp[:, None].astype(np.int32)], 1)
+        sc, ai, pos = plan.refine(volume, grids, c4)
+        eul = grids[ai, grp]
+        self.timings.update(refine_s=time.perf_counter() - t0)
+        return sc, eul, pos
+
+
+def _sync(t):
+    if hasattr(t, "device") and getattr(t.device, "type", "") == "cuda":
+        import torch
+        torch.cuda.synchronize()
+
+
+def recall(ref_xyz, got_xyz, dist=3.0):
+    """Fraction of reference positions with a found position within dist."""
+    if len(ref_xyz) == 0:
+        return 1.0
+    if len(got_xyz) == 0:
+        return 0.0
+    from scipy.spatial import cKDTree
+    d, _ = cKDTree(np.asarray(got_xyz, np.float64)).query(np.asarray(ref_xyz, np.float64))
+    return float(np.mean(d <= dist))
