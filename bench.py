@@ -4,7 +4,8 @@
 Workload (default, "c2"): a 512x512x256 float32 tomogram holding 300 copies of a
 48^3 phantom (12 Gaussian blobs, sigma 2, within radius 18) at seeded positions
 (min spacing 40) and uniform random orientations, plus N(0,1) noise at SNR 0.1;
-missing wedge +-60 deg about Y (tom_wedge semi-angle 30 deg) applied at box size;
+missing wedge of a +-60 deg tilt about Y in the data and in the template weight
+(tom_wedge semi-angle 30 deg, applied at box size);
 binary spherical mask radius 22; Euler grid 15 deg: 24 x 24 x 13 = 7,488
 orientations. A step is one launch group of `--batch` orientations scanned over
 the whole tomogram; the default times every group of the full angle set.
@@ -32,16 +33,21 @@ sys.path.insert(0, ROOT)
 METRIC = "FLCF voxel-orientations/s; surrogate vs plain wall-time at >=99% peak recall"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch group measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (fetch scaled x2.000, write x1.000 by a 1 GiB device copy in the
+# same runs); profiles/flcf_c2_hbm_traffic.md has the commands and counts.
+MEASURED_TRAFFIC = {("c2", 4): {"passA": 1.5514e9, "passB": 2.1476e9, "passC": 1.6414e9}}
+
 CONFIGS = {
     # name: volume (X,Y,Z), template edge, blobs, sigma, radius, particles,
     #       min spacing, SNR, wedge semi-angle, mask radius, angle step, seed
     "c1": dict(N=(64, 64, 64), n=24, k=6, sigma=1.5, R=8, P=5, spacing=24, snr=0.5, wedge=30.0,
                mask=("gauss", 10.0, 2.0), bp=(2.0, 20.0), step=(30, 30, 30), seed=1,
                desc="C1 plain template matching 64^3, 24^3 template, 30 deg (1,008 orientations)"),
-    "c2": dict(N=(512, 512, 256), n=48, k=12, sigma=2.0, R=18, P=300, spacing=40, snr=0.1, wedge=30.0,
+    "c2": dict(N=(512, 512, 256), n=48, k=12, sigma=2.0, R=18, P=300, spacing=40, snr=0.1, wedge=30.0, tilt=60.0,
                mask=("binary", 22.0, 0.0), bp=None, step=(15, 15, 15), seed=2,
                desc="C2 plain exhaustive FLCF 512x512x256, 48^3 template, 15 deg (7,488 orientations)"),
-    "c5": dict(N=(1024, 1024, 384), n=48, k=12, sigma=2.0, R=18, P=2000, spacing=48, snr=0.05, wedge=40.0,
+    "c5": dict(N=(1024, 1024, 384), n=48, k=12, sigma=2.0, R=18, P=2000, spacing=48, snr=0.05, wedge=40.0, tilt=50.0,
                mask=("binary", 22.0, 0.0), bp=None, step=(10, 10, 10), seed=5,
                desc="C5 exhaustive FLCF 1024x1024x384, 48^3 template, 10 deg (24,624 orientations)"),
 }
@@ -64,7 +70,7 @@ def make_tomogram(cfg, api, torch, dev):
     rng = np.random.default_rng(cfg["seed"])
     T = synth.phantom(rng, cfg["n"], cfg["k"], cfg["sigma"], cfg["R"])
     V, pos, eul, _ = synth.make_tomogram(cfg["N"], [T], cfg["P"], cfg["spacing"], cfg["snr"], rng, cfg["seed"],
-                                         api, torch, dev)
+                                         api, torch, dev, tilt_max=cfg.get("tilt"))
     return V, T, pos, eul
 
 
@@ -117,6 +123,40 @@ def cpu_baseline(cfg, V_host, T, M, W, angles, k, workers):
     return N * k / dt, t_pre, dt, maps
 
 
+def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exhaustive_s):
+    """Proposal network + exact local refinement on the benchmark tomogram;
+    recall of the exhaustive search's picklist peaks above 0.3."""
+    from tom_amd import surrogate as S
+    w = np.load(wpath, allow_pickle=False)
+    sg = S.Surrogate(w["params"], w["bins"])
+    ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
+    # warm (network allocation, code objects)
+    sg.propose(V, K=16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cand = sg.propose(V, K=args.topk)
+    sc, eul, pos = sg.refine(plan, V, cand)
+    torch.cuda.synchronize()
+    t_sur = time.perf_counter() - t0
+    rec_any = S.recall(ref[:, :3], pos)
+    rec_hit = S.recall(ref[:, :3], pos[sc > 0.3])
+    # exhaustive best score at the matched peaks vs the refined score there
+    log(f"[bench] surrogate: {len(cand)} candidates, {t_sur:.3f} s (exhaustive {exhaustive_s:.3f} s); "
+        f"recall of {len(ref)} exhaustive peaks > 0.3: {rec_any:.4f} (any candidate), "
+        f"{rec_hit:.4f} (refined score > 0.3); timings {sg.timings}")
+    meta = json.loads(str(w["meta"]))
+    return {"weights": os.path.relpath(wpath, ROOT), "candidates": int(len(cand)), "topk": args.topk,
+            "wall_s": round(t_sur, 4), "exhaustive_wall_s": round(exhaustive_s, 4),
+            "speedup": round(exhaustive_s / t_sur, 2), "exhaustive_peaks_gt_0.3": int(len(ref)),
+            "recall_any_candidate": rec_any, "recall_refined_gt_0.3": rec_hit,
+            "timings_s": {k: round(v, 4) for k, v in sg.timings.items()},
+            "unet_infer": {"flops": sg.infer_flops, "tflops_per_s": round(sg.infer_flops / sg.timings["infer_s"] / 1e12, 2),
+                           "frac_of_bf16_dense_peak": round(sg.infer_flops / sg.timings["infer_s"] / 2.5e15, 4)},
+            "refine_orientations_per_candidate": 343,
+            "training": {k: meta[k] for k in ("tomos", "size", "particles", "steps", "crop", "label_s", "train_s")
+                         if k in meta}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +168,10 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--no-profile", action="store_true", help="do not record per-kernel events")
     ap.add_argument("--calib-copy", action="store_true", help="run one 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--surrogate", default=os.path.join(ROOT, "tom_amd", "weights", "unet_c2.npz"),
+                    help="trained proposal-network weights (skipped when absent)")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--topk", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -217,6 +261,14 @@ def main():
     sc = score.cpu().numpy().transpose(2, 1, 0)
     found = float(np.mean([sc[tuple(p)] > 0.2 for p in pos])) if ntimed == A else None
 
+    # ---- learned proposal on the same tomogram, judged by the exhaustive run
+    sur = None
+    wpath = args.surrogate
+    if wpath and os.path.exists(wpath) and ntimed == A and not args.no_surrogate:
+        sur = surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath,
+                            exhaustive_s=(pre_ms / 1e3 + (graph["ms_per_step"] * groups / 1e3 if graph else dt)
+                                          + fin_ms / 1e3))
+
     # ---- roofline of the dominant kernel (event times over the timed region)
     names = ["rotate", "box_weight", "tmpl_fwd", "passA", "passB", "passC"]
     per_kernel = {names[i]: {"ms_total": round(float(ms_k[i]), 3), "launches": int(cnt_k[i]),
@@ -231,7 +283,8 @@ def main():
         avg_s = ms_k[i] / cnt_k[i] / 1e3
         ach = model[dom] / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.config, KA), {}).get(dom),
                 "bytes_per_launch": int(model[dom]), "avg_launch_ms": round(avg_s * 1e3, 4),
                 "batch_full": full}
 
@@ -279,7 +332,7 @@ def main():
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
                    "timed_path": "eager launches with per-kernel HIP events",
-                   "graph_replay": graph},
+                   "graph_replay": graph, "surrogate": sur},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

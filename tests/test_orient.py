@@ -1,0 +1,68 @@
+"""Orientation helpers and the 64 coarse bins of the learned proposal.
+
+Pins: rotmatrix equals the reference-convention matrix (oracle.rotmat, the
+restatement of tom_rotatec's rotation matrix, stored in column order) and
+tom_amd.api's tom_sum_rotation restatement; Euler and quaternion round trips;
+bin invariants (unit quaternions, deterministic, near-uniform nearest-neighbour
+spacing, covering radius well below that of random rotations)."""
+import numpy as np
+
+from tom_amd import orient as O
+from tom_amd.api import _rotmatrix
+
+
+def _rand_euler(rng, k):
+    return np.stack([rng.uniform(0, 360, k), rng.uniform(0, 360, k), rng.uniform(0, 180, k)], 1)
+
+
+def test_rotmatrix_matches_reference_conventions(oracle):
+    rng = np.random.default_rng(0)
+    for e in _rand_euler(rng, 50):
+        R = O.rotmatrix(e)
+        np.testing.assert_allclose(R, _rotmatrix(*e), atol=1e-14)
+        np.testing.assert_allclose(R.ravel(order="F"), oracle.rotmat(e.astype(np.float32)), atol=2e-6)
+
+
+def test_round_trips():
+    rng = np.random.default_rng(1)
+    e = _rand_euler(rng, 500)
+    e[:5, 2] = [0, 180, 0, 180, 90]  # degenerate theta
+    R = O.rotmatrix(e)
+    np.testing.assert_allclose(O.rotmatrix(O.euler_from_matrix(R)), R, atol=1e-12)
+    q = O.quat_from_matrix(R)
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1, atol=1e-14)
+    np.testing.assert_allclose(O.matrix_from_quat(q), R, atol=1e-12)
+    # distance: a rotation about z by 30 degrees is 30 degrees away
+    a = O.quat_from_matrix(O.rotmatrix([10, 0, 0]))
+    b = O.quat_from_matrix(O.rotmatrix([40, 0, 0]))
+    assert abs(O.rotation_distance_deg(a, b) - 30) < 1e-9
+
+
+def test_bins_invariants():
+    b = O.orientation_bins(64, 0)
+    assert b.shape == (64, 3)
+    np.testing.assert_array_equal(b, O.orientation_bins(64, 0))
+    q = O.quat_from_matrix(O.rotmatrix(b))
+    D = O.rotation_distance_deg(q[:, None], q[None])
+    np.fill_diagonal(D, np.inf)
+    nn = D.min(1)
+    assert nn.min() / nn.max() > 0.75
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal((50000, 4))
+    r /= np.linalg.norm(r, axis=1, keepdims=True)
+    cover = np.rad2deg(2 * np.arccos(np.clip(np.abs(r @ q.T).max(1), 0, 1)))
+    qr = rng.standard_normal((64, 4))
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    cover_rand = np.rad2deg(2 * np.arccos(np.clip(np.abs(r @ qr.T).max(1), 0, 1)))
+    assert cover.max() < 52 and cover.max() < 0.8 * cover_rand.max()
+    # every bin centre is its own nearest bin
+    np.testing.assert_array_equal(O.assign_bins(b, b), np.arange(64))
+
+
+def test_local_grid():
+    g = O.local_grid([350, 10, 90], 15, 5)
+    assert g.shape == (343, 3)
+    assert np.any(np.all(np.isclose(g, [350, 10, 90]), 1))
+    assert g[:, :2].min() >= 0 and g[:, :2].max() < 360
+    # phi outermost, theta innermost
+    assert g[1, 2] - g[0, 2] == 5 and g[0, 0] == g[48, 0]

@@ -45,24 +45,28 @@ struct Dims {
 };
 
 // ---------------------------------------------------------------- conv
-// grid: x = ceil(W/16) * H * D row segments, y = Cout / 64 (rounded up);
-// block = 4 waves, wave w handles output channels 64*y + 16*w .. +15.
+// Each wave computes 16 consecutive x voxels of one (y, z) row for all Cout
+// = 16 CT output channels: one A fragment (input, 8 bytes per lane) feeds CT
+// MFMAs. A block's 4 waves take 4 consecutive y rows, so neighbouring waves
+// share most of their input rows in L1. grid: x = ceil(W/16) ceil(H/4) D.
 // X: bf16 rows of stride xs starting at channel offset xo; Cin multiple of 16.
 // Wt: bf16 [taps][Cout][Cin]. Output: bf16 (ys, yo) and/or f32 [V][Cout].
+template <int CT>
 __global__ __launch_bounds__(NTU) void k_conv(const uint16_t* __restrict__ X, int xs, int xo, int Cin, Dims d,
                                               const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
-                                              int Cout, int taps, int relu, uint16_t* Yb, int ys, int yo,
-                                              float* Yf) {
+                                              int taps, int relu, uint16_t* Yb, int ys, int yo, float* Yf) {
+  constexpr int Cout = 16 * CT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int co0 = blockIdx.y * 64 + wv * 16;
-  if (co0 >= Cout) return;  // whole wave leaves; no block barrier below
-  const int nseg = (d.W + 15) / 16;
+  const int nseg = (d.W + 15) / 16, nyb = (d.H + 3) / 4;
   const int seg = blockIdx.x % nseg;
-  const int y = (blockIdx.x / nseg) % d.H;
-  const int z = blockIdx.x / (nseg * d.H);
+  const int y = ((blockIdx.x / nseg) % nyb) * 4 + wv;
+  const int z = blockIdx.x / (nseg * nyb);
+  if (y >= d.H) return;  // whole wave leaves; no block barrier below
   const int x0 = seg * 16;
   const int row = lane & 15, kq = lane >> 4;  // A row (voxel) / B col (cout); k quad
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  f32x4 acc[CT];
+#pragma unroll
+  for (int j = 0; j < CT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int KS = Cin / 16;
   for (int t = 0; t < taps; ++t) {
     int dz = 0, dy = 0, dx = 0;
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(NTU) void k_conv(const uint16_t* __restrict__ X, in
     const int zz = z + dz, yy = y + dy, xx = x0 + row + dx;
     const bool ok = zz >= 0 && zz < d.D && yy >= 0 && yy < d.H && xx >= 0 && xx < d.W && (x0 + row) < d.W;
     const uint16_t* xp = X + (((long)zz * d.H + yy) * d.W + xx) * xs + xo;
-    const uint16_t* wp = Wt + ((long)t * Cout + co0 + row) * Cin;
+    const uint16_t* wp = Wt + ((long)t * Cout + row) * Cin;
     for (int ks = 0; ks < KS; ++ks) {
       const int c = ks * 16 + kq * 4;
       bf16x4 a = {0, 0, 0, 0};
@@ -85,27 +89,33 @@ __global__ __launch_bounds__(NTU) void k_conv(const uint16_t* __restrict__ X, in
         a[2] = (short)(raw.y & 0xffff);
         a[3] = (short)(raw.y >> 16);
       }
-      const uint2 rw = *reinterpret_cast<const uint2*>(wp + c);
-      bf16x4 b;
-      b[0] = (short)(rw.x & 0xffff);
-      b[1] = (short)(rw.x >> 16);
-      b[2] = (short)(rw.y & 0xffff);
-      b[3] = (short)(rw.y >> 16);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < CT; ++j) {
+        const uint2 rw = *reinterpret_cast<const uint2*>(wp + (long)j * 16 * Cin + c);
+        bf16x4 b;
+        b[0] = (short)(rw.x & 0xffff);
+        b[1] = (short)(rw.x >> 16);
+        b[2] = (short)(rw.y & 0xffff);
+        b[3] = (short)(rw.y >> 16);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+      }
     }
   }
-  // D[row = 4*kq + i][col = row(lane)] : voxel x0 + 4*kq + i, channel co0 + (lane & 15)
-  const int co = co0 + row;
-  const float bb = bias ? bias[co] : 0.f;
+  // D[row = 4*kq + i][col = lane & 15]: voxel x0 + 4*kq + i, channel 16 j + (lane & 15)
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int x = x0 + 4 * kq + i;
-    if (x >= d.W) continue;
-    float v = acc[i] + bb;
-    if (relu) v = fmaxf(v, 0.f);
-    const long vox = ((long)z * d.H + y) * d.W + x;
-    if (Yb) Yb[vox * ys + yo + co] = f2bf(v);
-    if (Yf) Yf[vox * Cout + co] = v;
+  for (int j = 0; j < CT; ++j) {
+    const int co = j * 16 + row;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = x0 + 4 * kq + i;
+      if (x >= d.W) continue;
+      float v = acc[j][i] + bb;
+      if (relu) v = fmaxf(v, 0.f);
+      const long vox = ((long)z * d.H + y) * d.W + x;
+      if (Yb) Yb[vox * ys + yo + co] = f2bf(v);
+      if (Yf) Yf[vox * Cout + co] = v;
+    }
   }
 }
 
@@ -449,12 +459,24 @@ inline unsigned gridv(long n) {
 void conv_forward(const uint16_t* X, int xs, int xo, int Cin, int D, int H, int W, const uint16_t* Wt,
                   const float* bias, int Cout, int taps, int relu, uint16_t* Yb, int ys, int yo, float* Yf,
                   hipStream_t s) {
-  TOMHIP_REQUIRE(Cin % 16 == 0 && Cout % 16 == 0 && (taps == 27 || taps == 1), ERR_UNSUPPORTED,
-                 "conv: channels must be multiples of 16, taps 27 or 1");
+  TOMHIP_REQUIRE(Cin % 16 == 0 && Cout % 16 == 0 && Cout >= 16 && Cout <= 96 && (taps == 27 || taps == 1),
+                 ERR_UNSUPPORTED, "conv: channels must be multiples of 16 (Cout <= 96), taps 27 or 1");
   TOMHIP_REQUIRE(xo % 4 == 0 && xs % 4 == 0, ERR_UNSUPPORTED, "conv: input channel offset/stride must be multiples of 4");
   Dims d{D, H, W};
-  dim3 g((unsigned)(((W + 15) / 16) * H * D), (unsigned)((Cout + 63) / 64));
-  hipLaunchKernelGGL(k_conv, g, dim3(NTU), 0, s, X, xs, xo, Cin, d, Wt, bias, Cout, taps, relu, Yb, ys, yo, Yf);
+  dim3 g((unsigned)(((W + 15) / 16) * ((H + 3) / 4) * D));
+  switch (Cout / 16) {
+#define TOMHIP_CONV_CASE(ct)                                                                                     \
+  case ct:                                                                                                    \
+    hipLaunchKernelGGL(k_conv<ct>, g, dim3(NTU), 0, s, X, xs, xo, Cin, d, Wt, bias, taps, relu, Yb, ys, yo, Yf); \
+    break;
+    TOMHIP_CONV_CASE(1)
+    TOMHIP_CONV_CASE(2)
+    TOMHIP_CONV_CASE(3)
+    TOMHIP_CONV_CASE(4)
+    TOMHIP_CONV_CASE(5)
+    TOMHIP_CONV_CASE(6)
+#undef TOMHIP_CONV_CASE
+  }
   TOMHIP_CHECK(hipGetLastError());
 }
 

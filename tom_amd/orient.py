@@ -38,9 +38,9 @@ def euler_from_matrix(R) -> np.ndarray:
     st = np.sin(t)
     gen = st > 1e-9
     f = np.where(gen, np.arctan2(R[..., 2, 0], R[..., 2, 1]), 0.0)
-    # degenerate theta (0 or 180): only phi + psi (or psi - phi) is defined
-    p_deg = np.where(R[..., 2, 2] > 0, np.arctan2(R[..., 1, 0], R[..., 0, 0]),
-                     np.arctan2(-R[..., 1, 0], -R[..., 0, 0]))
+    # degenerate theta (0 or 180): only psi + phi (or psi - phi) is defined;
+    # with phi = 0 both cases give psi = atan2(R10, R00)
+    p_deg = np.arctan2(R[..., 1, 0], R[..., 0, 0])
     p = np.where(gen, np.arctan2(R[..., 0, 2], -R[..., 1, 2]), p_deg)
     out = np.stack([np.rad2deg(f), np.rad2deg(p), np.rad2deg(t)], -1)
     out[..., :2] = np.mod(out[..., :2], 360.0)

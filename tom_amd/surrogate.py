@@ -17,8 +17,20 @@ import numpy as np
 from . import api, orient
 
 
+# layers of the proposal U-Net: (Cin, Cout, taps, resolution level)
+UNET_LAYERS = [(16, 16, 27, 0), (16, 16, 27, 0), (16, 32, 27, 1), (32, 32, 27, 1), (32, 64, 27, 2), (64, 64, 27, 2),
+               (96, 32, 27, 1), (32, 32, 27, 1), (48, 16, 27, 0), (16, 16, 27, 0), (16, 80, 1, 0)]
+
+
+def unet_forward_flops(dims) -> float:
+    """MFMA flops of one forward pass over a volume of `dims` voxels (the
+    first layer counts its 15 zero padding input channels, as it computes them)."""
+    V = float(np.prod(dims))
+    return sum(2.0 * V / 8 ** lvl * t * ci * co for ci, co, t, lvl in UNET_LAYERS)
+
+
 # ---------------------------------------------------------------- labels
-def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_peaks: int = 100000):
+def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_peaks: int = 1000):
     """Picklist (tom_os3_returnPicklist: border erase, tom_peakc radius
     ceil(n/2)) of an exhaustive score map, keeping rows with value > threshold.
     Returns (P, 5) rows x y z (0-based) value angle-index."""
@@ -122,20 +134,39 @@ def greedy_nms(points, values, radius, k):
 
 
 class Surrogate:
-    """Proposal network (inference tiles of edge `tile`) + exact refinement."""
+    """Proposal network + exact refinement. Volumes of up to `whole_max`
+    voxels are inferred in one tile (no seams); larger ones in cubic tiles of
+    edge `tile` with `halo` voxels of context."""
 
-    def __init__(self, params, bins, tile=128, halo=24, half=15.0, step=5.0, nbin=64):
-        self.net = api.Unet((tile, tile, tile), nbin=nbin, seed=0, train=False)
-        self.net.set_params(params)
+    def __init__(self, params, bins, tile=304, halo=24, half=15.0, step=5.0, nbin=64, whole_max=160_000_000):
+        self.params = np.asarray(params, np.float32)
+        self.nbin = nbin
         self.bins = np.asarray(bins, np.float64)
-        self.halo = halo
+        self.tile, self.halo, self.whole_max = tile, halo, whole_max
         self.half, self.step = half, step
         self.timings = {}
+        self._net, self._net_dims = None, None
+
+    def _network(self, dims):
+        dims = tuple(int(d) for d in dims)
+        if int(np.prod(dims)) <= self.whole_max:
+            want, halo = tuple(-(-d // 4) * 4 for d in dims), 0
+        else:
+            want, halo = (self.tile,) * 3, self.halo
+        if self._net_dims != want:
+            if self._net is not None:
+                self._net.close()
+            self._net = api.Unet(want, nbin=self.nbin, seed=0, train=False)
+            self._net.set_params(self.params)
+            self._net_dims = want
+        return self._net, halo
 
     def propose(self, volume, K=2000, nms_radius=12.0, threshold=0.05):
         """Top-K candidates (K, 5): x y z bin prob."""
+        dims = tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume)
+        net, halo = self._network(dims)
         t0 = time.perf_counter()
-        prob, b = self.net.infer(volume, self.halo)
+        prob, b = net.infer(volume, halo)
         _sync(prob)
         t1 = time.perf_counter()
         lm = api.local_maxima(prob, 2, threshold)
@@ -150,6 +181,7 @@ class Surrogate:
             bins = bh[xyz[:, 0], xyz[:, 1], xyz[:, 2]]
         t3 = time.perf_counter()
         self.timings.update(infer_s=t1 - t0, local_max_s=t2 - t1, nms_s=t3 - t2)
+        self.infer_flops = unet_forward_flops(net.dims) * _tiles(dims, net.dims, halo)
         return np.concatenate([lm[:, :3], bins[:, None], lm[:, 3:4]], 1)
 
     def refine(self, plan, volume, cand):
@@ -163,6 +195,11 @@ class Surrogate:
         eul = grids[ai, grp]
         self.timings.update(refine_s=time.perf_counter() - t0)
         return sc, eul, pos
+
+
+def _tiles(dims, tdims, halo):
+    core = [int(t) - 2 * halo for t in tdims]
+    return int(np.prod([-(-int(d) // c) for d, c in zip(dims, core)]))
 
 
 def _sync(t):

@@ -1,0 +1,107 @@
+"""Train the proposal U-Net on tomograms labelled by the exhaustive search.
+
+    python -m tom_amd.train_surrogate --out tom_amd/weights/unet_c2.npz
+
+Training data: seeded synthetic tomograms (edge --size, --particles copies of
+the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
+missing wedge of a +-60 degree tilt series), each searched exhaustively on the
+GPU at 15 degrees; the picklist of the score map (tom_os3_returnPicklist
+semantics) above --label-threshold gives the particle targets and the best
+orientation there the bin targets. The template, mask and wedge weight are
+those of the 512x512x256 / 48^3 benchmark workload (seed 2), so the trained
+network proposes candidates for that search.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload_template():
+    """Template, mask, weight and angles of the 48^3 benchmark search."""
+    from . import api, synth
+    rng = np.random.default_rng(2)
+    T = synth.phantom(rng, 48, 12, 2.0, 18)
+    M = api.tom_spheremask_gauss((48, 48, 48), 22.0, 0.0)
+    W = api.weight((48, 48, 48), 30.0, 0.0, 0.0)
+    phi = np.arange(0, 360, 15.0)
+    the = np.arange(0, 180 + 1e-9, 15.0)
+    angles = np.asarray([(a, b, c) for a in phi for b in phi for c in the], np.float32)
+    return T, M, W, angles
+
+
+def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, seed, thr, bins):
+    from . import surrogate, synth
+    rng = np.random.default_rng(seed)
+    V, pos, eul, _ = synth.make_tomogram((size, size, size), [T], P, 40, snr, rng, seed, api, torch, dev, tilt_max=60.0)
+    with api.SearchPlan(V, T.shape, M, batch=4) as plan:
+        plan.set_template(tn, W, mu, sg)
+        plan.run(angles, 0)
+        score, ang = plan.finalize()
+    peaks = surrogate.exhaustive_peaks(score, ang, T.shape[0], thr)
+    vz = V.cpu().numpy()
+    tprob, tbin = surrogate.label_volumes(vz.shape, peaks, angles, bins)
+    return vz, tprob, tbin, peaks, pos
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "unet_c2.npz"))
+    ap.add_argument("--tomos", type=int, default=24)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--particles", type=int, default=80)
+    ap.add_argument("--snr-lo", type=float, default=0.05)
+    ap.add_argument("--snr-hi", type=float, default=0.3)
+    ap.add_argument("--label-threshold", type=float, default=0.2)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--crop", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--wpos", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=100)
+    args = ap.parse_args(argv)
+
+    import torch
+
+    from . import api, orient, surrogate
+    if not torch.cuda.is_available():
+        raise SystemExit("training needs a GPU")
+    dev = torch.device("cuda:0")
+    T, M, W, angles = workload_template()
+    bins = orient.orientation_bins(64, 0)
+    tn, mu, sg = api.prepare_template(T, M, W)
+    tr = surrogate.Trainer(crop=args.crop, nbin=64, seed=args.seed, lr=args.lr, wpos=args.wpos)
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    npk = []
+    for i in range(args.tomos):
+        snr = float(rng.uniform(args.snr_lo, args.snr_hi))
+        vz, tp, tb, peaks, pos = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
+                                                args.particles, snr, args.seed + i, args.label_threshold, bins)
+        tr.add(vz, tp, tb)
+        npk.append(len(peaks))
+        log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold} "
+            f"({time.perf_counter() - t0:.1f} s)")
+        torch.cuda.empty_cache()
+    t_label = time.perf_counter() - t0
+    t_train = tr.train(args.steps, log=log, every=250)
+    theta = tr.net.get_params()
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    meta = dict(vars(args), label_s=t_label, train_s=t_train, peaks_per_tomogram=npk,
+                final_loss=[float(np.mean([h[0] for h in tr.history[-100:]])),
+                            float(np.mean([h[1] for h in tr.history[-100:]]))])
+    np.savez_compressed(args.out, params=theta, bins=bins, meta=json.dumps(meta))
+    log(f"[train] saved {args.out}: label {t_label:.1f} s, train {t_train:.1f} s")
+    print(json.dumps(meta))
+
+
+if __name__ == "__main__":
+    main()
