@@ -157,13 +157,124 @@ def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exh
                          if k in meta}}
 
 
+C3 = dict(P=4096, edges=(32, 48, 64), A=27, step=5.0, seed=3,
+          desc="C3 batched subtomogram scoring: 4,096 particles, edges {32,48,64}, 27 local orientations each")
+
+
+def make_c3(api, torch, dev, P, seed):
+    """Particles of uneven edges, each a rotated, shifted (+-4) copy of the
+    phantom of its box size, missing wedge of its own tilt range
+    (U(45, 70) degrees), SNR U(0.05, 0.5); a 3x3x3 local grid of 5-degree
+    steps around a prior within 4 degrees of the true orientation."""
+    from tom_amd import synth
+    rng = np.random.default_rng(seed)
+    edges = C3["edges"]
+    boxes = []
+    for B in edges:
+        T = synth.phantom(rng, B, 12, 2.0 * B / 48, 18.0 * B / 48)
+        M = api.tom_spheremask_gauss((B, B, B), B / 2 - 4, 2.0)
+        Mcc = api.tom_spheremask_gauss((B, B, B), 5.0, 0.0)
+        boxes.append({"template": T, "mask": M, "mask_cc": np.fft.ifftshift(Mcc)})
+    pbox = rng.integers(0, len(edges), P).astype(np.int32)
+    true = synth.random_euler(rng, P)
+    shift = rng.integers(-4, 5, (P, 3))
+    snr = rng.uniform(0.05, 0.5, P)
+    tilt = rng.uniform(45.0, 70.0, P)
+    prior = true + rng.uniform(-4, 4, (P, 3)).astype(np.float32)
+    off = np.array([-C3["step"], 0, C3["step"]])
+    grids = np.stack([[(a[0] + i, a[1] + j, a[2] + k) for i in off for j in off for k in off] for a in prior])
+    sizes = np.array([edges[b] ** 3 for b in pbox])
+    packed = torch.empty(int(sizes.sum()), dtype=torch.float32, device=dev)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    for bi, B in enumerate(edges):
+        sel = np.nonzero(pbox == bi)[0]
+        Td = torch.from_numpy(np.ascontiguousarray(boxes[bi]["template"].transpose(2, 1, 0))).to(dev)
+        for s0 in range(0, sel.size, 512):
+            part = sel[s0:s0 + 512]
+            rot = api.tom_rotate_batch(Td, true[part])
+            for j, p in enumerate(part):
+                v = torch.roll(rot[j], shifts=(int(shift[p, 2]), int(shift[p, 1]), int(shift[p, 0])), dims=(0, 1, 2))
+                v = synth.wedge_filter(v, float(tilt[p]), torch)
+                v = v * float(np.sqrt(snr[p] / float(v.double().var())))
+                v = v + torch.randn(v.shape, generator=g, device=dev)
+                packed[offs[p]:offs[p] + sizes[p]] = v.reshape(-1)
+            del rot
+    semi = (90.0 - tilt).astype(np.float32)
+    return boxes, packed, pbox, semi, grids.astype(np.float32), offs, sizes
+
+
+def run_c3(args, api, torch, dev):
+    log(f"[bench] {C3['desc']}: generating particles")
+    boxes, packed, pbox, semi, grids, offs, sizes = make_c3(api, torch, dev, C3["P"], C3["seed"])
+    torch.cuda.synchronize()
+    P, A = C3["P"], C3["A"]
+    for _ in range(max(1, args.warmup)):
+        api.subtomo_score_batch(boxes, packed, pbox, semi, grids)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(max(1, args.steps if args.steps > 0 else 3)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        best, peak, ccc = api.subtomo_score_batch(boxes, packed, pbox, semi, grids)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3))
+    wall = min(t[0] for t in times)
+    value = P * A / wall
+    edges = np.array([C3["edges"][b] for b in pbox], np.float64)
+    flops = float(np.sum(4 * 2.5 * edges ** 3 * np.log2(edges ** 3))) * A
+    log(f"[bench] {P} particles x {A} orientations: {wall * 1e3:.1f} ms -> {value:.4e} particle-orientations/s "
+        f"({flops / wall / 1e12:.2f} TFLOP/s FFT-model)")
+    # parity + CPU baseline on a bounded sample of particles
+    from oracle import tom_oracle as O
+    k = args.cpu_particles
+    idx = np.linspace(0, P - 1, k).round().astype(int) if k > 0 else np.zeros(0, int)
+    host = packed.cpu().numpy()
+    t0 = time.perf_counter()
+    err, mism = 0.0, 0
+    for p in idx:
+        B = C3["edges"][pbox[p]]
+        v = host[offs[p]:offs[p] + sizes[p]].reshape(B, B, B).transpose(2, 1, 0)
+        bx = boxes[pbox[p]]
+        i, pos, val, vals = O.subtomo_score(v, bx["template"], bx["mask"], bx["mask_cc"], float(semi[p]),
+                                            grids[p].astype(np.float64))
+        err = max(err, abs(float(ccc[p]) - val))
+        srt = np.sort(vals)
+        if srt[-1] - srt[-2] > 1e-4 and (best[p] != i or list(peak[p]) != list(pos)):
+            mism += 1
+    dt_cpu = time.perf_counter() - t0
+    cpu = None
+    if k > 0:
+        cpu = {"value": k * A / dt_cpu, "unit": "particle-orientations/s", "cores": 1, "kind": "port",
+               "sample": f"{k} of the {P} particles (all {A} orientations each), FP64 oracle corr3d restatement, "
+                         f"numpy/pocketfft single thread: {dt_cpu:.1f} s"}
+        log(f"[bench] oracle sample of {k} particles: max |ccc - fp64| {err:.2e}, decision mismatches {mism}")
+    out = {"metric": "subtomogram particle-orientations/s", "value": value, "unit": "particle-orientations/s",
+           "n_gpus": 1, "steps": len(times), "warmup": args.warmup, "ms_per_step": wall * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": C3["desc"], "particles": P, "orientations": A,
+                      "box_counts": {str(e): int(np.sum(pbox == i)) for i, e in enumerate(C3["edges"])},
+                      "event_ms": min(t[1] for t in times) * 1e3, "fft_model_tflop": flops / 1e12,
+                      "fft_model_tflops_per_s": flops / wall / 1e12,
+                      "max_abs_err_vs_fp64_sample": err, "decision_mismatches_sample": mism},
+           "roofline": {"bound": "valu", "achieved": flops / wall / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": flops / wall / 1e12 / 157.3, "traffic": None},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="launch groups to time (0 = the full angle set)")
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--batch", default="4", help="orientations per launch group, or 'auto' (autotuner)")
+    ap.add_argument("--tiles", default="16,16", help="column tiles of the z- and y-FFT passes (8/16/32)")
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--cpu-particles", type=int, default=6, help="c3: particles in the oracle sample")
     ap.add_argument("--cpu-angles", type=int, default=4, help="orientations in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--no-profile", action="store_true", help="do not record per-kernel events")
@@ -188,12 +299,15 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     dev = torch.device("cuda:0")
+    if args.config == "c3":
+        return run_c3(args, api, torch, dev)
     cfg = CONFIGS[args.config]
     X, Y, Z = cfg["N"]
     N = X * Y * Z
     angles = angle_grid(cfg["step"])
     A = angles.shape[0]
-    KA = args.batch
+    KA = 4 if args.batch == "auto" else int(args.batch)
+    tiles = tuple(int(v) for v in args.tiles.split(","))
     groups = (A + KA - 1) // KA
     steps = args.steps if args.steps > 0 else groups
     steps = min(steps, groups)
@@ -214,8 +328,18 @@ def main():
     # ---- one-off precompute: F_I + FP64 local statistics
     tn, mu_t, sig_t = api.prepare_template(T, M, W)
     torch.cuda.synchronize()
+    tuned = None
+    if args.batch == "auto":
+        from tom_amd import autotune
+        tuned = autotune.tune_search(V, tn, M, W, mu_t, sig_t, angles, log=log)
+        KA = tuned["batch"]
+        tiles = (tuned["tile_z"], tuned["tile_y"])
+        groups = (A + KA - 1) // KA
+        steps = min(args.steps if args.steps > 0 else groups, groups)
+        log(f"[bench] autotuned batch {KA}")
     t0 = time.perf_counter()
     plan = api.SearchPlan(V, (cfg["n"],) * 3, M, batch=KA)
+    plan.set_tiles(*tiles)
     torch.cuda.synchronize()
     pre_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] precompute (F_I + FP64 local stats) {pre_ms:.1f} ms")
@@ -328,11 +452,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": cfg["desc"], "volume": list(cfg["N"]), "template": cfg["n"],
-                   "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA,
+                   "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA, "tiles_zy": list(tiles),
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
                    "timed_path": "eager launches with per-kernel HIP events",
-                   "graph_replay": graph, "surrogate": sur},
+                   "graph_replay": graph, "autotune": tuned, "surrogate": sur},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

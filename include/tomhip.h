@@ -112,6 +112,10 @@ int tomhip_search_profile(tomhip_search* s, int enable, double* ms, int64_t* cou
 /* Replay full angle batches through one captured hipGraph (1, default) or
  * launch every kernel eagerly (0). Results are identical. */
 int tomhip_search_set_graph(tomhip_search* s, int enable);
+/* Column-tile widths (8, 16 or 32; default 16) of the z-FFT pass and the
+ * y-FFT pass: workgroup shape only, results are identical. No reference
+ * counterpart; set by the autotuner. */
+int tomhip_search_set_tiles(tomhip_search* s, int tile_z, int tile_y);
 void tomhip_search_destroy(tomhip_search* s);
 
 /* ---------------------------------------------------------------- one-shot drivers */
@@ -175,6 +179,16 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
  * Needs an FLCF plan with its template set. volume: the searched volume. */
 int tomhip_search_refine(tomhip_search* h, const float* volume, const float* euler, int G, int A,
                          const int32_t* cand, int64_t P, float* score, int32_t* angle, int32_t* pos);
+
+/* ---------------------------------------------------------------- secondary scores */
+/* Peak-to-sidelobe ratio map of a correlation map (os3_psr,
+ * Oscar3/c++Picker/src/tom/os3/os3_correlation.cpp:183-262; MATLAB
+ * tom_os3_analyzePeak.m:61-103 'PSR'): psr = (xcf - mean) / std with the local
+ * mean and std of xcf under a ring of the template's size n (binary sphere of
+ * radius n/2 minus the sphere of radius max(n/4, 5)), FP64 statistics, 0 where
+ * the std is 0. The second-order score (os3_soc, :164-171) is tomhip_os3_corr_f32
+ * of the xcf map with the template's autocorrelation. */
+int tomhip_psr_f32(tomhip_ctx* ctx, const float* xcf, const int64_t N[3], const int64_t n[3], float* out);
 
 /* ---------------------------------------------------------------- learned proposal */
 /* No counterpart in the reference: a 3-level 3D U-Net (16/32/64 channels, two

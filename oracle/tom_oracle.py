@@ -38,6 +38,18 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
                         tom_peak.m:27-50 (value and spline position)
   find_max              Analysis/tom_peakinc.c:23-52 -> numpy argmax
   fourier               Filtrans/tom_fourier.m:42-50 -> the 4x4 KAT tom_fourier.m:15-22
+  norm_mask             src/tom/core/volume_fcn.cpp:168-247 -> direct sums
+  simple_wedge_half     src/tom/core/wedge.cpp:149-243 -> grid predicate
+  subtomo_score         src/tom/corr/corr.cpp:969-1400, correlation_handler.cpp:96-120
+                        -> planted copy scores 1; brute-force circular correlation
+  mask_cc_sphere        peak-search mask of the 'peak' handler -> closed form
+  psr_ring / psr        src/tom/os3/os3_correlation.cpp:183-262 -> direct sums
+                        under the ring (test_psr_direct_sums)
+  erase_borders, peakc, return_picklist
+                        Oscar3/matlabPicker/results/tom_os3_returnPicklist.m:40-95,
+                        Analysis/tom_peakc.m:39-86 -> separated planted peaks come
+                        back in descending order with their erase spheres
+                        (test_picklist_planted_peaks)
 
 Deliberate deviations shared with the product (documented, not bugs):
   * Local variance is formed from the globally mean-shifted volume and shifted
@@ -523,6 +535,29 @@ def mask_cc_sphere(shape, radius: float) -> np.ndarray:
     ax = [np.fft.fftfreq(s, 1.0 / s) for s in shape]
     X, Y, Z = np.meshgrid(*ax, indexing="ij")
     return (np.sqrt(X * X + Y * Y + Z * Z) <= radius).astype(np.float32)
+
+
+# ----------------------------------------------------------------------------
+# secondary scores
+# ----------------------------------------------------------------------------
+def psr_ring(n: int) -> np.ndarray:
+    """Ring mask of os3_psr (Oscar3/c++Picker/src/tom/os3/os3_correlation.cpp:
+    203-243): binary sphere of radius n/2 minus the binary sphere of radius
+    max(n/4, 5), both about floor(n/2)."""
+    return sphere_mask_binary(n, n // 2) - sphere_mask_binary(n, max(n // 4, 5))
+
+
+def psr(xcf: np.ndarray, n: int) -> np.ndarray:
+    """Peak-to-sidelobe ratio map (os3_correlation.cpp:245-259): (xcf - mean) /
+    std under the ring, FP64 local statistics, 0 where the std is 0."""
+    M = psr_ring(n).astype(np.float64)
+    m = float((M > 0).sum())
+    x = np.asarray(xcf, np.float64)
+    mu, sig = local_stats(x, M, m)
+    out = np.zeros_like(x)
+    nz = sig > 0
+    out[nz] = (x[nz] - mu[nz]) / sig[nz]
+    return out
 
 
 # ----------------------------------------------------------------------------

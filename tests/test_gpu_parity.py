@@ -191,8 +191,8 @@ def test_find_template_no_weight_binary_mask(api, oracle):
 
 
 def test_search_plan_batches_equal(api, oracle):
-    """Batch size, split runs and hipGraph replay versus eager launches do not
-    change the result (ties included)."""
+    """Batch size, split runs, pass tile widths and hipGraph replay versus
+    eager launches do not change the result (ties included)."""
     say("plan batches")
     rng = np.random.default_rng(8)
     V = rng.standard_normal((64, 32, 32)).astype(np.float32)
@@ -201,9 +201,11 @@ def test_search_plan_batches_equal(api, oracle):
     ang = oracle.angle_grid(60, 60, 60)
     tn, mu, sg = api.prepare_template(T, M)
     outs = []
-    for batch, split, graph in [(1, 1, False), (4, 1, True), (4, 1, False), (7, 3, True), (3, 1, True)]:
+    for batch, split, graph, tiles in [(1, 1, False, (16, 16)), (4, 1, True, (16, 16)), (4, 1, False, (8, 32)),
+                                       (7, 3, True, (32, 8)), (3, 1, True, (32, 32))]:
         with api.SearchPlan(V, T.shape, M, batch=batch) as p:
             p.set_graph(graph)
+            p.set_tiles(*tiles)
             p.set_template(tn, None, mu, sg)
             step = int(np.ceil(len(ang) / split))
             for s0 in range(0, len(ang), step):
@@ -282,3 +284,33 @@ def test_refine_matches_exhaustive_maps(api, oracle):
         if srt[0] - srt[1] > MARGIN:
             ref = next(v for v in vals if v[0] == best)
             assert an[i] == ref[1] and tuple(pos[i]) == ref[2]
+
+
+def test_autotune_picks_a_batch_with_identical_results(api, oracle):
+    """The autotuner times candidate batches and checks that every batch gives
+    the same maps before it reports a choice."""
+    say("autotune")
+    from tom_amd import autotune
+    rng = np.random.default_rng(31)
+    V = rng.standard_normal((64, 32, 32)).astype(np.float32)
+    T = _phantom(rng, 16, 4, 1.5, 5)
+    M = oracle.sphere_mask_binary(16, 7)
+    tn, mu, sg = api.prepare_template(T, M)
+    res = autotune.tune_search(V, tn, M, None, mu, sg, oracle.angle_grid(60, 60, 60), batches=(1, 3, 4, 7),
+                               tiles=(8, 16, 32), groups=3, use_cache=False)
+    say(f"  {res}")
+    assert res["batch"] in (1, 3, 4, 7) and res["tile_z"] in (8, 16, 32) and res["tile_y"] in (8, 16, 32)
+    assert not res["cached"]
+
+
+def test_psr_matches_oracle(api, oracle):
+    say("psr")
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal((64, 48, 32)).astype(np.float32)
+    x[30, 20, 10] += 8
+    for n in (16, 24):
+        got = api.tom_os3_psr(x, n)
+        ref = oracle.psr(x, n)
+        err = np.abs(got - ref).max()
+        say(f"  n={n}: max |psr - fp64| = {err:.2e}")
+        assert err < 1e-5

@@ -293,3 +293,53 @@ def test_oscar_orientation(oracle):
     raw = oracle.correlation_raw(V, T.astype(np.float64))
     neg = raw[np.ix_(*[(-np.arange(12)) % 12] * 3)]
     np.testing.assert_allclose(ccf, neg, atol=1e-10)
+
+
+def test_psr_direct_sums(oracle):
+    """PSR = (x - local mean) / local std under the os3_psr ring, against
+    direct sums mu[k] = sum_u x[k + h - u] M[u] / m (h = floor(n/2), indices
+    modulo N), population variance."""
+    rng = np.random.default_rng(40)
+    N, n = (16, 14, 12), 12
+    x = rng.standard_normal(N)
+    M = oracle.psr_ring(n)
+    assert M.min() == 0 and M.max() == 1
+    r = np.sqrt(sum(g * g for g in np.meshgrid(*[np.arange(n) - n // 2] * 3, indexing="ij")))
+    np.testing.assert_array_equal(M > 0, (r < 6) & ~(r < 5))
+    m = M.sum()
+    h = n // 2
+    mu = np.zeros(N)
+    e2 = np.zeros(N)
+    for u in zip(*np.nonzero(M)):
+        sh = tuple(h - ui for ui in u)  # x[k + h - u] = roll(x, u - h)[k]
+        xs = np.roll(x, tuple(-s for s in sh), axis=(0, 1, 2))
+        mu += xs
+        e2 += xs * xs
+    mu /= m
+    sd = np.sqrt(np.maximum(e2 / m - mu * mu, 0))
+    np.testing.assert_allclose(oracle.psr(x, n), (x - mu) / sd, rtol=1e-9, atol=1e-9)
+
+
+def test_picklist_planted_peaks(oracle):
+    """Separated sharp peaks of known heights come back in descending order at
+    their positions (1-based), each pick erasing a sphere of radius
+    ceil(n/2) around it; border voxels (tom_os3_eraseBorders) never appear."""
+    N, n = (40, 36, 32), 6
+    a = np.zeros(N)
+    pos = [(10, 10, 10), (28, 12, 20), (12, 26, 22), (30, 28, 9)]
+    hts = [0.5, 0.9, 0.7, 0.6]
+    rng = np.random.default_rng(3)
+    a += 0.01 * rng.random(N)
+    for p, h in zip(pos, hts):
+        a[p] = h
+    a[0, 0, 0] = 5.0  # in the erased border
+    ang = np.arange(np.prod(N)).reshape(N, order="F").astype(np.int32)
+    rows = oracle.return_picklist(a.astype(np.float32), ang, (n, n, n), 4)
+    order = np.argsort(hts)[::-1]
+    for r, k in zip(rows, order):
+        p = pos[k]
+        assert tuple(int(v) for v in r[:3]) == tuple(q + 1 for q in p)
+        assert abs(r[3] - hts[k]) < 1e-7 and r[4] == ang[p]
+    # the first pick erased its sphere (radius 3): a neighbour at distance 2 is 0
+    _, _, after = oracle.peakc(oracle.erase_borders(a, (n, n, n)).astype(np.float32), 3)
+    assert after[28, 12, 22] == 0 and after[28, 12, 20] == 0 and after[28, 12, 24] != 0

@@ -47,6 +47,8 @@ SIGNATURES = {
     "tomhip_picklist_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp, vp]),
     "tomhip_search_refine": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64, vp, vp, vp]),
     "tomhip_local_maxima_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int64, vp, vp]),
+    "tomhip_psr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "tomhip_search_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
     "tomhip_unet_create": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, vp]),
     "tomhip_unet_destroy": (None, [vp]),
     "tomhip_unet_num_params": (ctypes.c_int, [vp, vp]),

@@ -306,6 +306,10 @@ class SearchPlan:
                                                          _p(pos)))
         return sc, an, pos
 
+    def set_tiles(self, tile_z: int = 16, tile_y: int = 16):
+        """Column-tile widths of the z- and y-FFT passes (8, 16 or 32)."""
+        self.ctx.check(self.ctx.lib.tomhip_search_set_tiles(self.h, int(tile_z), int(tile_y)))
+
     def set_graph(self, enable: bool):
         """Replay full angle batches through a captured hipGraph (default)."""
         self.ctx.check(self.ctx.lib.tomhip_search_set_graph(self.h, int(enable)))
@@ -408,7 +412,8 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
 
     boxes: list of dicts {"template": (B,B,B), "mask": (B,B,B), "mask_cc":
     (B,B,B) or None}, one per box edge B in {32, 48, 64} (MATLAB indexing).
-    particles: list of (B,B,B) arrays; particle_box[p] indexes boxes;
+    particles: list of (B,B,B) arrays, or one contiguous float32 torch tensor
+    holding them packed in order (x fastest); particle_box[p] indexes boxes;
     wedge_semi_deg[p] (<= 0: no wedge); angles: (P, A, 3) degrees.
     Returns (best angle index 0-based (-1 none), peak (P,3) 0-based unshifted,
     ccc (P,))."""
@@ -427,7 +432,14 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
     sizes = [int(edges[i]) ** 3 for i in pb]
     off = np.zeros(P, np.int64)
     off[1:] = np.cumsum(sizes)[:-1]
-    packed = np.concatenate([np.asfortranarray(v, np.float32).ravel(order="F") for v in particles])
+    if _is_torch(particles):
+        # already packed on the device: particle p at off[p], x fastest
+        if particles.dtype != torch.float32 or not particles.is_contiguous() or particles.numel() < sum(sizes):
+            raise ValueError("packed particles must be a contiguous float32 tensor holding every particle")
+        packed = particles
+        ctx.use_torch_stream()
+    else:
+        packed = np.concatenate([np.asfortranarray(v, np.float32).ravel(order="F") for v in particles])
     semi = np.ascontiguousarray(wedge_semi_deg, np.float32)
     ang = np.ascontiguousarray(np.asarray(angles, np.float32).reshape(P, -1, 3))
     A = ang.shape[1]
@@ -435,7 +447,7 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
     peak = np.zeros((P, 3), np.int32)
     ccc = np.zeros(P, np.float32)
     ctx.check(ctx.lib.tomhip_subtomo_score_batch(ctx.h, len(boxes), _p(edges), _p(T), _p(M),
-                                                 None if Mcc is None else _p(Mcc), P, _p(pb), _p(packed), _p(off),
+                                                 None if Mcc is None else _p(Mcc), P, _p(pb), _ptr(packed), _p(off),
                                                  _p(semi), _p(ang), A, _p(best), _p(peak), _p(ccc)))
     return best, peak, ccc
 
@@ -647,3 +659,15 @@ def local_maxima(peaks, radius: int, threshold: float, capacity: int = 1 << 20, 
     lin = (out[:, 2].astype(np.int64) * int(N[1]) + out[:, 1].astype(np.int64)) * int(N[0]) + out[:, 0].astype(np.int64)
     order = np.lexsort((lin, -out[:, 3]))
     return out[order]
+
+
+def tom_os3_psr(xcf, template_size, ctx: Optional[Context] = None):
+    """Peak-to-sidelobe ratio map of a correlation map (os3_psr): local
+    normalisation under the ring of the template's size."""
+    xb = _Buf(xcf)
+    ctx = _prep_ctx(ctx, xb)
+    N = _dims3(xb.dims)
+    n = _dims3(template_size if np.ndim(template_size) else (template_size,) * 3)
+    out = _empty_like(xb, xb.dims)
+    ctx.check(ctx.lib.tomhip_psr_f32(ctx.h, xb.ptr, _p(N), _p(n), _ptr(out)))
+    return out

@@ -419,6 +419,14 @@ int tomhip_search_profile(tomhip_search* h, int enable, double* ms, int64_t* cou
   });
 }
 
+int tomhip_search_set_tiles(tomhip_search* h, int tile_z, int tile_y) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h, ERR_WRONG_INPUT, "null pointer");
+    h->s->set_tiles(tile_z, tile_y);
+  });
+}
+
 int tomhip_search_set_graph(tomhip_search* h, int enable) {
   tomhip_ctx* ctx = h ? h->ctx : nullptr;
   return guard(ctx, [&] {
@@ -643,6 +651,40 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
       throw;
     }
     (void)hipFree(work);
+  });
+}
+
+int tomhip_psr_f32(tomhip_ctx* ctx, const float* xcf, const int64_t N[3], const int64_t n[3], float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && xcf && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "map");
+    check_dims3(n, "template");
+    const long NN = N[0] * N[1] * N[2], n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf x(s, xcf, sizeof(float) * NN, true, false);
+    DevBuf o(s, out, sizeof(float) * NN, false, true);
+    float* ring = nullptr;
+    double *mu = nullptr, *sig = nullptr;
+    auto release = [&] {
+      (void)hipFree(ring);
+      (void)hipFree(mu);
+      (void)hipFree(sig);
+    };
+    try {
+      TOMHIP_CHECK(hipMalloc(&ring, sizeof(float) * n3));
+      TOMHIP_CHECK(hipMalloc(&mu, sizeof(double) * NN));
+      TOMHIP_CHECK(hipMalloc(&sig, sizeof(double) * NN));
+      launch_psr_ring((int)n[0], (int)n[1], (int)n[2], ring, s);
+      local_stats(*ctx->tw, s, x.get<float>(), (int)N[0], (int)N[1], (int)N[2], ring, (int)n[0], (int)n[1],
+                  (int)n[2], mu, sig);
+      launch_psr(x.get<float>(), mu, sig, o.get<float>(), NN, s);
+      o.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
   });
 }
 

@@ -22,6 +22,12 @@ struct KernelTimes {
 };
 enum KernelId : int { K_ROT = 0, K_BOX = 1, K_TFWD = 2, K_PASSA = 3, K_PASSB = 4, K_PASSC = 5 };
 
+// FP64 local mean / std of a volume under a template-sized mask (device
+// pointers; mu, sig: X*Y*Z doubles), as tom_os3_mean / tom_os3_std with
+// m = #{mask > 0}; variance at or below 1e-12 x the global variance -> std 0.
+void local_stats(Twiddles& tw, hipStream_t s, const float* volume, int X, int Y, int Z, const float* mask, int nx,
+                 int ny, int nz, double* mu, double* sig);
+
 class Search {
  public:
   // volume: device, x-fastest X*Y*Z floats. mask: device nx*ny*nz (FLCF mode;
@@ -52,6 +58,11 @@ class Search {
   void refine(const float* volume, const float* rm_host, int G, int A, const int32_t* cand_dev, long P,
               float* bscore, int32_t* bangle, int32_t* bpos);
 
+  // Column-tile widths of pass A (z FFTs) and pass B (y FFTs): 8, 16 or 32
+  // columns per workgroup (default 16). Results do not depend on them.
+  void set_tiles(int tile_a, int tile_b);
+  bool tiles_ok(int tile_a, int tile_b) const;
+
   void set_profile(bool on) { profile_ = on; }
   // Replay full angle batches through one captured hipGraph (default on;
   // profiling runs eager so every launch can be timed).
@@ -73,6 +84,8 @@ class Search {
   void run_batch(const float* rm_dev, int ka, long first_index, const long* desc);
   void rotate_templates(const float* rm_dev, int count, float* tout, const long* desc, long rm_step);
   void box_filter(int count, float* t, c32* work);
+  void launch_passA(int ka);
+  void launch_passB(int ka);
   void capture_graph();
   void drop_graph();
   template <class F>
@@ -111,6 +124,7 @@ class Search {
   bool remask_ = true;
   // graph of one full batch, replayed with the device descriptor desc_
   bool use_graph_ = true;
+  int tile_a_ = 16, tile_b_ = 16;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   long* desc_ = nullptr;  // {batch, first index}

@@ -30,6 +30,7 @@
 // the score is a strictly increasing function of c at every voxel, so the
 // max/argmax runs on c and the normalisation is applied once in finalize().
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -721,51 +722,112 @@ Search::~Search() {
   }
 }
 
-void Search::precompute_stats(const float* volume) {
-  // global mean and variance (FP64)
+namespace {
+
+// FP64 mean g and variance of a volume (two passes over partial sums)
+void global_moments(const float* volume, long N, hipStream_t s, double* g_out, double* var_out) {
   const int nb = 1024;
   double* part = nullptr;
   TOMHIP_CHECK(hipMalloc(&part, sizeof(double) * nb));
   std::vector<double> hp(nb);
-  hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s_, volume, N, 0.0, 0, part);
-  TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s_));
-  TOMHIP_CHECK(hipStreamSynchronize(s_));
-  double sum = 0;
-  for (double v : hp) sum += v;
-  const double g = sum / (double)N;
-  hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s_, volume, N, g, 1, part);
-  TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s_));
-  TOMHIP_CHECK(hipStreamSynchronize(s_));
-  double ss = 0;
-  for (double v : hp) ss += v;
-  const double varg = ss / (double)N;
+  try {
+    hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s, volume, N, 0.0, 0, part);
+    TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    double sum = 0;
+    for (double v : hp) sum += v;
+    const double g = sum / (double)N;
+    hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NT), 0, s, volume, N, g, 1, part);
+    TOMHIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    double ss = 0;
+    for (double v : hp) ss += v;
+    *g_out = g;
+    *var_out = ss / (double)N;
+  } catch (...) {
+    (void)hipFree(part);
+    throw;
+  }
   (void)hipFree(part);
+}
 
+// Local mean / std under the mask from A = FFT((V-g) + i (V-g)^2) (FP64;
+// A is overwritten): tom_os3_mean / tom_os3_std with the exact shift-back.
+void stats_from_spectrum(c64* A, Twiddles& tw, hipStream_t s, int X, int Y, int Z, const float* mask, int nx,
+                         int ny, int nz, double m, double mask_sum, double g, double varg, double* mu, double* sig) {
+  const long N = (long)X * Y * Z;
+  c64* B = nullptr;
+  TOMHIP_CHECK(hipMalloc(&B, sizeof(c64) * N));
+  try {
+    hipLaunchKernelGGL(k_place_mask, dim3(grid_for(N)), dim3(NT), 0, s, mask, nx, ny, nz, X, Y, Z, B);
+    TOMHIP_CHECK(hipGetLastError());
+    fft3_c2c<double>(B, X, Y, Z, 1, -1, tw, s);
+    hipLaunchKernelGGL(k_mul_c64, dim3(grid_for(N)), dim3(NT), 0, s, A, (const c64*)B, N);
+    TOMHIP_CHECK(hipGetLastError());
+    fft3_c2c<double>(A, X, Y, Z, 1, 1, tw, s);
+    const double inv_nm = 1.0 / ((double)N * m);
+    hipLaunchKernelGGL(k_stats, dim3(grid_for(N)), dim3(NT), 0, s, (const c64*)A, N, inv_nm, g, mask_sum / m,
+                       1e-12 * varg, mu, sig);
+    TOMHIP_CHECK(hipGetLastError());
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(B);
+    throw;
+  }
+  (void)hipFree(B);
+}
+
+}  // namespace
+
+void local_stats(Twiddles& tw, hipStream_t s, const float* volume, int X, int Y, int Z, const float* mask, int nx,
+                 int ny, int nz, double* mu, double* sig) {
+  TOMHIP_REQUIRE(supported_len(X) && supported_len(Y) && supported_len(Z), ERR_UNSUPPORTED,
+                 "volume edges must be supported FFT lengths");
+  TOMHIP_REQUIRE(nx <= X && ny <= Y && nz <= Z, ERR_WRONG_INPUT, "mask larger than the volume");
+  const long N = (long)X * Y * Z, n3 = (long)nx * ny * nz;
+  std::vector<float> hm(n3);
+  TOMHIP_CHECK(hipMemcpyAsync(hm.data(), mask, sizeof(float) * n3, hipMemcpyDeviceToHost, s));
+  TOMHIP_CHECK(hipStreamSynchronize(s));
+  double m = 0, msum = 0;
+  for (float v : hm) {
+    m += (v > 0);
+    msum += v;
+  }
+  TOMHIP_REQUIRE(m > 0, ERR_WRONG_INPUT, "mask has no positive voxel");
+  double g, varg;
+  global_moments(volume, N, s, &g, &varg);
   c64* A = nullptr;
   TOMHIP_CHECK(hipMalloc(&A, sizeof(c64) * N));
-  hipLaunchKernelGGL(k_make_c, dim3(grid_for(N)), dim3(NT), 0, s_, volume, N, g, A);
-  TOMHIP_CHECK(hipGetLastError());
-  fft3_c2c<double>(A, X, Y, Z, 1, -1, tw_, s_);
-  hipLaunchKernelGGL(k_extract_fi, dim3(grid_for((long)Z * Y * (Xh + 1))), dim3(NT), 0, s_, A, X, Y, Z,
-                     g * (double)N, FI_, FIn_);
-  TOMHIP_CHECK(hipGetLastError());
-  if (mode == MODE_FLCF) {
-    c64* B = nullptr;
-    TOMHIP_CHECK(hipMalloc(&B, sizeof(c64) * N));
-    hipLaunchKernelGGL(k_place_mask, dim3(grid_for(N)), dim3(NT), 0, s_, mask_, nx, ny, nz, X, Y, Z, B);
+  try {
+    hipLaunchKernelGGL(k_make_c, dim3(grid_for(N)), dim3(NT), 0, s, volume, N, g, A);
     TOMHIP_CHECK(hipGetLastError());
-    fft3_c2c<double>(B, X, Y, Z, 1, -1, tw_, s_);
-    hipLaunchKernelGGL(k_mul_c64, dim3(grid_for(N)), dim3(NT), 0, s_, A, (const c64*)B, N);
-    TOMHIP_CHECK(hipGetLastError());
-    fft3_c2c<double>(A, X, Y, Z, 1, 1, tw_, s_);
-    const double inv_nm = 1.0 / ((double)N * m);
-    hipLaunchKernelGGL(k_stats, dim3(grid_for(N)), dim3(NT), 0, s_, (const c64*)A, N, inv_nm, g, mask_sum_ / m,
-                       1e-12 * varg, mu_, sig_);
-    TOMHIP_CHECK(hipGetLastError());
-    TOMHIP_CHECK(hipStreamSynchronize(s_));
-    (void)hipFree(B);
+    fft3_c2c<double>(A, X, Y, Z, 1, -1, tw, s);
+    stats_from_spectrum(A, tw, s, X, Y, Z, mask, nx, ny, nz, m, msum, g, varg, mu, sig);
+  } catch (...) {
+    (void)hipFree(A);
+    throw;
   }
-  TOMHIP_CHECK(hipStreamSynchronize(s_));
+  (void)hipFree(A);
+}
+
+void Search::precompute_stats(const float* volume) {
+  double g, varg;
+  global_moments(volume, N, s_, &g, &varg);
+  c64* A = nullptr;
+  TOMHIP_CHECK(hipMalloc(&A, sizeof(c64) * N));
+  try {
+    hipLaunchKernelGGL(k_make_c, dim3(grid_for(N)), dim3(NT), 0, s_, volume, N, g, A);
+    TOMHIP_CHECK(hipGetLastError());
+    fft3_c2c<double>(A, X, Y, Z, 1, -1, tw_, s_);
+    hipLaunchKernelGGL(k_extract_fi, dim3(grid_for((long)Z * Y * (Xh + 1))), dim3(NT), 0, s_, A, X, Y, Z,
+                       g * (double)N, FI_, FIn_);
+    TOMHIP_CHECK(hipGetLastError());
+    if (mode == MODE_FLCF) stats_from_spectrum(A, tw_, s_, X, Y, Z, mask_, nx, ny, nz, m, mask_sum_, g, varg, mu_, sig_);
+    TOMHIP_CHECK(hipStreamSynchronize(s_));
+  } catch (...) {
+    (void)hipFree(A);
+    throw;
+  }
   (void)hipFree(A);
 }
 
@@ -935,6 +997,68 @@ void Search::box_filter(int count, float* t, c32* work) {
   TOMHIP_CHECK(hipGetLastError());
 }
 
+namespace {
+template <class F>
+void dispatch_tile(int tile, F&& f) {
+  switch (tile) {
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: throw Error(ERR_UNSUPPORTED, "tile widths are 8, 16 or 32");
+  }
+}
+constexpr long kLdsMax = 150 * 1024;
+}  // namespace
+
+bool Search::tiles_ok(int ta, int tb) const {
+  auto lds_a = [&](int tw) { return (long)Z * (tw + 1) * 8 + (long)Z * 8 <= kLdsMax; };
+  auto lds_b = [&](int tw) { return (long)Y * tw * 8 + (long)Y * 8 <= kLdsMax; };
+  const bool va = (ta == 8 || ta == 16 || ta == 32) && Xh % ta == 0 && lds_a(ta);
+  const bool vb = (tb == 8 || tb == 16 || tb == 32) && Xh % tb == 0 && lds_b(tb);
+  return va && vb;
+}
+
+void Search::set_tiles(int ta, int tb) {
+  TOMHIP_REQUIRE(tiles_ok(ta, tb), ERR_UNSUPPORTED, "tile widths must be 8, 16 or 32, divide X/2 and fit in LDS");
+  if (ta != tile_a_ || tb != tile_b_) drop_graph();
+  tile_a_ = ta;
+  tile_b_ = tb;
+}
+
+void Search::launch_passA(int ka) {
+  dispatch_tile(tile_a_, [&](auto TWc) {
+    constexpr int TW = decltype(TWc)::value;
+    dispatch_len(Z, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if constexpr ((long)L * (TW + 1) * 8 + (long)L * 8 <= kLdsMax) {
+        hipLaunchKernelGGL((k_passA<L, TW, true>), dim3(1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
+                           (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+        if (Xh > TW)
+          hipLaunchKernelGGL((k_passA<L, TW, false>), dim3(Xh / TW - 1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_,
+                             (const c32*)Tyn_, (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+      } else {
+        throw Error(ERR_UNSUPPORTED, "pass A tile does not fit in LDS");
+      }
+    });
+  });
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+void Search::launch_passB(int ka) {
+  dispatch_tile(tile_b_, [&](auto TWc) {
+    constexpr int TW = decltype(TWc)::value;
+    dispatch_len(Y, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if constexpr ((long)L * TW * 8 + (long)L * 8 <= kLdsMax) {
+        hipLaunchKernelGGL((k_passB<L, TW>), dim3(Xh / TW, Z, ka), dim3(NT_B), 0, s_, Q_, tw_.f32(L), Z, Xh);
+      } else {
+        throw Error(ERR_UNSUPPORTED, "pass B tile does not fit in LDS");
+      }
+    });
+  });
+  TOMHIP_CHECK(hipGetLastError());
+}
+
 void Search::run_batch(const float* rm_dev, int ka, long first_index, const long* desc) {
   timed(K_ROT, [&] { rotate_templates(rm_dev, ka, t1_, desc, 9L * ka); });
   if (wsh_) timed(K_BOX, [&] { box_filter(ka, t1_, tbox_); });
@@ -952,24 +1076,8 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index, const long
     });
     TOMHIP_CHECK(hipGetLastError());
   });
-  timed(K_PASSA, [&] {
-    dispatch_len(Z, [&](auto Lc) {
-      constexpr int L = decltype(Lc)::value;
-      hipLaunchKernelGGL((k_passA<L, 16, true>), dim3(1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
-                         (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
-      if (Xh > 16)
-        hipLaunchKernelGGL((k_passA<L, 16, false>), dim3(Xh / 16 - 1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_,
-                           (const c32*)Tyn_, (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
-    });
-    TOMHIP_CHECK(hipGetLastError());
-  });
-  timed(K_PASSB, [&] {
-    dispatch_len(Y, [&](auto Lc) {
-      constexpr int L = decltype(Lc)::value;
-      hipLaunchKernelGGL((k_passB<L, 16>), dim3(Xh / 16, Z, ka), dim3(NT_B), 0, s_, Q_, tw_.f32(L), Z, Xh);
-    });
-    TOMHIP_CHECK(hipGetLastError());
-  });
+  timed(K_PASSA, [&] { launch_passA(ka); });
+  timed(K_PASSB, [&] { launch_passB(ka); });
   timed(K_PASSC, [&] {
     dispatch_xh(Xh, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;

@@ -18,4 +18,8 @@ void launch_spheremask(int nx, int ny, int nz, float radius, float sigma, float 
 void launch_weight(int nx, int ny, int nz, int use_wedge, double semi_angle_deg, int use_bp, double lo,
                    double hi, int shifted, float* out, hipStream_t s);
 void global_max(const float* a, long n, float* val, long* idx, hipStream_t s);
+// os3_psr's ring mask (sphere nx/2 minus sphere max(nx/4, 5)) of a box
+void launch_psr_ring(int nx, int ny, int nz, float* out, hipStream_t s);
+// out = (x - mu) / sig where sig > 0, else 0
+void launch_psr(const float* x, const double* mu, const double* sig, float* out, long n, hipStream_t s);
 }  // namespace tomhip

@@ -308,4 +308,40 @@ void global_max(const float* a, long n, float* val, long* idx, hipStream_t s) {
   (void)hipFree(pi);
 }
 
+namespace {
+
+// PSR ring of os3_psr (Oscar3/c++Picker/src/tom/os3/os3_correlation.cpp:183-262):
+// binary sphere of radius nx/2 minus the binary sphere of radius
+// max(nx/4, 5), both tom::init_spheremask spheres about the box centre.
+__global__ void k_ring(int nx, int ny, int nz, float r_out, float r_in, float* out) {
+  const long n = (long)nx * ny * nz;
+  const float cx = (float)(nx / 2), cy = (float)(ny / 2), cz = (float)(nz / 2);
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const float x = (float)(i % nx) - cx, y = (float)((i / nx) % ny) - cy, z = (float)(i / ((long)nx * ny)) - cz;
+    const float r = sqrtf(x * x + (y * y + z * z));
+    out[i] = (r < r_out ? 1.f : 0.f) - (r < r_in ? 1.f : 0.f);
+  }
+}
+
+// (x - mu) / sig where sig > 0, else 0 (element_wise_div with 0 fill)
+__global__ void k_psr(const float* x, const double* mu, const double* sig, float* out, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT)
+    out[i] = sig[i] > 0 ? (float)(((double)x[i] - mu[i]) / sig[i]) : 0.f;
+}
+
+}  // namespace
+
+void launch_psr_ring(int nx, int ny, int nz, float* out, hipStream_t s) {
+  const float r_out = (float)(nx / 2);
+  float r_in = (float)(nx / 4);
+  if (r_in < 5) r_in = 5;
+  hipLaunchKernelGGL(k_ring, dim3(gridn((long)nx * ny * nz)), dim3(NT), 0, s, nx, ny, nz, r_out, r_in, out);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+void launch_psr(const float* x, const double* mu, const double* sig, float* out, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_psr, dim3(gridn(n)), dim3(NT), 0, s, x, mu, sig, out, n);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace tomhip
