@@ -123,21 +123,31 @@ def cpu_baseline(cfg, V_host, T, M, W, angles, k, workers):
     return N * k / dt, t_pre, dt, maps
 
 
-def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exhaustive_s):
+def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exhaustive_s, plan_template, pre_s):
     """Proposal network + exact local refinement on the benchmark tomogram;
     recall of the exhaustive search's picklist peaks above 0.3."""
     from tom_amd import surrogate as S
     w = np.load(wpath, allow_pickle=False)
     sg = S.Surrogate(w["params"], w["bins"])
     ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
+    T, M, W, tn, mu_t, sig_t = plan_template
     # warm (network allocation, code objects)
-    sg.propose(V, K=16)
+    aux = S.aux_map(plan, T, M, W)
+    sg.propose(V, K=16, aux=aux)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    cand = sg.propose(V, K=args.topk)
+    aux = S.aux_map(plan, T, M, W)
+    plan.set_template(tn, W, mu_t, sig_t)
+    torch.cuda.synchronize()
+    t_aux = time.perf_counter() - t0
+    cand = sg.propose(V, K=args.topk, threshold=0.02, aux=aux)
     sc, eul, pos = sg.refine(plan, V, cand)
     torch.cuda.synchronize()
-    t_sur = time.perf_counter() - t0
+    # the refinement needs the plan's one-off precompute (F_I, FP64 statistics)
+    # just as the exhaustive search does
+    t_sur = time.perf_counter() - t0 + pre_s
+    sg.timings["aux_s"] = t_aux
+    sg.timings["precompute_s"] = pre_s
     rec_any = S.recall(ref[:, :3], pos)
     rec_hit = S.recall(ref[:, :3], pos[sc > 0.3])
     # exhaustive best score at the matched peaks vs the refined score there
@@ -273,6 +283,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--batch", default="4", help="orientations per launch group, or 'auto' (autotuner)")
     ap.add_argument("--tiles", default="16,16", help="column tiles of the z- and y-FFT passes (8/16/32)")
+    ap.add_argument("--zchunk", type=int, default=0, help="z-chunk planes of the y-FFT and c2r passes (0 = whole)")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--cpu-particles", type=int, default=6, help="c3: particles in the oracle sample")
     ap.add_argument("--cpu-angles", type=int, default=4, help="orientations in the CPU-baseline sample (0 = skip)")
@@ -308,6 +319,7 @@ def main():
     A = angles.shape[0]
     KA = 4 if args.batch == "auto" else int(args.batch)
     tiles = tuple(int(v) for v in args.tiles.split(","))
+    zchunk = args.zchunk
     groups = (A + KA - 1) // KA
     steps = args.steps if args.steps > 0 else groups
     steps = min(steps, groups)
@@ -334,12 +346,14 @@ def main():
         tuned = autotune.tune_search(V, tn, M, W, mu_t, sig_t, angles, log=log)
         KA = tuned["batch"]
         tiles = (tuned["tile_z"], tuned["tile_y"])
+        zchunk = tuned["zchunk"]
         groups = (A + KA - 1) // KA
         steps = min(args.steps if args.steps > 0 else groups, groups)
         log(f"[bench] autotuned batch {KA}")
     t0 = time.perf_counter()
     plan = api.SearchPlan(V, (cfg["n"],) * 3, M, batch=KA)
     plan.set_tiles(*tiles)
+    plan.set_zchunk(zchunk)
     torch.cuda.synchronize()
     pre_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] precompute (F_I + FP64 local stats) {pre_ms:.1f} ms")
@@ -391,7 +405,8 @@ def main():
     if wpath and os.path.exists(wpath) and ntimed == A and not args.no_surrogate:
         sur = surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath,
                             exhaustive_s=(pre_ms / 1e3 + (graph["ms_per_step"] * groups / 1e3 if graph else dt)
-                                          + fin_ms / 1e3))
+                                          + fin_ms / 1e3), plan_template=(T, M, W, tn, mu_t, sig_t),
+                            pre_s=pre_ms / 1e3)
 
     # ---- roofline of the dominant kernel (event times over the timed region)
     names = ["rotate", "box_weight", "tmpl_fwd", "passA", "passB", "passC"]
@@ -452,7 +467,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": cfg["desc"], "volume": list(cfg["N"]), "template": cfg["n"],
-                   "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA, "tiles_zy": list(tiles),
+                   "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA, "tiles_zy": list(tiles), "zchunk": zchunk,
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
                    "timed_path": "eager launches with per-kernel HIP events",

@@ -116,6 +116,10 @@ int tomhip_search_set_graph(tomhip_search* s, int enable);
  * y-FFT pass: workgroup shape only, results are identical. No reference
  * counterpart; set by the autotuner. */
 int tomhip_search_set_tiles(tomhip_search* s, int tile_z, int tile_y);
+/* Run the y-FFT and c2r/max passes over z-chunks of this many planes (0 =
+ * whole volume), so the c2r pass reads the y pass's output back from the
+ * memory-side cache. Results are identical. No reference counterpart. */
+int tomhip_search_set_zchunk(tomhip_search* s, int planes);
 void tomhip_search_destroy(tomhip_search* s);
 
 /* ---------------------------------------------------------------- one-shot drivers */
@@ -192,7 +196,7 @@ int tomhip_psr_f32(tomhip_ctx* ctx, const float* xcf, const int64_t N[3], const 
 
 /* ---------------------------------------------------------------- learned proposal */
 /* No counterpart in the reference: a 3-level 3D U-Net (16/32/64 channels, two
- * 3x3x3 convolutions + ReLU per level, 2x2x2 max-pool, trilinear x2
+ * 3x3x3 convolutions + ReLU per level, 2x2x2 average pool, trilinear x2
  * upsampling, skip concatenation) with a 1x1x1 head giving a particle logit
  * and nbin orientation-bin logits per voxel. BF16 MFMA convolutions with FP32
  * accumulation, FP32 master weights, Adam. It proposes candidate positions and
@@ -207,7 +211,8 @@ int tomhip_unet_num_params(tomhip_unet* h, int64_t* n);
  * 1a 1b, head), each layer W[taps][Cout][Cin] then bias[Cout] */
 int tomhip_unet_get_params(tomhip_unet* h, float* out);
 int tomhip_unet_set_params(tomhip_unet* h, const float* in);
-int tomhip_unet_forward(tomhip_unet* h, const float* volume);
+/* volume and the optional aux map (NULL = zeros) fill input channels 0 and 1 */
+int tomhip_unet_forward(tomhip_unet* h, const float* volume, const float* aux);
 /* logits [V][80]: channel 0 particle, 1..nbin orientation bins */
 int tomhip_unet_get_logits(tomhip_unet* h, float* out);
 /* loss: weighted BCE (positive weight wpos) of channel 0 against tprob[V],
@@ -220,9 +225,10 @@ int tomhip_unet_adam(tomhip_unet* h, float lr);
 /* prob[V] = sigmoid(logit 0), bin[V] = argmax of the bin logits (0-based) */
 int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin);
 /* Whole-volume inference in overlapping tiles of the network's size (halo
- * voxels of context on each side, tile origins on multiples of 4). */
-int tomhip_unet_infer(tomhip_unet* h, const float* volume, const int64_t N[3], int halo, float* prob,
-                      int32_t* bin);
+ * voxels of context on each side, tile origins on multiples of 4). volume and
+ * the optional aux map are each normalised by their global mean and std. */
+int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
+                      float* prob, int32_t* bin);
 
 /* Candidate extraction for the learned proposal (no reference counterpart):
  * local maxima of map (x fastest) above threshold within the cube of

@@ -5,14 +5,15 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the HIP U-Net
 product path.
 
 The network (layer order = parameter order of tomhip_unet_get_params):
-  e1a 16->16, e1b 16->16 (skip1), pool,
-  e2a 16->32, e2b 32->32 (skip2), pool,
+  e1a 16->16, e1b 16->16 (skip1), 2x2x2 average pool,
+  e2a 16->32, e2b 32->32 (skip2), average pool,
   b1 32->64, b2 64->64, up x2 (trilinear, half-pixel, clamped),
   d2a [up(b2) | skip2] 96->32, d2b 32->32, up x2,
   d1a [up(d2b) | skip1] 48->16, d1b 16->16,
   head 1x1x1 16->80 (channel 0 particle logit, 1..nbin orientation bins).
 3x3x3 convolutions, zero padding, ReLU except the head. The input volume sits
-in channel 0 of 16 (the other 15 are zero).
+in channel 0 of 16, an optional auxiliary map in channel 1 (the other
+channels are zero).
 Parameters per layer: W[t][co][ci] (t = 9(dz+1) + 3(dy+1) + (dx+1), output
 voxel v reads input v + (dx, dy, dz)) followed by bias[co].
 
@@ -59,14 +60,16 @@ def unpack(theta, dtype=torch.float64):
     return out
 
 
-def forward(vol, theta, bf16=True, params=None):
-    """vol: [D][H][W] array (z, y, x). Returns logits [D][H][W][80] and the
-    parameter tensors (for autograd)."""
+def forward(vol, theta, bf16=True, params=None, aux=None):
+    """vol (and the optional aux map): [D][H][W] arrays (z, y, x). Returns
+    logits [D][H][W][80] and the parameter tensors (for autograd)."""
     P = params if params is not None else unpack(theta)
     q = bf16_round if bf16 else (lambda a: a)
     x = torch.as_tensor(np.asarray(vol), dtype=torch.float64)
     x0 = torch.zeros((1, 16) + tuple(x.shape), dtype=torch.float64)
     x0[0, 0] = q(x)
+    if aux is not None:
+        x0[0, 1] = q(torch.as_tensor(np.asarray(aux), dtype=torch.float64))
 
     def conv(i, inp):
         w, b = P[i]
@@ -82,10 +85,10 @@ def forward(vol, theta, bf16=True, params=None):
 
     a1 = st(conv(0, x0))
     s1 = st(conv(1, a1))
-    p1 = F.max_pool3d(s1, 2)
+    p1 = st(F.avg_pool3d(s1, 2))
     a2 = st(conv(2, p1))
     s2 = st(conv(3, a2))
-    p2 = F.max_pool3d(s2, 2)
+    p2 = st(F.avg_pool3d(s2, 2))
     b1 = st(conv(4, p2))
     b2 = st(conv(5, b1))
     u2 = st(F.interpolate(b2, scale_factor=2, mode="trilinear", align_corners=False))
@@ -114,13 +117,13 @@ def loss(logits, tprob, tbin, nbin, wpos):
     return bce, ce
 
 
-def gradients(vol, theta, tprob, tbin, nbin, wpos, bf16=True):
+def gradients(vol, theta, tprob, tbin, nbin, wpos, bf16=True, aux=None):
     """(bce, ce, flat gradient of bce + ce) in FP64."""
     P = unpack(theta)
     for w, b in P:
         w.requires_grad_(True)
         b.requires_grad_(True)
-    logits, _ = forward(vol, theta, bf16=bf16, params=P)
+    logits, _ = forward(vol, theta, bf16=bf16, params=P, aux=aux)
     bce, ce = loss(logits, tprob, tbin, nbin, wpos)
     (bce + ce).backward()
     g = []

@@ -191,8 +191,9 @@ def test_find_template_no_weight_binary_mask(api, oracle):
 
 
 def test_search_plan_batches_equal(api, oracle):
-    """Batch size, split runs, pass tile widths and hipGraph replay versus
-    eager launches do not change the result (ties included)."""
+    """Batch size, split runs, pass tile widths, z-chunking and hipGraph
+    replay versus eager launches do not change the result (ties included) on
+    this volume."""
     say("plan batches")
     rng = np.random.default_rng(8)
     V = rng.standard_normal((64, 32, 32)).astype(np.float32)
@@ -201,11 +202,13 @@ def test_search_plan_batches_equal(api, oracle):
     ang = oracle.angle_grid(60, 60, 60)
     tn, mu, sg = api.prepare_template(T, M)
     outs = []
-    for batch, split, graph, tiles in [(1, 1, False, (16, 16)), (4, 1, True, (16, 16)), (4, 1, False, (8, 32)),
-                                       (7, 3, True, (32, 8)), (3, 1, True, (32, 32))]:
+    for batch, split, graph, tiles, zc in [(1, 1, False, (16, 16), 0), (4, 1, True, (16, 16), 0),
+                                           (4, 1, False, (8, 32), 8), (7, 3, True, (32, 8), 12),
+                                           (3, 1, True, (32, 32), 5)]:
         with api.SearchPlan(V, T.shape, M, batch=batch) as p:
             p.set_graph(graph)
             p.set_tiles(*tiles)
+            p.set_zchunk(zc)
             p.set_template(tn, None, mu, sg)
             step = int(np.ceil(len(ang) / split))
             for s0 in range(0, len(ang), step):
@@ -297,9 +300,10 @@ def test_autotune_picks_a_batch_with_identical_results(api, oracle):
     M = oracle.sphere_mask_binary(16, 7)
     tn, mu, sg = api.prepare_template(T, M)
     res = autotune.tune_search(V, tn, M, None, mu, sg, oracle.angle_grid(60, 60, 60), batches=(1, 3, 4, 7),
-                               tiles=(8, 16, 32), groups=3, use_cache=False)
+                               tiles=(8, 16, 32), zchunks=(0, 8), groups=3, use_cache=False)
     say(f"  {res}")
     assert res["batch"] in (1, 3, 4, 7) and res["tile_z"] in (8, 16, 32) and res["tile_y"] in (8, 16, 32)
+    assert res["zchunk"] in (0, 8)
     assert not res["cached"]
 
 

@@ -23,14 +23,15 @@ def test_label_volumes():
     bins = orient.orientation_bins(64, 0)
     angles = np.asarray([[0, 0, 0], [10, 20, 30]], np.float32)
     peaks = np.array([[5, 6, 7, 0.5, 1], [0, 0, 0, 0.4, 0]], float)
-    tp, tb = S.label_volumes((10, 12, 14), peaks, angles, bins, radius=2.0)
+    tp, tb = S.label_volumes((12, 14, 16), peaks, angles, bins, sigma=2.0, bin_radius=2.0)
     b = orient.assign_bins(angles, bins)
     assert tp[7, 6, 5] == 1 and tb[7, 6, 5] == b[1]
-    assert tp[7, 6, 7] == 1 and tp[7, 6, 8] == 0  # radius 2 along x
+    assert abs(tp[7, 6, 7] - np.exp(-4 / 8)) < 1e-6 and tb[7, 6, 7] == b[1] and tb[7, 6, 8] == -1
+    assert tp[7, 6, 11] == np.float32(np.exp(-36 / 8)) and tp[7, 6, 12] == 0  # cut at 3 sigma
     assert tp[0, 0, 0] == 1 and tb[0, 0, 0] == b[0]
-    assert tp.sum() == (tb >= 0).sum()
-    # a sphere of radius 2 has 33 voxels; at the corner one octant (11) remains
-    assert tp.sum() == 33 + 11
+    # the nearer peak wins where the two overlap
+    assert tp[2, 2, 2] == np.float32(np.exp(-12 / 8))
+    assert (tb >= 0).sum() == 33 + 11
 
 
 def test_recall():
@@ -71,3 +72,27 @@ def test_local_maxima_brute_force(api):
     ref = sorted(ref, key=lambda r: (-r[3], (r[2] * Y + r[1]) * X + r[0]))
     np.testing.assert_array_equal(lm, np.asarray(ref, np.float32))
     assert (3, 4, 5, 2.0) in [tuple(r) for r in lm.tolist()]
+
+
+def test_dense_targets():
+    bins = orient.orientation_bins(64, 0)
+    angles = np.asarray([[0, 0, 0], [10, 20, 30], [200, 100, 60]], np.float32)
+    sc = np.array([[[0.0, 0.1, 0.3], [0.5, 0.7, 0.26]]], np.float32)
+    an = np.array([[[0, 1, 2], [1, 2, 0]]], np.int32)
+    tp, tb = S.dense_targets(sc, an, angles, bins)
+    np.testing.assert_allclose(tp, [[[0, 0, 0.5], [1, 1, 0.4]]], atol=1e-6)
+    b = orient.assign_bins(angles, bins)
+    np.testing.assert_array_equal(tb, [[[-1, -1, b[2]], [b[1], b[2], b[0]]]])
+
+
+def test_spherical_average():
+    rng = np.random.default_rng(7)
+    T = rng.random((16, 16, 16)).astype(np.float32)
+    A = S.spherical_average(T)
+    g = np.meshgrid(*[np.arange(16) - 8] * 3, indexing="ij")
+    r = np.rint(np.sqrt(sum(x * x for x in g))).astype(int)
+    for k in (0, 1, 5, 8):
+        sel = r == k
+        assert np.allclose(A[sel], T[sel].mean(), atol=1e-6)
+    # invariant under an exchange of axes (a distance-preserving map about the centre)
+    np.testing.assert_allclose(S.spherical_average(T.transpose(1, 2, 0)), A, atol=1e-6)

@@ -18,14 +18,14 @@ NBIN = 64
 
 def _theta(rng, scale=1.0):
     """He-scaled random parameters laid out as the HIP network stores them
-    (zero for the 15 padding input channels of the first layer and for the
+    (zero for the 14 padding input channels of the first layer and for the
     unused head channels)."""
     out = []
     for li, (ci, co, t, _) in enumerate(U.SPEC):
-        fan = t * (1 if li == 0 else ci)
+        fan = t * (2 if li == 0 else ci)
         W = rng.standard_normal((t, co, ci)) * np.sqrt(2.0 / fan) * scale
         if li == 0:
-            W[:, :, 1:] = 0
+            W[:, :, 2:] = 0
         if t == 1:
             W[:, 1 + NBIN:, :] = 0
         b = rng.standard_normal(co) * 0.05
@@ -132,13 +132,14 @@ def test_unet_forward_matches_restatement(api):
     rng = np.random.default_rng(10)
     shape = (16, 16, 24)
     vol = rng.standard_normal(shape).astype(np.float32)
+    aux = rng.standard_normal(shape).astype(np.float32)
     theta = _theta(rng).astype(np.float32)
     with _gpu_net(api, shape, theta) as net:
-        net.forward(_as_f(vol))
+        net.forward(_as_f(vol), _as_f(aux))
         got = net.logits()
         back = net.get_params()
     np.testing.assert_array_equal(back, theta)
-    ref, _ = U.forward(vol, theta.astype(np.float64), bf16=True)
+    ref, _ = U.forward(vol, theta.astype(np.float64), bf16=True, aux=aux)
     ref = ref.detach().numpy()
     scale = np.abs(ref).max()
     d = np.abs(got - ref)
@@ -152,13 +153,14 @@ def test_unet_backward_matches_restatement(api):
     rng = np.random.default_rng(11)
     shape = (16, 16, 24)
     vol = rng.standard_normal(shape).astype(np.float32)
+    aux = rng.standard_normal(shape).astype(np.float32)
     theta = _theta(rng).astype(np.float32)
     tprob, tbin = _targets(rng, shape)
     with _gpu_net(api, shape, theta) as net:
-        net.forward(_as_f(vol))
+        net.forward(_as_f(vol), _as_f(aux))
         bce, ce = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
         g = net.get_grads()
-    rb, rc, rg = U.gradients(vol, theta.astype(np.float64), tprob, tbin, NBIN, 4.0, bf16=True)
+    rb, rc, rg = U.gradients(vol, theta.astype(np.float64), tprob, tbin, NBIN, 4.0, bf16=True, aux=aux)
     print(f"  loss gpu ({bce:.6f}, {ce:.6f}) ref ({rb:.6f}, {rc:.6f})", flush=True)
     assert abs(bce - rb) <= 1e-3 * abs(rb) and abs(ce - rc) <= 1e-3 * abs(rc)
     off = 0
@@ -192,8 +194,10 @@ def test_unet_adam_descends_and_infer_tiles(api):
         print(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
         assert losses[-1] < 0.9 * losses[0]
         vn = (vol - vol.astype(np.float64).mean()) / vol.astype(np.float64).std()
-        net.forward(_as_f(vn.astype(np.float32)))
+        aux = (rng.random(shape) * 3 + 1).astype(np.float32)
+        an = (aux - aux.astype(np.float64).mean()) / aux.astype(np.float64).std()
+        net.forward(_as_f(vn.astype(np.float32)), _as_f(an.astype(np.float32)))
         p0, b0 = net.predict()
-        p1, b1 = net.infer(_as_f(vol), halo=0)
+        p1, b1 = net.infer(_as_f(vol), halo=0, aux=_as_f(aux))
     np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-6)
     assert (b1 == b0).mean() > 0.999

@@ -310,6 +310,10 @@ class SearchPlan:
         """Column-tile widths of the z- and y-FFT passes (8, 16 or 32)."""
         self.ctx.check(self.ctx.lib.tomhip_search_set_tiles(self.h, int(tile_z), int(tile_y)))
 
+    def set_zchunk(self, planes: int = 0):
+        """Run the y-FFT and c2r/max passes over z-chunks (0 = whole volume)."""
+        self.ctx.check(self.ctx.lib.tomhip_search_set_zchunk(self.h, int(planes)))
+
     def set_graph(self, enable: bool):
         """Replay full angle batches through a captured hipGraph (default)."""
         self.ctx.check(self.ctx.lib.tomhip_search_set_graph(self.h, int(enable)))
@@ -569,12 +573,15 @@ class Unet:
             raise ValueError("parameter vector has the wrong length")
         self.ctx.check(self.ctx.lib.tomhip_unet_set_params(self.h, _p(t)))
 
-    def forward(self, vol):
+    def forward(self, vol, aux=None):
+        """vol (and the optional aux map): normalised inputs of the network size."""
         vb = _Buf(vol)
-        if tuple(int(x) for x in vb.dims) != tuple(int(x) for x in self.dims):
-            raise ValueError("volume does not match the network input size")
-        _prep_ctx(self.ctx, vb)
-        self.ctx.check(self.ctx.lib.tomhip_unet_forward(self.h, vb.ptr))
+        ab = _Buf(aux) if aux is not None else None
+        for b in (vb, ab):
+            if b is not None and tuple(int(x) for x in b.dims) != tuple(int(x) for x in self.dims):
+                raise ValueError("input does not match the network input size")
+        _prep_ctx(self.ctx, vb, ab)
+        self.ctx.check(self.ctx.lib.tomhip_unet_forward(self.h, vb.ptr, None if ab is None else ab.ptr))
 
     def logits(self) -> np.ndarray:
         """float32 (nz, ny, nx, 80)."""
@@ -612,15 +619,18 @@ class Unet:
         self.ctx.check(self.ctx.lib.tomhip_unet_predict(self.h, _ptr(prob), _ptr(b)))
         return prob, b
 
-    def infer(self, vol, halo: int = 24):
+    def infer(self, vol, halo: int = 24, aux=None):
         """Whole-volume (prob, bin) maps in overlapping tiles of the network
-        size; the volume is normalised by its global mean and std."""
+        size; the volume (and the aux map) are normalised by their global mean
+        and std."""
         vb = _Buf(vol)
-        ctx = _prep_ctx(self.ctx, vb)
+        ab = _Buf(aux) if aux is not None else None
+        ctx = _prep_ctx(self.ctx, vb, ab)
         N = _dims3(vb.dims)
         prob = _empty_like(vb, vb.dims)
         b = _empty_like(vb, vb.dims, np.int32)
-        ctx.check(ctx.lib.tomhip_unet_infer(self.h, vb.ptr, _p(N), int(halo), _ptr(prob), _ptr(b)))
+        ctx.check(ctx.lib.tomhip_unet_infer(self.h, vb.ptr, None if ab is None else ab.ptr, _p(N), int(halo),
+                                            _ptr(prob), _ptr(b)))
         return prob, b
 
     def close(self):

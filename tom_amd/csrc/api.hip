@@ -427,6 +427,14 @@ int tomhip_search_set_tiles(tomhip_search* h, int tile_z, int tile_y) {
   });
 }
 
+int tomhip_search_set_zchunk(tomhip_search* h, int planes) {
+  tomhip_ctx* ctx = h ? h->ctx : nullptr;
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(h, ERR_WRONG_INPUT, "null pointer");
+    h->s->set_zchunk(planes);
+  });
+}
+
 int tomhip_search_set_graph(tomhip_search* h, int enable) {
   tomhip_ctx* ctx = h ? h->ctx : nullptr;
   return guard(ctx, [&] {
@@ -855,12 +863,13 @@ int tomhip_unet_set_params(tomhip_unet* h, const float* in) {
   });
 }
 
-int tomhip_unet_forward(tomhip_unet* h, const float* volume) {
+int tomhip_unet_forward(tomhip_unet* h, const float* volume, const float* aux) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && volume, ERR_WRONG_INPUT, "null pointer");
     hipStream_t s = h->ctx->stream;
     DevBuf v(s, volume, sizeof(float) * h->V, true, false);
-    h->u->forward(v.get<float>());
+    DevBuf a(s, aux, sizeof(float) * h->V, true, false);
+    h->u->forward(v.get<float>(), aux ? a.get<float>() : nullptr);
     TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -924,18 +933,19 @@ int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin) {
   });
 }
 
-int tomhip_unet_infer(tomhip_unet* h, const float* volume, const int64_t N[3], int halo, float* prob,
-                      int32_t* bin) {
+int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
+                      float* prob, int32_t* bin) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && volume && (prob || bin), ERR_WRONG_INPUT, "null pointer");
     check_dims3(N, "volume");
     const long NN = N[0] * N[1] * N[2];
     hipStream_t s = h->ctx->stream;
     DevBuf v(s, volume, sizeof(float) * NN, true, false);
+    DevBuf a(s, aux, sizeof(float) * NN, true, false);
     DevBuf p(s, prob, sizeof(float) * NN, false, true);
     DevBuf b(s, bin, sizeof(int32_t) * NN, false, true);
-    h->u->infer(v.get<float>(), (int)N[0], (int)N[1], (int)N[2], halo, prob ? p.get<float>() : nullptr,
-                bin ? b.get<int32_t>() : nullptr);
+    h->u->infer(v.get<float>(), aux ? a.get<float>() : nullptr, (int)N[0], (int)N[1], (int)N[2], halo,
+                prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr);
     p.commit();
     b.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));

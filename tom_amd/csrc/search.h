@@ -61,6 +61,10 @@ class Search {
   // Column-tile widths of pass A (z FFTs) and pass B (y FFTs): 8, 16 or 32
   // columns per workgroup (default 16). Results do not depend on them.
   void set_tiles(int tile_a, int tile_b);
+  // Run passes B and C over z-chunks of this many planes (0 = whole volume)
+  // so that C reads B's output from the memory-side cache. Results are
+  // identical.
+  void set_zchunk(int planes);
   bool tiles_ok(int tile_a, int tile_b) const;
 
   void set_profile(bool on) { profile_ = on; }
@@ -85,7 +89,8 @@ class Search {
   void rotate_templates(const float* rm_dev, int count, float* tout, const long* desc, long rm_step);
   void box_filter(int count, float* t, c32* work);
   void launch_passA(int ka);
-  void launch_passB(int ka);
+  void launch_passB(int ka, int z0, int zn);
+  void launch_passC(int ka, int z0, int zn, long first_index, const long* desc);
   void capture_graph();
   void drop_graph();
   template <class F>
@@ -125,6 +130,7 @@ class Search {
   // graph of one full batch, replayed with the device descriptor desc_
   bool use_graph_ = true;
   int tile_a_ = 16, tile_b_ = 16;
+  int zchunk_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   long* desc_ = nullptr;  // {batch, first index}

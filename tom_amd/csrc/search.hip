@@ -317,11 +317,11 @@ __global__ __launch_bounds__(NT_A) void k_passA(const c32* Ty, const c32* Tyn, c
 // the last stage stores global memory directly.
 constexpr int NT_B = 512;
 template <int Y, int TW>
-__global__ __launch_bounds__(NT_B) void k_passB(c32* Q, const c32* twYg, int Z, int Xh) {
+__global__ __launch_bounds__(NT_B) void k_passB(c32* Q, const c32* twYg, int Z, int Xh, int z0) {
   __shared__ c32 buf[Y * TW];
   __shared__ c32 twY[Y];
   load_table<float, Y, NT_B>(twY, twYg);
-  c32* base = Q + ((long)blockIdx.z * Z + blockIdx.y) * Y * Xh + blockIdx.x * TW;
+  c32* base = Q + ((long)blockIdx.z * Z + z0 + blockIdx.y) * Y * Xh + blockIdx.x * TW;
   fft_g2g<float, Y, TW, NT_B, 1, Inter<TW>>(
       buf, twY, [&](int c, int e) { return base[(long)e * Xh + c]; },
       [&](int, int, int c, int e, c32 v) { base[(long)e * Xh + c] = v; });
@@ -343,7 +343,7 @@ constexpr int NT_C = 512;
 // resident maps follows.
 template <int XH>
 __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void k_passC(const c32* Q, float* maxc, int32_t* argm, const c32* twHg,
-                                                const c32* twXg, long nlines, long qstride, int ka,
+                                                const c32* twXg, long line0, long nlines, long qstride, int ka,
                                                 int first_index, const long* desc) {
   if (desc) first_index = (int)(desc[1] + desc[0] * ka);
   constexpr int NL = passc_lines<XH>();
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
   load_table<float, XH, NT_C>(twH, twHg);
   load_table<float, XH, NT_C>(twR, twXg);  // first XH entries of the length-2XH table
   __syncthreads();
-  const long l0 = (long)blockIdx.x * NL;
+  const long l0 = line0 + (long)blockIdx.x * NL;
   const int nlv = (int)((nlines - l0 < NL) ? (nlines - l0) : NL);
   static_assert(IN::R <= 8, "batch indices are packed 4 bits per output");
   float bx[IN::PER][IN::R], by[IN::PER][IN::R];
@@ -506,9 +506,12 @@ __global__ void k_finalize_xcf(const float* maxc, const int32_t* argm, float* sc
 // Exact FLCF of a local orientation set at candidate positions and their
 // neighbours (offsets in {-1, 0, 1}^3): c[k] = sum_u t[u] V[k + u + s - h]
 // (indices modulo the volume, as the FFT path wraps), then the score of
-// k_finalize_flcf. One block per candidate; thread rows (uy, uz) of the
-// template; along ux a sliding window of volume samples serves the three
-// x offsets of each of the 9 (y, z) offset rows.
+// k_finalize_flcf. One block per candidate walks the template planes uz in
+// order; the three volume planes that the z offsets need for plane uz sit in
+// an LDS ring of (ny + 2) x (nx + 2) samples each. Thread t owns template row
+// uy = t / C and an x segment of it; along the segment a sliding 3-sample
+// window per (y, z) offset row serves the three x offsets, so each
+// template sample costs 9 LDS reads for 27 KS fused multiply-adds.
 constexpr int NT_R = 256;
 
 __device__ __forceinline__ int wrapi(int a, int n) {
@@ -523,52 +526,72 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
                                                  const double* __restrict__ mu, const double* __restrict__ sig,
                                                  double inv_m, double mu_t, double sig_t, int a0, int nvalid,
                                                  float* bscore, int32_t* bangle, int32_t* bpos) {
+  extern __shared__ float ring[];  // 3 planes of PH x PW
   const int4 c = cand[blockIdx.x];
   const long n3 = (long)nx * ny * nz;
+  const int PW = nx + 2, PH = ny + 2, PS = PW * PH;
+  const int C = (NT_R / ny) > 0 ? NT_R / ny : 1;  // segments per template row
+  const int SEG = (nx + C - 1) / C;
+  const int uy = threadIdx.x / C, seg = threadIdx.x % C;
+  const int ux0 = seg * SEG, ux1 = min(nx, ux0 + SEG);
+  const bool active = uy < ny && ux0 < ux1;
+  // volume window origin: voxel (c + u + s - h) for u = 0, offset -1 .. +1
+  const int vx0 = c.x + sx - 1, vy0 = c.y + sy - 1, vz0 = c.z + sz - 1;
+  auto load_plane = [&](int slot, int pz) {
+    const int zz = wrapi(vz0 + pz, Z);
+    float* dst = ring + slot * PS;
+    for (int i = threadIdx.x; i < PS; i += NT_R) {
+      const int yy = wrapi(vy0 + i / PW, Y), xx = wrapi(vx0 + i % PW, X);
+      dst[i] = V[((long)zz * Y + yy) * X + xx];
+    }
+  };
   float acc[KS][27];
 #pragma unroll
   for (int k = 0; k < KS; ++k)
 #pragma unroll
     for (int q = 0; q < 27; ++q) acc[k][q] = 0.f;
-  const int rows = ny * nz;
-  for (int row = threadIdx.x; row < rows; row += NT_R) {
-    const int uy = row % ny, uz = row / ny;
-    const float* vr[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int yy = wrapi(c.y + (r % 3 - 1) + uy + sy, Y);
-      const int zz = wrapi(c.z + (r / 3 - 1) + uz + sz, Z);
-      vr[r] = V + ((long)zz * Y + yy) * X;
-    }
-    int xm = wrapi(c.x - 1 + sx, X);
-    int x0 = xm + 1 == X ? 0 : xm + 1;
-    int xi = x0 + 1 == X ? 0 : x0 + 1;
-    float w0[9], w1[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      w0[r] = vr[r][xm];
-      w1[r] = vr[r][x0];
-    }
-    const float* tr = t + (long)c.w * n3 + (long)row * nx;
-    for (int ux = 0; ux < nx; ++ux) {
-      float tv[KS];
-#pragma unroll
-      for (int k = 0; k < KS; ++k) tv[k] = tr[(long)k * G * n3 + ux];
+  const float* tb = t + (long)c.w * n3;
+  // planes pz = uz + oz + 1 (oz in -1..1); slot of plane pz is pz % 3
+  load_plane(0, 0);
+  load_plane(1, 1);
+  for (int uz = 0; uz < nz; ++uz) {
+    load_plane((uz + 2) % 3, uz + 2);
+    __syncthreads();
+    if (active) {
+      const float* rows[9];
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
-        const float w2 = vr[r][xi];
-#pragma unroll
-        for (int k = 0; k < KS; ++k) {
-          acc[k][3 * r] += tv[k] * w0[r];
-          acc[k][3 * r + 1] += tv[k] * w1[r];
-          acc[k][3 * r + 2] += tv[k] * w2;
-        }
-        w0[r] = w1[r];
-        w1[r] = w2;
+        const int oz = r / 3, oy = r % 3;  // 0..2 = offsets -1..1
+        rows[r] = ring + ((uz + oz) % 3) * PS + (uy + oy) * PW;
       }
-      xi = xi + 1 == X ? 0 : xi + 1;
+      float w0[9], w1[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        w0[r] = rows[r][ux0];
+        w1[r] = rows[r][ux0 + 1];
+      }
+      const float* trow = tb + ((long)uz * ny + uy) * nx;
+      for (int ux = ux0; ux < ux1; ++ux) {
+        float tv[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) tv[k] = trow[(long)k * G * n3 + ux];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+          const float w2 = rows[r][ux + 2];
+#pragma unroll
+          for (int k = 0; k < KS; ++k) {
+            acc[k][3 * r] += tv[k] * w0[r];
+            acc[k][3 * r + 1] += tv[k] * w1[r];
+            acc[k][3 * r + 2] += tv[k] * w2;
+          }
+          w0[r] = w1[r];
+          w1[r] = w2;
+        }
+      }
     }
+    __syncthreads();
   }
+  // q = 3 r + ox with r = 3 (oz + 1) + (oy + 1): x-fastest position order
   __shared__ float red[NT_R / 64][KS * 27];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -1044,13 +1067,13 @@ void Search::launch_passA(int ka) {
   TOMHIP_CHECK(hipGetLastError());
 }
 
-void Search::launch_passB(int ka) {
+void Search::launch_passB(int ka, int z0, int zn) {
   dispatch_tile(tile_b_, [&](auto TWc) {
     constexpr int TW = decltype(TWc)::value;
     dispatch_len(Y, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
       if constexpr ((long)L * TW * 8 + (long)L * 8 <= kLdsMax) {
-        hipLaunchKernelGGL((k_passB<L, TW>), dim3(Xh / TW, Z, ka), dim3(NT_B), 0, s_, Q_, tw_.f32(L), Z, Xh);
+        hipLaunchKernelGGL((k_passB<L, TW>), dim3(Xh / TW, zn, ka), dim3(NT_B), 0, s_, Q_, tw_.f32(L), Z, Xh, z0);
       } else {
         throw Error(ERR_UNSUPPORTED, "pass B tile does not fit in LDS");
       }
@@ -1077,17 +1100,32 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index, const long
     TOMHIP_CHECK(hipGetLastError());
   });
   timed(K_PASSA, [&] { launch_passA(ka); });
-  timed(K_PASSB, [&] { launch_passB(ka); });
-  timed(K_PASSC, [&] {
-    dispatch_xh(Xh, [&](auto Lc) {
-      constexpr int L = decltype(Lc)::value;
-      constexpr int NL = passc_lines<L>();
-      const long nl = (long)Y * Z;
-      hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT_C), 0, s_, (const c32*)Q_, maxc_,
-                         argm_, tw_.f32(L), tw_.f32(X), nl, (long)Z * Y * Xh, ka, (int)first_index, desc);
-    });
-    TOMHIP_CHECK(hipGetLastError());
+  // passes B and C by z-chunks: a chunk of Q written by B is read back by C
+  // while it is still in the memory-side cache
+  const int zc = (zchunk_ > 0 && zchunk_ < Z) ? zchunk_ : Z;
+  for (int z0 = 0; z0 < Z; z0 += zc) {
+    const int zn = std::min(zc, Z - z0);
+    timed(K_PASSB, [&] { launch_passB(ka, z0, zn); });
+    timed(K_PASSC, [&] { launch_passC(ka, z0, zn, first_index, desc); });
+  }
+}
+
+void Search::launch_passC(int ka, int z0, int zn, long first_index, const long* desc) {
+  dispatch_xh(Xh, [&](auto Lc) {
+    constexpr int L = decltype(Lc)::value;
+    constexpr int NL = passc_lines<L>();
+    const long line0 = (long)z0 * Y, line1 = (long)(z0 + zn) * Y;
+    const long nl = line1 - line0;
+    hipLaunchKernelGGL(k_passC<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT_C), 0, s_, (const c32*)Q_, maxc_,
+                       argm_, tw_.f32(L), tw_.f32(X), line0, line1, (long)Z * Y * Xh, ka, (int)first_index, desc);
   });
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+void Search::set_zchunk(int planes) {
+  TOMHIP_REQUIRE(planes >= 0, ERR_WRONG_INPUT, "z chunk must be >= 0");
+  if (planes != zchunk_) drop_graph();
+  zchunk_ = planes;
 }
 
 void Search::finalize(float* score, int32_t* angle) {
@@ -1109,7 +1147,7 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
   TOMHIP_REQUIRE(started_, ERR_WRONG_INPUT, "set the template before refining");
   TOMHIP_REQUIRE(G >= 1 && A >= 1 && P >= 1, ERR_WRONG_INPUT, "empty refinement");
   TOMHIP_REQUIRE((long)G * A <= (1L << 24) && P <= (1L << 30), ERR_WRONG_INPUT, "refinement too large");
-  constexpr int KS = 2;
+  constexpr int KS = 4;
   const long n3 = (long)nx * ny * nz;
   const long ntp = (long)KS * G;
   float* rm = nullptr;
@@ -1134,7 +1172,8 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
       const int cnt = std::min(KS, A - a0);
       rotate_templates(rm + 9L * a0 * G, cnt * G, tt, nullptr, 0);
       if (wsh_) box_filter(cnt * G, tt, work);
-      hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P), dim3(NT_R), 0, s_, volume, X, Y, Z, (const float*)tt, G,
+      const size_t lds = sizeof(float) * 3 * (nx + 2) * (ny + 2);
+      hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P), dim3(NT_R), lds, s_, volume, X, Y, Z, (const float*)tt, G,
                          nx, ny, nz, sx, sy, sz, reinterpret_cast<const int4*>(cand_dev), (const double*)mu_,
                          (const double*)sig_, inv_m, mu_t, sig_t, a0, cnt, bscore, bangle, bpos);
       TOMHIP_CHECK(hipGetLastError());

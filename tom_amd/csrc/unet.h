@@ -28,8 +28,9 @@ class Unet {
   // all parameters in layer order, each layer W [taps][Cout][Cin] then bias
   void params(float* host, bool get);
   void grads(float* host);
-  // vol: device f32 [D][H][W] (x fastest), already normalised
-  void forward(const float* vol);
+  // vol: device f32 [D][H][W] (x fastest), already normalised; aux (optional,
+  // same layout, normalised): a second input channel
+  void forward(const float* vol, const float* aux);
   double forward_flops() const;
   // tprob: device f32 [V] particle targets in [0, 1]; tbin: device int32 [V]
   // bin index or -1; npos = #{tbin >= 0}. loss_out (host, optional) gets
@@ -37,10 +38,11 @@ class Unet {
   void backward(const float* tprob, const int32_t* tbin, float wpos, long npos, double* loss_out);
   void adam(float lr);
   void predict(float* prob, int32_t* bin);
-  // whole volume vol (device f32 X x Y x Z), normalised here by its global
-  // mean and standard deviation, in tiles of the network size with halo
-  // voxels of context on each side; prob / bin (device) receive every voxel
-  void infer(const float* vol, int X, int Y, int Z, int halo, float* prob, int32_t* bin);
+  // whole volume vol (device f32 X x Y x Z) and optional aux map of the same
+  // size, each normalised here by its global mean and standard deviation, in
+  // tiles of the network size with halo voxels of context on each side;
+  // prob / bin (device) receive every voxel
+  void infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin);
   const float* logits() const;  // device f32 [V][80]
   int D() const;
   int H() const;

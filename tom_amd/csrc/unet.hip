@@ -1,5 +1,5 @@
 // Learned proposal network: a 3-level 3D U-Net (16/32/64 channels, two 3x3x3
-// conv + ReLU per level, max-pool down, trilinear x2 up, skip concatenation)
+// conv + ReLU per level, 2x2x2 average pool down, trilinear x2 up, skip concatenation)
 // with a 1x1x1 head giving one particle logit and NBIN orientation-bin logits
 // per voxel. No reference counterpart: this is the surrogate that proposes
 // candidates so the exact FLCF only runs as a local refinement.
@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -119,6 +120,99 @@ __global__ __launch_bounds__(NTU) void k_conv(const uint16_t* __restrict__ X, in
   }
 }
 
+// LDS-tiled convolution for the full-resolution layers: a block stages its
+// input tile of (16 + 2) x (4 + 2) x (TZ + 2) voxels x Cin channels (zero
+// outside the volume) in LDS once; wave w owns row y0 + w of the tile and its
+// TZ planes of 16 x voxels, so every A fragment is an LDS read and one weight
+// fragment (global, cache-resident) feeds TZ x CT MFMAs. Channel pitch Cin + 4
+// in LDS keeps 8-byte fragments aligned and spreads the voxel rows over banks.
+constexpr int CTX = 16, CTY = 4;
+template <int CT, int CTZ>
+__global__ __launch_bounds__(NTU) void k_conv_t(const uint16_t* __restrict__ X, int xs, int xo, int Cin, Dims d,
+                                                const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                                int taps, int relu, uint16_t* Yb, int ys, int yo, float* Yf) {
+  extern __shared__ uint16_t tile[];
+  constexpr int Cout = 16 * CT;
+  constexpr int SX = CTX + 2, SY = CTY + 2, SZ = CTZ + 2;
+  const int CP = Cin + 4;
+  const int nbx = (d.W + CTX - 1) / CTX, nby = (d.H + CTY - 1) / CTY;
+  const int x0 = (blockIdx.x % nbx) * CTX, y0 = ((blockIdx.x / nbx) % nby) * CTY, z0 = (blockIdx.x / (nbx * nby)) * CTZ;
+  const int nch = Cin / 4;
+  const int total = SX * SY * SZ * nch;
+  for (int i = threadIdx.x; i < total; i += NTU) {
+    const int ch = i % nch, v = i / nch;
+    const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / (SX * SY) - 1;
+    uint2 val = make_uint2(0u, 0u);
+    if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D)
+      val = *reinterpret_cast<const uint2*>(X + (((long)gz * d.H + gy) * d.W + gx) * xs + xo + ch * 4);
+    *reinterpret_cast<uint2*>(tile + (long)v * CP + ch * 4) = val;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const int y = y0 + wv;
+  if (y >= d.H) return;  // no barrier follows
+  f32x4 acc[CTZ][CT];
+#pragma unroll
+  for (int zp = 0; zp < CTZ; ++zp)
+#pragma unroll
+    for (int j = 0; j < CT; ++j) acc[zp][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int KS = Cin / 16;
+  for (int t = 0; t < taps; ++t) {
+    int dz = 0, dy = 0, dx = 0;
+    if (taps == 27) {
+      dz = t / 9 - 1;
+      dy = (t / 3) % 3 - 1;
+      dx = t % 3 - 1;
+    }
+    const uint16_t* wp = Wt + ((long)t * Cout + row) * Cin;
+    const uint16_t* ap = tile + (long)(((dz + 1) * SY + (wv + dy + 1)) * SX + (row + dx + 1)) * CP;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int c = ks * 16 + kq * 4;
+      bf16x4 b[CT];
+#pragma unroll
+      for (int j = 0; j < CT; ++j) {
+        const uint2 rw = *reinterpret_cast<const uint2*>(wp + (long)j * 16 * Cin + c);
+        b[j][0] = (short)(rw.x & 0xffff);
+        b[j][1] = (short)(rw.x >> 16);
+        b[j][2] = (short)(rw.y & 0xffff);
+        b[j][3] = (short)(rw.y >> 16);
+      }
+#pragma unroll
+      for (int zp = 0; zp < CTZ; ++zp) {
+        const uint2 raw = *reinterpret_cast<const uint2*>(ap + (long)zp * SY * SX * CP + c);
+        bf16x4 a;
+        a[0] = (short)(raw.x & 0xffff);
+        a[1] = (short)(raw.x >> 16);
+        a[2] = (short)(raw.y & 0xffff);
+        a[3] = (short)(raw.y >> 16);
+#pragma unroll
+        for (int j = 0; j < CT; ++j) acc[zp][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[j], acc[zp][j], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int zp = 0; zp < CTZ; ++zp) {
+    const int z = z0 + zp;
+    if (z >= d.D) continue;
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int co = j * 16 + row;
+      const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + 4 * kq + i;
+        if (x >= d.W) continue;
+        float v = acc[zp][j][i] + bb;
+        if (relu) v = fmaxf(v, 0.f);
+        const long vox = ((long)z * d.H + y) * d.W + x;
+        if (Yb) Yb[vox * ys + yo + co] = f2bf(v);
+        if (Yf) Yf[vox * Cout + co] = v;
+      }
+    }
+  }
+}
+
 // dW[t][co][ci] += sum over a chunk of voxels of dY[v][co] X[v + d_t][ci].
 // grid: x = voxel chunks (VCH voxels each), y = taps * (Cout/16) * (Cin/16).
 constexpr int VCH = 1024;
@@ -172,10 +266,13 @@ __global__ __launch_bounds__(64) void k_conv_dw(const uint16_t* __restrict__ X, 
 }
 
 // ---------------------------------------------------------------- elementwise
-__global__ void k_input(const float* vol, long V, uint16_t* X0) {
-  // channel 0 = the (normalised) tomogram, channels 1..15 = 0
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < V * 16; i += (long)gridDim.x * NTU)
-    X0[i] = (i % 16 == 0) ? f2bf(vol[i / 16]) : 0;
+__global__ void k_input(const float* vol, const float* aux, long V, uint16_t* X0) {
+  // channel 0 = the (normalised) tomogram, channel 1 = the auxiliary map
+  // (or 0), channels 2..15 = 0
+  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < V * 16; i += (long)gridDim.x * NTU) {
+    const int c = (int)(i % 16);
+    X0[i] = c == 0 ? f2bf(vol[i / 16]) : (c == 1 && aux) ? f2bf(aux[i / 16]) : (uint16_t)0;
+  }
 }
 
 // relu backward: dpre = dpost * (post > 0) -> bf16 for the MFMA, + bias grad
@@ -199,40 +296,37 @@ __global__ void k_relu_bwd(const float* dpost, const uint16_t* post, int ps, int
   if (threadIdx.x == 0) atomicAdd(&dbias[c], red[0]);
 }
 
-// 2x2x2 max pool, channels-last; records the winning offset (0..7)
-__global__ void k_pool(const uint16_t* X, int xs, int xo, int C, Dims d, uint16_t* Y, uint8_t* arg) {
+// 2x2x2 average pool, channels-last (an average integrates the signal of a
+// low-SNR tomogram; a maximum would pick up noise)
+__global__ void k_pool(const uint16_t* X, int xs, int xo, int C, Dims d, uint16_t* Y) {
   const Dims o{d.D / 2, d.H / 2, d.W / 2};
   const long n = o.V() * C;
   for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
     const int c = (int)(i % C);
     const long v = i / C;
     const int x = (int)(v % o.W), y = (int)((v / o.W) % o.H), z = (int)(v / ((long)o.W * o.H));
-    float best = -FLT_MAX;
-    int bk = 0;
+    float sum = 0.f;
     for (int k = 0; k < 8; ++k) {
       const long s = (((long)(2 * z + (k >> 2)) * d.H + (2 * y + ((k >> 1) & 1))) * d.W + (2 * x + (k & 1)));
-      const float val = bf2f(X[s * xs + xo + c]);
-      if (val > best) {
-        best = val;
-        bk = k;
-      }
+      sum += bf2f(X[s * xs + xo + c]);
     }
-    Y[v * C + c] = f2bf(best);
-    arg[i] = (uint8_t)bk;
+    Y[v * C + c] = f2bf(sum * 0.125f);
   }
 }
 
-// pool backward: dX (f32, [Vin][C], accumulated) += dY routed to the winner
-__global__ void k_pool_bwd(const float* dY, const uint8_t* arg, int C, Dims d, float* dX) {
+// pool backward: dX (f32, [Vin][C], accumulated) += dY / 8 on each of the 8
+__global__ void k_pool_bwd(const float* dY, int C, Dims d, float* dX) {
   const Dims o{d.D / 2, d.H / 2, d.W / 2};
   const long n = o.V() * C;
   for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
     const int c = (int)(i % C);
     const long v = i / C;
     const int x = (int)(v % o.W), y = (int)((v / o.W) % o.H), z = (int)(v / ((long)o.W * o.H));
-    const int k = arg[i];
-    const long s = (((long)(2 * z + (k >> 2)) * d.H + (2 * y + ((k >> 1) & 1))) * d.W + (2 * x + (k & 1)));
-    dX[s * C + c] += dY[i];
+    const float g = dY[i] * 0.125f;
+    for (int k = 0; k < 8; ++k) {
+      const long s = (((long)(2 * z + (k >> 2)) * d.H + (2 * y + ((k >> 1) & 1))) * d.W + (2 * x + (k & 1)));
+      dX[s * C + c] += g;
+    }
   }
 }
 
@@ -463,19 +557,49 @@ void conv_forward(const uint16_t* X, int xs, int xo, int Cin, int D, int H, int 
                  ERR_UNSUPPORTED, "conv: channels must be multiples of 16 (Cout <= 96), taps 27 or 1");
   TOMHIP_REQUIRE(xo % 4 == 0 && xs % 4 == 0, ERR_UNSUPPORTED, "conv: input channel offset/stride must be multiples of 4");
   Dims d{D, H, W};
-  dim3 g((unsigned)(((W + 15) / 16) * ((H + 3) / 4) * D));
-  switch (Cout / 16) {
+  // LDS tile of (CTZ + 2) planes; 4 planes per wave when the tile fits in
+  // 64 KB, else 2
+  auto lds_for = [&](int tz) { return sizeof(uint16_t) * (CTX + 2) * (CTY + 2) * (tz + 2) * (size_t)(Cin + 4); };
+  const int tz = lds_for(4) <= 65536 ? 4 : 2;
+  const size_t lds = lds_for(tz);
+  if (lds <= 65536) {
+    dim3 g((unsigned)(((W + CTX - 1) / CTX) * ((H + CTY - 1) / CTY) * ((D + tz - 1) / tz)));
+    auto go = [&](auto TZc) {
+      constexpr int TZ = decltype(TZc)::value;
+      switch (Cout / 16) {
+#define TOMHIP_CONV_CASE(ct)                                                                                     \
+  case ct:                                                                                                    \
+    hipLaunchKernelGGL((k_conv_t<ct, TZ>), g, dim3(NTU), lds, s, X, xs, xo, Cin, d, Wt, bias, taps, relu, Yb, ys, \
+                       yo, Yf);                                                                               \
+    break;
+        TOMHIP_CONV_CASE(1)
+        TOMHIP_CONV_CASE(2)
+        TOMHIP_CONV_CASE(3)
+        TOMHIP_CONV_CASE(4)
+        TOMHIP_CONV_CASE(5)
+        TOMHIP_CONV_CASE(6)
+#undef TOMHIP_CONV_CASE
+      }
+    };
+    if (tz == 4)
+      go(std::integral_constant<int, 4>{});
+    else
+      go(std::integral_constant<int, 2>{});
+  } else {
+    dim3 g((unsigned)(((W + 15) / 16) * ((H + 3) / 4) * D));
+    switch (Cout / 16) {
 #define TOMHIP_CONV_CASE(ct)                                                                                     \
   case ct:                                                                                                    \
     hipLaunchKernelGGL(k_conv<ct>, g, dim3(NTU), 0, s, X, xs, xo, Cin, d, Wt, bias, taps, relu, Yb, ys, yo, Yf); \
     break;
-    TOMHIP_CONV_CASE(1)
-    TOMHIP_CONV_CASE(2)
-    TOMHIP_CONV_CASE(3)
-    TOMHIP_CONV_CASE(4)
-    TOMHIP_CONV_CASE(5)
-    TOMHIP_CONV_CASE(6)
+      TOMHIP_CONV_CASE(1)
+      TOMHIP_CONV_CASE(2)
+      TOMHIP_CONV_CASE(3)
+      TOMHIP_CONV_CASE(4)
+      TOMHIP_CONV_CASE(5)
+      TOMHIP_CONV_CASE(6)
 #undef TOMHIP_CONV_CASE
+    }
   }
   TOMHIP_CHECK(hipGetLastError());
 }
@@ -508,7 +632,6 @@ struct Unet::Impl {
   std::vector<ConvLayer> L;
   // activations (bf16)
   uint16_t *x0, *a1, *c1, *p1, *a2, *c2, *p2, *b1, *b2, *d2a, *d2b, *d1a, *d1b;
-  uint8_t *arg1, *arg2;
   float* logits;
   // gradients (f32) and bf16 pre-activation gradients
   float *g_logits, *g_d1b, *g_d1a, *g_c1, *g_d2b, *g_d2a, *g_c2, *g_b2, *g_b1, *g_p2, *g_s2, *g_a2, *g_p1, *g_s1,
@@ -519,6 +642,7 @@ struct Unet::Impl {
   int step = 0;
   // tiled inference buffers
   float* tile_in = nullptr;
+  float* tile_aux = nullptr;
   float* tile_prob = nullptr;
   int32_t* tile_bin = nullptr;
   double* sums = nullptr;
@@ -568,13 +692,13 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
     c.wb = I.alloc<uint16_t>(nw);
     c.wf = I.alloc<uint16_t>(nw);
     // He initialisation over the real fan-in (the first layer sees 1 channel)
-    const int fan_in = c.taps * (I.L.empty() ? 1 : c.Cin);
+    const int fan_in = c.taps * (I.L.empty() ? 2 : c.Cin);
     std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan_in));
     std::vector<float> h(c.np, 0.f);
     for (long i = 0; i < nw; ++i) {
       const int ci = (int)(i % c.Cin);
       const int co = (int)((i / c.Cin) % c.Cout);
-      if (I.L.empty() && ci != 0) continue;          // padded input channels
+      if (I.L.empty() && ci > 1) continue;           // padded input channels
       if (c.taps == 1 && co > nbin) continue;        // unused head channels
       h[i] = nd(rng);
     }
@@ -597,8 +721,6 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   I.d2b = I.alloc<uint16_t>(V1 * 32);
   I.d1a = I.alloc<uint16_t>(V0 * 16);
   I.d1b = I.alloc<uint16_t>(V0 * 16);
-  I.arg1 = I.alloc<uint8_t>(V1 * 16);
-  I.arg2 = I.alloc<uint8_t>(V2 * 32);
   I.logits = I.alloc<float>(V0 * kHeadCh);
   I.loss = I.alloc<double>(2);
   I.train = train;
@@ -672,12 +794,12 @@ void Unet::grads(float* out) {
   TOMHIP_CHECK(hipStreamSynchronize(I.s));
 }
 
-void Unet::forward(const float* vol) {
+void Unet::forward(const float* vol, const float* aux) {
   Impl& I = *d_;
   hipStream_t s = I.s;
   const int D = I.D, H = I.H, W = I.W;
   auto& L = I.L;
-  hipLaunchKernelGGL(k_input, dim3(gridv(I.V0 * 16)), dim3(NTU), 0, s, vol, I.V0, I.x0);
+  hipLaunchKernelGGL(k_input, dim3(gridv(I.V0 * 16)), dim3(NTU), 0, s, vol, aux, I.V0, I.x0);
   auto conv = [&](int li, const uint16_t* X, int xs, int xo, int d, int h, int w, uint16_t* Yb, int ys, int yo,
                   float* Yf) {
     ConvLayer& c = L[li];
@@ -687,10 +809,10 @@ void Unet::forward(const float* vol) {
   conv(0, I.x0, 16, 0, D, H, W, I.a1, 16, 0, nullptr);
   conv(1, I.a1, 16, 0, D, H, W, I.c1, 48, 32, nullptr);  // skip1 -> c1[32:48]
   Dims d0{D, H, W}, d1{D / 2, H / 2, W / 2}, d2{D / 4, H / 4, W / 4};
-  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, I.c1, 48, 32, 16, d0, I.p1, I.arg1);
+  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, I.c1, 48, 32, 16, d0, I.p1);
   conv(2, I.p1, 16, 0, d1.D, d1.H, d1.W, I.a2, 32, 0, nullptr);
   conv(3, I.a2, 32, 0, d1.D, d1.H, d1.W, I.c2, 96, 64, nullptr);  // skip2 -> c2[64:96]
-  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, I.c2, 96, 64, 32, d1, I.p2, I.arg2);
+  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, I.c2, 96, 64, 32, d1, I.p2);
   conv(4, I.p2, 32, 0, d2.D, d2.H, d2.W, I.b1, 64, 0, nullptr);
   conv(5, I.b1, 64, 0, d2.D, d2.H, d2.W, I.b2, 64, 0, nullptr);
   hipLaunchKernelGGL(k_up, dim3(gridv(I.V1 * 64)), dim3(NTU), 0, s, I.b2, 64, d2, I.c2, 96, 0);
@@ -769,12 +891,10 @@ void Unet::backward(const float* tprob, const int32_t* tbin, float wpos, long np
                      I.g_b2);
   layer_bwd(5, I.g_b2, I.b2, 64, 0, I.b1, 64, 0, d2, I.g_b1);
   layer_bwd(4, I.g_b1, I.b1, 64, 0, I.p2, 32, 0, d2, I.g_p2);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, (const float*)I.g_p2,
-                     (const uint8_t*)I.arg2, 32, d1, I.g_s2);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, (const float*)I.g_p2, 32, d1, I.g_s2);
   layer_bwd(3, I.g_s2, I.c2, 96, 64, I.a2, 32, 0, d1, I.g_a2);
   layer_bwd(2, I.g_a2, I.a2, 32, 0, I.p1, 16, 0, d1, I.g_p1);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, (const float*)I.g_p1,
-                     (const uint8_t*)I.arg1, 16, d0, I.g_s1);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, (const float*)I.g_p1, 16, d0, I.g_s1);
   layer_bwd(1, I.g_s1, I.c1, 48, 32, I.a1, 16, 0, d0, I.g_a1);
   layer_bwd(0, I.g_a1, I.a1, 16, 0, I.x0, 16, 0, d0, nullptr);
   TOMHIP_CHECK(hipGetLastError());
@@ -800,7 +920,7 @@ void Unet::adam(float lr) {
   refresh_weights();
 }
 
-void Unet::infer(const float* vol, int X, int Y, int Z, int halo, float* prob, int32_t* bin) {
+void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin) {
   Impl& I = *d_;
   hipStream_t s = I.s;
   TOMHIP_REQUIRE(halo >= 0 && halo % 4 == 0, ERR_WRONG_INPUT, "halo must be a non-negative multiple of 4");
@@ -810,18 +930,23 @@ void Unet::infer(const float* vol, int X, int Y, int Z, int halo, float* prob, i
     I.tile_in = I.alloc<float>(I.V0);
     I.tile_prob = I.alloc<float>(I.V0);
     I.tile_bin = I.alloc<int32_t>(I.V0);
-    I.sums = I.alloc<double>(2);
+    I.sums = I.alloc<double>(4);
   }
+  if (aux && !I.tile_aux) I.tile_aux = I.alloc<float>(I.V0);
   const long n = (long)X * Y * Z;
-  TOMHIP_CHECK(hipMemsetAsync(I.sums, 0, sizeof(double) * 2, s));
+  TOMHIP_CHECK(hipMemsetAsync(I.sums, 0, sizeof(double) * 4, s));
   hipLaunchKernelGGL(k_vsum, dim3(gridv(n)), dim3(NTU), 0, s, vol, n, I.sums);
+  if (aux) hipLaunchKernelGGL(k_vsum, dim3(gridv(n)), dim3(NTU), 0, s, aux, n, I.sums + 2);
   const Dims t{I.D, I.H, I.W};
   for (int oz = -halo; oz + halo < Z; oz += cz)
     for (int oy = -halo; oy + halo < Y; oy += cy)
       for (int ox = -halo; ox + halo < X; ox += cx) {
         hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, vol, X, Y, Z, ox, oy, oz, t,
                            (const double*)I.sums, n, I.tile_in);
-        forward(I.tile_in);
+        if (aux)
+          hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, aux, X, Y, Z, ox, oy, oz, t,
+                             (const double*)(I.sums + 2), n, I.tile_aux);
+        forward(I.tile_in, aux ? I.tile_aux : nullptr);
         predict(I.tile_prob, I.tile_bin);
         hipLaunchKernelGGL(k_tile_out, dim3(gridv((long)cx * cy * cz)), dim3(NTU), 0, s, (const float*)I.tile_prob,
                            (const int32_t*)I.tile_bin, t, halo, cx, cy, cz, ox, oy, oz, X, Y, Z, prob, bin);

@@ -29,6 +29,31 @@ def unet_forward_flops(dims) -> float:
     return sum(2.0 * V / 8 ** lvl * t * ci * co for ci, co, t, lvl in UNET_LAYERS)
 
 
+# ---------------------------------------------------------------- auxiliary input
+def spherical_average(T):
+    """Rotational average of a template (x, y, z) about its centre floor(n/2):
+    each voxel takes the mean of the voxels at the same rounded distance."""
+    T = np.asarray(T, np.float64)
+    g = np.meshgrid(*[np.arange(s) - s // 2 for s in T.shape], indexing="ij")
+    r = np.rint(np.sqrt(sum(x * x for x in g))).astype(np.int64)
+    sums = np.bincount(r.ravel(), T.ravel())
+    cnt = np.bincount(r.ravel())
+    return (sums / np.maximum(cnt, 1))[r].astype(np.float32)
+
+
+def aux_map(plan, template, mask, W=None):
+    """FLCF score map of the rotationally averaged template (one orientation
+    of the exhaustive machinery): an orientation-free matched filter that the
+    network gets as its second input channel. Leaves the plan's template set to
+    the averaged one."""
+    Ta = spherical_average(template)
+    tn, mu, sg = api.prepare_template(Ta, mask, W)
+    plan.set_template(tn, W, mu, sg)
+    plan.run(np.zeros((1, 3), np.float32), 0)
+    sc, _ = plan.finalize()
+    return sc
+
+
 # ---------------------------------------------------------------- labels
 def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_peaks: int = 1000):
     """Picklist (tom_os3_returnPicklist: border erase, tom_peakc radius
@@ -41,24 +66,50 @@ def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_pea
     return pk
 
 
-def label_volumes(shape_zyx, peaks, angles, bins, radius: float = 2.0):
-    """Targets for the proposal network: tprob = 1 within `radius` of each
-    peak, tbin = bin of the peak's best orientation there (-1 elsewhere).
+def label_volumes(shape_zyx, peaks, angles, bins, sigma: float = 3.0, bin_radius: float = 4.0):
+    """Targets for the proposal network: tprob = exp(-d^2 / (2 sigma^2)) of
+    the distance d to the nearest peak (0 beyond 3 sigma), tbin = bin of that
+    peak's best orientation within bin_radius of it (-1 elsewhere).
     peaks: (P, 5) rows x y z value angle-index; angles: the search's list."""
     tprob = np.zeros(shape_zyx, np.float32)
     tbin = np.full(shape_zyx, -1, np.int32)
     if len(peaks) == 0:
         return tprob, tbin
     b = orient.assign_bins(angles[peaks[:, 4].astype(np.int64)], bins)
-    r = int(np.ceil(radius))
-    off = np.array([(dz, dy, dx) for dz in range(-r, r + 1) for dy in range(-r, r + 1) for dx in range(-r, r + 1)
-                    if dx * dx + dy * dy + dz * dz <= radius * radius])
+    r = int(np.ceil(3 * sigma))
+    g = np.arange(-r, r + 1)
+    dz, dy, dx = np.meshgrid(g, g, g, indexing="ij")
+    d2 = (dx * dx + dy * dy + dz * dz).ravel()
+    keep = d2 <= (3 * sigma) ** 2
+    off = np.stack([dz.ravel(), dy.ravel(), dx.ravel()], 1)[keep]
+    val = np.exp(-d2[keep] / (2 * sigma * sigma)).astype(np.float32)
+    inb = d2[keep] <= bin_radius * bin_radius
     Z, Y, X = shape_zyx
+    best = np.zeros(shape_zyx, np.float32)
     for (x, y, z), bb in zip(peaks[:, :3].astype(np.int64), b):
         zz, yy, xx = z + off[:, 0], y + off[:, 1], x + off[:, 2]
         ok = (zz >= 0) & (zz < Z) & (yy >= 0) & (yy < Y) & (xx >= 0) & (xx < X)
-        tprob[zz[ok], yy[ok], xx[ok]] = 1.0
-        tbin[zz[ok], yy[ok], xx[ok]] = bb
+        zz, yy, xx, v, ib = zz[ok], yy[ok], xx[ok], val[ok], inb[ok]
+        up = v > best[zz, yy, xx]
+        best[zz[up], yy[up], xx[up]] = v[up]
+        sel = up & ib
+        tbin[zz[sel], yy[sel], xx[sel]] = bb
+    tprob[:] = best
+    return tprob, tbin
+
+
+def dense_targets(score_zyx, angle_zyx, angles, bins, s0: float = 0.1, s1: float = 0.5, bin_thr: float = 0.25):
+    """Dense targets from the exhaustive maps themselves: tprob = clip((S - s0)
+    / (s1 - s0), 0, 1) of the best FLCF score S at every voxel, tbin = bin of
+    the voxel's best orientation where S > bin_thr (-1 elsewhere)."""
+    S = np.asarray(score_zyx, np.float32)
+    tprob = np.clip((S - s0) / (s1 - s0), 0.0, 1.0).astype(np.float32)
+    tbin = np.full(S.shape, -1, np.int32)
+    sel = S > bin_thr
+    if sel.any():
+        a = np.asarray(angle_zyx)[sel].astype(np.int64)
+        ua, inv = np.unique(a, return_inverse=True)
+        tbin[sel] = orient.assign_bins(np.asarray(angles)[ua], bins)[inv]
     return tprob, tbin
 
 
@@ -68,24 +119,28 @@ class Trainer:
     tomograms. Volumes are normalised by their global mean and std, as
     Unet.infer does."""
 
-    def __init__(self, crop=64, nbin=64, seed=0, lr=1e-3, wpos=20.0):
+    def __init__(self, crop=64, nbin=64, seed=0, lr=1e-3, wpos=20.0, lr_min=5e-5):
         self.crop = int(crop)
         self.net = api.Unet((crop, crop, crop), nbin=nbin, seed=seed, train=True)
         self.rng = np.random.default_rng(seed)
-        self.lr = lr
+        self.lr, self.lr_min = lr, lr_min
         self.wpos = wpos
         self.data = []
         self.history = []
 
-    def add(self, volume_zyx, tprob, tbin):
-        """volume_zyx: numpy (z, y, x) float32; targets of the same shape."""
-        v = np.asarray(volume_zyx, np.float64)
-        vn = ((v - v.mean()) / v.std()).astype(np.float32)
-        pos = np.argwhere(tprob > 0)
-        self.data.append((vn, tprob, tbin, pos))
+    def add(self, volume_zyx, tprob, tbin, aux_zyx=None):
+        """volume_zyx (and the aux map): numpy (z, y, x) float32; targets of
+        the same shape."""
+        def norm(a):
+            a = np.asarray(a, np.float64)
+            return ((a - a.mean()) / a.std()).astype(np.float32)
+        vn = norm(volume_zyx)
+        an = norm(aux_zyx) if aux_zyx is not None else None
+        pos = np.argwhere(tprob > 0.5)
+        self.data.append((vn, tprob, tbin, pos, an))
 
     def _crop(self):
-        vn, tp, tb, pos = self.data[self.rng.integers(len(self.data))]
+        vn, tp, tb, pos, an = self.data[self.rng.integers(len(self.data))]
         c = self.crop
         Z, Y, X = vn.shape
         if len(pos) and self.rng.random() < 0.7:
@@ -95,20 +150,22 @@ class Trainer:
         z, y, x = (int(np.clip(z, 0, Z - c)), int(np.clip(y, 0, Y - c)), int(np.clip(x, 0, X - c)))
         sl = (slice(z, z + c), slice(y, y + c), slice(x, x + c))
         f = lambda a: np.asfortranarray(np.ascontiguousarray(a[sl]).transpose(2, 1, 0))  # noqa: E731
-        return f(vn), f(tp), f(tb)
+        return f(vn), f(tp), f(tb), (f(an) if an is not None else None)
 
-    def step(self):
-        v, tp, tb = self._crop()
-        self.net.forward(v)
+    def step(self, lr=None):
+        v, tp, tb, a = self._crop()
+        self.net.forward(v, a)
         bce, ce = self.net.backward(tp, tb, self.wpos)
-        self.net.adam(self.lr)
+        self.net.adam(self.lr if lr is None else lr)
         self.history.append((bce, ce))
         return bce, ce
 
     def train(self, steps, log=None, every=100):
+        """Cosine learning-rate decay from lr to lr_min over the steps."""
         t0 = time.perf_counter()
         for i in range(steps):
-            bce, ce = self.step()
+            lr = self.lr_min + 0.5 * (self.lr - self.lr_min) * (1 + np.cos(np.pi * i / max(1, steps - 1)))
+            bce, ce = self.step(lr)
             if log and (i % every == 0 or i == steps - 1):
                 log(f"[train] step {i}: bce {bce:.4f} ce {ce:.4f} ({time.perf_counter() - t0:.1f} s)")
         return time.perf_counter() - t0
@@ -161,12 +218,13 @@ class Surrogate:
             self._net_dims = want
         return self._net, halo
 
-    def propose(self, volume, K=2000, nms_radius=12.0, threshold=0.05):
-        """Top-K candidates (K, 5): x y z bin prob."""
+    def propose(self, volume, K=2000, nms_radius=12.0, threshold=0.05, aux=None):
+        """Top-K candidates (K, 5): x y z bin prob. aux: the aux_map of the
+        volume (when the network was trained with one)."""
         dims = tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume)
         net, halo = self._network(dims)
         t0 = time.perf_counter()
-        prob, b = net.infer(volume, halo)
+        prob, b = net.infer(volume, halo, aux=aux)
         _sync(prob)
         t1 = time.perf_counter()
         lm = api.local_maxima(prob, 2, threshold)

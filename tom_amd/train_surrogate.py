@@ -7,7 +7,12 @@ the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
 missing wedge of a +-60 degree tilt series), each searched exhaustively on the
 GPU at 15 degrees; the picklist of the score map (tom_os3_returnPicklist
 semantics) above --label-threshold gives the particle targets and the best
-orientation there the bin targets. The template, mask and wedge weight are
+orientation there the bin targets (a Gaussian heat map of sigma 3 voxels around
+each peak; bins within 4 voxels), on top of dense targets from the exhaustive
+maps at every voxel (the best score mapped linearly from [0.1, 0.5] to [0, 1],
+and the bin of the best orientation wherever the score exceeds 0.25). The
+network's second input channel is the FLCF map of the rotationally averaged
+template (surrogate.aux_map). The template, mask and wedge weight are
 those of the 512x512x256 / 48^3 benchmark workload (seed 2), so the trained
 network proposes candidates for that search.
 """
@@ -47,10 +52,15 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
         plan.set_template(tn, W, mu, sg)
         plan.run(angles, 0)
         score, ang = plan.finalize()
+        aux = surrogate.aux_map(plan, T, M, W).cpu().numpy()
     peaks = surrogate.exhaustive_peaks(score, ang, T.shape[0], thr)
     vz = V.cpu().numpy()
-    tprob, tbin = surrogate.label_volumes(vz.shape, peaks, angles, bins)
-    return vz, tprob, tbin, peaks, pos
+    tp_peak, tb_peak = surrogate.label_volumes(vz.shape, peaks, angles, bins)
+    tp_dense, tb_dense = surrogate.dense_targets(score.cpu().numpy(), ang.cpu().numpy(), angles, bins)
+    # dense FLCF targets everywhere, the peak heat map on top at the picks
+    tprob = np.maximum(tp_dense, tp_peak)
+    tbin = np.where(tb_peak >= 0, tb_peak, tb_dense)
+    return vz, tprob, tbin, peaks, pos, aux
 
 
 def main(argv=None):
@@ -61,11 +71,11 @@ def main(argv=None):
     ap.add_argument("--particles", type=int, default=80)
     ap.add_argument("--snr-lo", type=float, default=0.05)
     ap.add_argument("--snr-hi", type=float, default=0.3)
-    ap.add_argument("--label-threshold", type=float, default=0.2)
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--crop", type=int, default=64)
+    ap.add_argument("--label-threshold", type=float, default=0.3)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--crop", type=int, default=96)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--wpos", type=float, default=20.0)
+    ap.add_argument("--wpos", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=100)
     args = ap.parse_args(argv)
 
@@ -84,11 +94,12 @@ def main(argv=None):
     npk = []
     for i in range(args.tomos):
         snr = float(rng.uniform(args.snr_lo, args.snr_hi))
-        vz, tp, tb, peaks, pos = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
-                                                args.particles, snr, args.seed + i, args.label_threshold, bins)
-        tr.add(vz, tp, tb)
+        vz, tp, tb, peaks, pos, aux = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
+                                                     args.particles, snr, args.seed + i, args.label_threshold, bins)
+        tr.add(vz, tp, tb, aux)
         npk.append(len(peaks))
-        log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold} "
+        log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold}, "
+            f"{surrogate.recall(pos, peaks[:, :3]):.3f} of the {len(pos)} planted particles among them "
             f"({time.perf_counter() - t0:.1f} s)")
         torch.cuda.empty_cache()
     t_label = time.perf_counter() - t0
