@@ -281,8 +281,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="launch groups to time (0 = the full angle set)")
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--batch", default="4", help="orientations per launch group, or 'auto' (autotuner)")
-    ap.add_argument("--tiles", default="16,16", help="column tiles of the z- and y-FFT passes (8/16/32)")
+    ap.add_argument("--batch", default="16", help="orientations per launch group, or 'auto' (autotuner)")
+    ap.add_argument("--tiles", default="16,32", help="column tiles of the z- and y-FFT passes (8/16/32)")
     ap.add_argument("--zchunk", type=int, default=0, help="z-chunk planes of the y-FFT and c2r passes (0 = whole)")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--cpu-particles", type=int, default=6, help="c3: particles in the oracle sample")
@@ -317,7 +317,7 @@ def main():
     N = X * Y * Z
     angles = angle_grid(cfg["step"])
     A = angles.shape[0]
-    KA = 4 if args.batch == "auto" else int(args.batch)
+    KA = 16 if args.batch == "auto" else int(args.batch)
     tiles = tuple(int(v) for v in args.tiles.split(","))
     zchunk = args.zchunk
     groups = (A + KA - 1) // KA

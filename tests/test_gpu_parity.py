@@ -318,3 +318,34 @@ def test_psr_matches_oracle(api, oracle):
         err = np.abs(got - ref).max()
         say(f"  n={n}: max |psr - fp64| = {err:.2e}")
         assert err < 1e-5
+
+
+def test_batch_position_independent_with_weight(api, oracle):
+    """With a box weight, an orientation scores the same at every position of
+    a launch group (a duplicate never beats its first occurrence), and batch
+    sizes give bit-identical maps (the weighting pass handles each template
+    in whole blocks)."""
+    say("batch position with weight")
+    rng = np.random.default_rng(33)
+    V = rng.standard_normal((64, 48, 32)).astype(np.float32)
+    T = _phantom(rng, 24, 5, 1.5, 7)
+    M = oracle.sphere_mask_gauss(24, 10, 2)
+    W = (oracle.wedge((24, 24, 24), 30) * oracle.bandpass_weight((24, 24, 24), 1, 10)).astype(np.float32)
+    tn, mu, sg = api.prepare_template(T, M, W)
+    A, B = [30, 60, 90], [100, 20, 45]
+    dup = np.asarray([A, A, A, A, B, A, A], np.float32)
+    with api.SearchPlan(V, T.shape, M, batch=4) as p:
+        p.set_template(tn, W, mu, sg)
+        p.run(dup, 0)
+        _, an = p.finalize()
+    assert set(np.unique(an).tolist()) <= {0, 4}
+    ang = oracle.angle_grid(60, 60, 60)[::3]
+    outs = []
+    for batch in (1, 3, 4):
+        with api.SearchPlan(V, T.shape, M, batch=batch) as p:
+            p.set_template(tn, W, mu, sg)
+            p.run(ang, 0)
+            outs.append(p.finalize())
+    for sc, an in outs[1:]:
+        np.testing.assert_array_equal(sc, outs[0][0])
+        np.testing.assert_array_equal(an, outs[0][1])

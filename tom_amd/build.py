@@ -27,7 +27,12 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"),
 ]
 # per-file extra flags: the rotation must round exactly like the reference C
-EXTRA = {"rotate.hip": ["-ffp-contract=off"]}
+# No automatic multiply-add contraction where results must not depend on how
+# work is grouped (FFT butterflies; the float-exact rotation); the hot loops
+# spell their fused multiply-adds out.
+_NOCONTRACT = ["-ffp-contract=off"]
+EXTRA = {"rotate.hip": _NOCONTRACT, "search.hip": _NOCONTRACT, "subtomo.hip": _NOCONTRACT,
+         "util.hip": _NOCONTRACT, "picks.hip": _NOCONTRACT}
 SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "unet.hip", "api.hip", "em_io.cpp"]
 
 

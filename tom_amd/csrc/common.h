@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -27,14 +28,19 @@ template <typename T>
 __host__ __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return mk(a.x + b.x, a.y + b.y); }
 template <typename T>
 __host__ __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+// Complex products with explicit fused multiply-adds: the FFT sources are
+// compiled without automatic contraction, so every copy of a butterfly rounds
+// the same way wherever a line sits in a block.
+__host__ __device__ __forceinline__ float fma_(float a, float b, float c) { return fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double fma_(double a, double b, double c) { return fma(a, b, c); }
 template <typename T>
 __host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
-  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return mk(fma_(a.x, b.x, -(a.y * b.y)), fma_(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 template <typename T>
 __host__ __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
-  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return mk(fma_(a.x, b.x, a.y * b.y), fma_(a.y, b.x, -(a.x * b.y)));
 }
 template <typename T>
 __host__ __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return mk(a.x, -a.y); }

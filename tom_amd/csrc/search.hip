@@ -145,14 +145,25 @@ __global__ void k_square(const float* a, float* b, long n) {
 
 // ---------------------------------------------------------------- box filter
 // real lines of L -> complex, forward FFT along x
+// Blocks never straddle two templates: every template's line j sits at the
+// same place in its block whatever the template's position in the batch, so a
+// template's filtered result does not depend on how the batch is grouped.
+__device__ __forceinline__ void box_block_lines(long lines_per_vol, int NL, long& l0, long& nl) {
+  const long nbt = (lines_per_vol + NL - 1) / NL;
+  const long a = blockIdx.x / nbt, b = blockIdx.x % nbt;
+  l0 = a * lines_per_vol + b * NL;
+  nl = (lines_per_vol - b * NL < NL) ? (lines_per_vol - b * NL) : NL;
+}
+
 template <int L>
-__global__ __launch_bounds__(NT) void k_box_x_fwd(const float* t, c32* out, long nlines, const c32* twg) {
+__global__ __launch_bounds__(NT) void k_box_x_fwd(const float* t, c32* out, long lines_per_vol, const c32* twg) {
   constexpr int NL = contig_lines<float, L>();
   __shared__ c32 buf[NL * L];
   __shared__ c32 tw[L];
   load_table<float, L, NT>(tw, twg);
-  const long l0 = (long)blockIdx.x * NL;
-  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * L);
+  long l0, nl;
+  box_block_lines(lines_per_vol, NL, l0, nl);
+  const int nval = (int)(nl * L);
   for (int i = threadIdx.x; i < NL * L; i += NT) buf[i] = mk((i < nval) ? t[l0 * L + i] : 0.f, 0.f);
   __syncthreads();
   lds_fft<float, L, NL, NT, -1, Contig<L>>(buf, tw);
@@ -161,14 +172,15 @@ __global__ __launch_bounds__(NT) void k_box_x_fwd(const float* t, c32* out, long
 
 // inverse FFT along x, real part / n3, times mask -> float
 template <int L>
-__global__ __launch_bounds__(NT) void k_box_x_inv(const c32* in, float* t, const float* M, long nlines,
-                                                  long lines_per_vol, float inv_n3, const c32* twg) {
+__global__ __launch_bounds__(NT) void k_box_x_inv(const c32* in, float* t, const float* M, long lines_per_vol,
+                                                  float inv_n3, const c32* twg) {
   constexpr int NL = contig_lines<float, L>();
   __shared__ c32 buf[NL * L];
   __shared__ c32 tw[L];
   load_table<float, L, NT>(tw, twg);
-  const long l0 = (long)blockIdx.x * NL;
-  const int nval = (int)(((nlines - l0 < NL) ? (nlines - l0) : NL) * L);
+  long l0, nl;
+  box_block_lines(lines_per_vol, NL, l0, nl);
+  const int nval = (int)(nl * L);
   for (int i = threadIdx.x; i < NL * L; i += NT) buf[i] = (i < nval) ? in[l0 * L + i] : mk(0.f, 0.f);
   __syncthreads();
   lds_fft<float, L, NL, NT, 1, Contig<L>>(buf, tw);
@@ -213,8 +225,8 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c3
     for (int ux = 0; ux < nx; ++ux) {
       const c32 w = twX[m];
       const float v = row[ux];
-      re += v * w.x;
-      im -= v * w.y;
+      re = fmaf(v, w.x, re);
+      im = fmaf(-v, w.y, im);
       m += kx;
       if (m >= X) m -= X;
     }
@@ -580,9 +592,9 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
           const float w2 = rows[r][ux + 2];
 #pragma unroll
           for (int k = 0; k < KS; ++k) {
-            acc[k][3 * r] += tv[k] * w0[r];
-            acc[k][3 * r + 1] += tv[k] * w1[r];
-            acc[k][3 * r + 2] += tv[k] * w2;
+            acc[k][3 * r] = fmaf(tv[k], w0[r], acc[k][3 * r]);
+            acc[k][3 * r + 1] = fmaf(tv[k], w1[r], acc[k][3 * r + 1]);
+            acc[k][3 * r + 2] = fmaf(tv[k], w2, acc[k][3 * r + 2]);
           }
           w0[r] = w1[r];
           w1[r] = w2;
@@ -981,12 +993,12 @@ void Search::box_filter(int count, float* t, c32* work) {
   // t <- M .* ifftn(fftn(t) .* W) per template (tom_os3_findTemplate.m:264-267
   // followed by the masking inside tom_os3_corr)
   const long n3 = (long)nx * ny * nz;
-  const long nl = (long)count * ny * nz;
+  const long lpv = (long)ny * nz;  // x lines per template
   dispatch_len(nx, [&](auto Lc) {
     constexpr int L = decltype(Lc)::value;
     constexpr int NL = contig_lines<float, L>();
-    hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const float*)t, work,
-                       nl, tw_.f32(L));
+    hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)(count * ((lpv + NL - 1) / NL))), dim3(NT), 0, s_,
+                       (const float*)t, work, lpv, tw_.f32(L));
   });
   TOMHIP_CHECK(hipGetLastError());
   const long XY = (long)nx * ny;
@@ -1014,8 +1026,8 @@ void Search::box_filter(int count, float* t, c32* work) {
   dispatch_len(nx, [&](auto Lc) {
     constexpr int L = decltype(Lc)::value;
     constexpr int NL = contig_lines<float, L>();
-    hipLaunchKernelGGL(k_box_x_inv<L>, dim3((unsigned)((nl + NL - 1) / NL)), dim3(NT), 0, s_, (const c32*)work, t,
-                       mask_, nl, (long)ny * nz, (float)(1.0 / (double)n3), tw_.f32(L));
+    hipLaunchKernelGGL(k_box_x_inv<L>, dim3((unsigned)(count * ((lpv + NL - 1) / NL))), dim3(NT), 0, s_,
+                       (const c32*)work, t, mask_, lpv, (float)(1.0 / (double)n3), tw_.f32(L));
   });
   TOMHIP_CHECK(hipGetLastError());
 }
