@@ -150,6 +150,13 @@ def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exh
     sg.timings["precompute_s"] = pre_s
     rec_any = S.recall(ref[:, :3], pos)
     rec_hit = S.recall(ref[:, :3], pos[sc > 0.3])
+    from scipy.spatial import cKDTree
+    d_cand = cKDTree(cand[:, :3]).query(ref[:, :3])[0] if len(cand) else np.full(len(ref), np.inf)
+    d_ref = cKDTree(pos).query(ref[:, :3])[0] if len(pos) else np.full(len(ref), np.inf)
+    dist = {"candidate_p50_p90_max": [round(float(np.percentile(d_cand, q)), 2) for q in (50, 90, 100)],
+            "refined_p50_p90_max": [round(float(np.percentile(d_ref, q)), 2) for q in (50, 90, 100)],
+            "recall_refined_within_5": float(np.mean(d_ref <= 5))}
+    log(f"[bench] surrogate distances to the exhaustive peaks: {dist}")
     # exhaustive best score at the matched peaks vs the refined score there
     log(f"[bench] surrogate: {len(cand)} candidates, {t_sur:.3f} s (exhaustive {exhaustive_s:.3f} s); "
         f"recall of {len(ref)} exhaustive peaks > 0.3: {rec_any:.4f} (any candidate), "
@@ -158,7 +165,7 @@ def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exh
     return {"weights": os.path.relpath(wpath, ROOT), "candidates": int(len(cand)), "topk": args.topk,
             "wall_s": round(t_sur, 4), "exhaustive_wall_s": round(exhaustive_s, 4),
             "speedup": round(exhaustive_s / t_sur, 2), "exhaustive_peaks_gt_0.3": int(len(ref)),
-            "recall_any_candidate": rec_any, "recall_refined_gt_0.3": rec_hit,
+            "recall_any_candidate": rec_any, "recall_refined_gt_0.3": rec_hit, "match_distances": dist,
             "timings_s": {k: round(v, 4) for k, v in sg.timings.items()},
             "unet_infer": {"flops": sg.infer_flops, "tflops_per_s": round(sg.infer_flops / sg.timings["infer_s"] / 1e12, 2),
                            "frac_of_bf16_dense_peak": round(sg.infer_flops / sg.timings["infer_s"] / 2.5e15, 4)},

@@ -226,9 +226,11 @@ int tomhip_unet_adam(tomhip_unet* h, float lr);
 int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin);
 /* Whole-volume inference in overlapping tiles of the network's size (halo
  * voxels of context on each side, tile origins on multiples of 4). volume and
- * the optional aux map are each normalised by their global mean and std. */
+ * the optional aux map are each normalised by their global mean and std.
+ * flags bit 0: write the particle logit instead of its sigmoid (no
+ * saturation at 1, so local maxima stay sharp). */
 int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
-                      float* prob, int32_t* bin);
+                      int flags, float* prob, int32_t* bin);
 
 /* Candidate extraction for the learned proposal (no reference counterpart):
  * local maxima of map (x fastest) above threshold within the cube of
@@ -236,6 +238,13 @@ int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, con
  * value, unordered. *count = number found; at most capacity rows written. */
 int tomhip_local_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, float threshold,
                             int64_t capacity, float* out, int64_t* count);
+
+/* Move each candidate (rows of `stride` int32 starting x y z, 0-based, in
+ * place) to the largest value of map within the cube of half-edge radius
+ * around it (clipped to the volume, ties to the lowest index). No reference
+ * counterpart: it recentres proposals on a matched-filter map. */
+int tomhip_snap_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, int32_t* cand,
+                           int64_t P, int stride);
 
 /* ---------------------------------------------------------------- EM files */
 /* EM format header (Oscar3/c++Picker/include/tom/core/io.h:49-63). */

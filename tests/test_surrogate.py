@@ -96,3 +96,25 @@ def test_spherical_average():
         assert np.allclose(A[sel], T[sel].mean(), atol=1e-6)
     # invariant under an exchange of axes (a distance-preserving map about the centre)
     np.testing.assert_allclose(S.spherical_average(T.transpose(1, 2, 0)), A, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_snap_maxima_brute_force(api):
+    print("snap maxima", flush=True)
+    rng = np.random.default_rng(9)
+    a = rng.random((20, 18, 16)).astype(np.float32)
+    a[5, 5, 5] = a[6, 5, 5] = 3.0  # tie: the lower linear index (x fastest) wins
+    cand = np.array([[4, 4, 4], [0, 0, 0], [19, 17, 15], [10, 9, 8]], np.int32)
+    got = api.snap_maxima(a, cand, 2)
+    for c, g in zip(cand, got):
+        lo = np.maximum(c - 2, 0)
+        hi = np.minimum(c + 3, a.shape)
+        sub = a[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        best = None
+        for z in range(sub.shape[2]):
+            for y in range(sub.shape[1]):
+                for x in range(sub.shape[0]):
+                    if best is None or sub[x, y, z] > best[0]:
+                        best = (sub[x, y, z], (x + lo[0], y + lo[1], z + lo[2]))
+        assert tuple(g) == best[1]
+    assert tuple(got[0]) == (5, 5, 5)

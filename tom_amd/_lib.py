@@ -50,6 +50,7 @@ SIGNATURES = {
     "tomhip_psr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "tomhip_search_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
     "tomhip_search_set_zchunk": (ctypes.c_int, [vp, ctypes.c_int]),
+    "tomhip_snap_maxima_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int]),
     "tomhip_unet_create": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, vp]),
     "tomhip_unet_destroy": (None, [vp]),
     "tomhip_unet_num_params": (ctypes.c_int, [vp, vp]),
@@ -61,7 +62,7 @@ SIGNATURES = {
     "tomhip_unet_get_grads": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_adam": (ctypes.c_int, [vp, ctypes.c_float]),
     "tomhip_unet_predict": (ctypes.c_int, [vp, vp, vp]),
-    "tomhip_unet_infer": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp, vp]),
+    "tomhip_unet_infer": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
     "tomhip_em_read_header": (ctypes.c_int, [ctypes.c_char_p, vp]),
     "tomhip_em_read_f32": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int64, vp]),
     "tomhip_em_write_f32": (ctypes.c_int, [ctypes.c_char_p, vp, vp, vp]),

@@ -619,10 +619,10 @@ class Unet:
         self.ctx.check(self.ctx.lib.tomhip_unet_predict(self.h, _ptr(prob), _ptr(b)))
         return prob, b
 
-    def infer(self, vol, halo: int = 24, aux=None):
+    def infer(self, vol, halo: int = 24, aux=None, logit: bool = False):
         """Whole-volume (prob, bin) maps in overlapping tiles of the network
         size; the volume (and the aux map) are normalised by their global mean
-        and std."""
+        and std. logit: return the particle logit instead of its sigmoid."""
         vb = _Buf(vol)
         ab = _Buf(aux) if aux is not None else None
         ctx = _prep_ctx(self.ctx, vb, ab)
@@ -630,7 +630,7 @@ class Unet:
         prob = _empty_like(vb, vb.dims)
         b = _empty_like(vb, vb.dims, np.int32)
         ctx.check(ctx.lib.tomhip_unet_infer(self.h, vb.ptr, None if ab is None else ab.ptr, _p(N), int(halo),
-                                            _ptr(prob), _ptr(b)))
+                                            1 if logit else 0, _ptr(prob), _ptr(b)))
         return prob, b
 
     def close(self):
@@ -681,3 +681,15 @@ def tom_os3_psr(xcf, template_size, ctx: Optional[Context] = None):
     out = _empty_like(xb, xb.dims)
     ctx.check(ctx.lib.tomhip_psr_f32(ctx.h, xb.ptr, _p(N), _p(n), _ptr(out)))
     return out
+
+
+def snap_maxima(peaks, cand_xyz, radius: int, ctx: Optional[Context] = None):
+    """Move each (x, y, z) row to the largest map value within the cube of
+    half-edge radius around it; returns the new (P, 3) int32 rows."""
+    mb = _Buf(peaks)
+    ctx = _prep_ctx(ctx, mb)
+    N = _dims3(mb.dims)
+    c = np.array(np.asarray(cand_xyz, np.int32).reshape(-1, 3), dtype=np.int32, order="C", copy=True)
+    if c.shape[0]:
+        ctx.check(ctx.lib.tomhip_snap_maxima_f32(ctx.h, mb.ptr, _p(N), int(radius), _p(c), c.shape[0], 3))
+    return c

@@ -714,6 +714,32 @@ int tomhip_local_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3
   });
 }
 
+int tomhip_snap_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, int32_t* cand,
+                           int64_t P, int stride) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && cand, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "map");
+    TOMHIP_REQUIRE(radius >= 0 && radius <= 32, ERR_WRONG_INPUT, "radius must be in [0, 32]");
+    TOMHIP_REQUIRE(P >= 0 && P < (1L << 30) && stride >= 3, ERR_WRONG_INPUT, "bad candidate table");
+    if (P == 0) return;
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf m(s, map, sizeof(float) * NN, true, false);
+    DevBuf c(s, cand, sizeof(int32_t) * stride * P, true, true);
+    std::vector<int32_t> hc((size_t)stride * P);
+    TOMHIP_CHECK(hipMemcpyAsync(hc.data(), c.get<int32_t>(), sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, s));
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < P; ++i) {
+      const int32_t* q = hc.data() + (size_t)stride * i;
+      TOMHIP_REQUIRE(q[0] >= 0 && q[0] < N[0] && q[1] >= 0 && q[1] < N[1] && q[2] >= 0 && q[2] < N[2],
+                     ERR_WRONG_INPUT, "candidate outside the volume");
+    }
+    snap_maxima(m.get<float>(), (int)N[0], (int)N[1], (int)N[2], radius, c.get<int32_t>(), P, stride, s);
+    c.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
                                const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
                                const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
@@ -934,7 +960,7 @@ int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin) {
 }
 
 int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
-                      float* prob, int32_t* bin) {
+                      int flags, float* prob, int32_t* bin) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && volume && (prob || bin), ERR_WRONG_INPUT, "null pointer");
     check_dims3(N, "volume");
@@ -945,7 +971,7 @@ int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, con
     DevBuf p(s, prob, sizeof(float) * NN, false, true);
     DevBuf b(s, bin, sizeof(int32_t) * NN, false, true);
     h->u->infer(v.get<float>(), aux ? a.get<float>() : nullptr, (int)N[0], (int)N[1], (int)N[2], halo,
-                prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr);
+                prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr, (flags & 1) != 0);
     p.commit();
     b.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));

@@ -14,4 +14,7 @@ long picklist(const float* map, int X, int Y, int Z, const int32_t* angles, int 
 // no particular order. Returns the number found (may exceed cap; only cap
 // rows are written).
 long local_maxima(const float* map, int X, int Y, int Z, int r, float thr, long cap, float* out, hipStream_t s);
+// Move each candidate (device rows of `stride` ints starting x y z) to the
+// largest map value within the cube of half-edge r around it.
+void snap_maxima(const float* map, int X, int Y, int Z, int r, int* cand, long P, int stride, hipStream_t s);
 }  // namespace tomhip

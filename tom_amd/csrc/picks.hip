@@ -305,4 +305,58 @@ long local_maxima(const float* map, int X, int Y, int Z, int r, float thr, long 
   return (long)total;
 }
 
+namespace {
+
+// One block per candidate: move (x, y, z) to the largest map value within the
+// cube of half-edge r around it (clipped to the volume; ties to the lowest
+// linear index).
+__global__ void k_snap(const float* __restrict__ a, int X, int Y, int Z, int r, int* cand, int stride) {
+  __shared__ float bv[NTP];
+  __shared__ long bi[NTP];
+  int* c = cand + (long)blockIdx.x * stride;
+  const int cx = c[0], cy = c[1], cz = c[2];
+  const int e = 2 * r + 1;
+  const long tot = (long)e * e * e;
+  float best = -FLT_MAX;
+  long bidx = -1;
+  for (long t = threadIdx.x; t < tot; t += NTP) {
+    const int x = cx + (int)(t % e) - r, y = cy + (int)((t / e) % e) - r, z = cz + (int)(t / ((long)e * e)) - r;
+    if (x < 0 || x >= X || y < 0 || y >= Y || z < 0 || z >= Z) continue;
+    const long i = ((long)z * Y + y) * X + x;
+    const float v = a[i];
+    if (bidx < 0 || v > best || (v == best && i < bidx)) {
+      best = v;
+      bidx = i;
+    }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = bidx;
+  __syncthreads();
+  for (int k = NTP / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      const float v2 = bv[threadIdx.x + k];
+      const long i2 = bi[threadIdx.x + k];
+      if (i2 >= 0 && (bi[threadIdx.x] < 0 || v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x]))) {
+        bv[threadIdx.x] = v2;
+        bi[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && bi[0] >= 0) {
+    const long i = bi[0];
+    c[0] = (int)(i % X);
+    c[1] = (int)((i / X) % Y);
+    c[2] = (int)(i / ((long)X * Y));
+  }
+}
+
+}  // namespace
+
+void snap_maxima(const float* map, int X, int Y, int Z, int r, int* cand, long P, int stride, hipStream_t s) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_snap, dim3((unsigned)P), dim3(NTP), 0, s, map, X, Y, Z, r, cand, stride);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace tomhip

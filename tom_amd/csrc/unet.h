@@ -37,12 +37,14 @@ class Unet {
   // {mean BCE, mean CE}.
   void backward(const float* tprob, const int32_t* tbin, float wpos, long npos, double* loss_out);
   void adam(float lr);
-  void predict(float* prob, int32_t* bin);
+  // prob = sigmoid of the particle logit (the logit itself if logit)
+  void predict(float* prob, int32_t* bin, bool logit = false);
   // whole volume vol (device f32 X x Y x Z) and optional aux map of the same
   // size, each normalised here by its global mean and standard deviation, in
   // tiles of the network size with halo voxels of context on each side;
   // prob / bin (device) receive every voxel
-  void infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin);
+  void infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin,
+             bool logit = false);
   const float* logits() const;  // device f32 [V][80]
   int D() const;
   int H() const;

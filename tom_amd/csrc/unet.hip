@@ -480,10 +480,11 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, long n, flo
   }
 }
 
-__global__ void k_predict(const float* logits, int CH, int nbin, long V, float* prob, int32_t* bin) {
+// prob = sigmoid(particle logit), or the logit itself (no saturation at 1)
+__global__ void k_predict(const float* logits, int CH, int nbin, long V, float* prob, int32_t* bin, int raw) {
   for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
     const float* L = logits + v * CH;
-    if (prob) prob[v] = 1.f / (1.f + expf(-L[0]));
+    if (prob) prob[v] = raw ? L[0] : 1.f / (1.f + expf(-L[0]));
     if (bin) {
       int b = 0;
       for (int c = 1; c < nbin; ++c)
@@ -833,10 +834,10 @@ double Unet::forward_flops() const {
   return f;
 }
 
-void Unet::predict(float* prob, int32_t* bin) {
+void Unet::predict(float* prob, int32_t* bin, bool logit) {
   Impl& I = *d_;
   hipLaunchKernelGGL(k_predict, dim3(gridv(I.V0)), dim3(NTU), 0, I.s, (const float*)I.logits, kHeadCh, I.nbin,
-                     I.V0, prob, bin);
+                     I.V0, prob, bin, logit ? 1 : 0);
   TOMHIP_CHECK(hipGetLastError());
 }
 
@@ -920,7 +921,8 @@ void Unet::adam(float lr) {
   refresh_weights();
 }
 
-void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin) {
+void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin,
+                 bool logit) {
   Impl& I = *d_;
   hipStream_t s = I.s;
   TOMHIP_REQUIRE(halo >= 0 && halo % 4 == 0, ERR_WRONG_INPUT, "halo must be a non-negative multiple of 4");
@@ -947,7 +949,7 @@ void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int ha
           hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, aux, X, Y, Z, ox, oy, oz, t,
                              (const double*)(I.sums + 2), n, I.tile_aux);
         forward(I.tile_in, aux ? I.tile_aux : nullptr);
-        predict(I.tile_prob, I.tile_bin);
+        predict(I.tile_prob, I.tile_bin, logit);
         hipLaunchKernelGGL(k_tile_out, dim3(gridv((long)cx * cy * cz)), dim3(NTU), 0, s, (const float*)I.tile_prob,
                            (const int32_t*)I.tile_bin, t, halo, cx, cy, cz, ox, oy, oz, X, Y, Z, prob, bin);
       }

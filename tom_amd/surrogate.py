@@ -195,12 +195,14 @@ class Surrogate:
     voxels are inferred in one tile (no seams); larger ones in cubic tiles of
     edge `tile` with `halo` voxels of context."""
 
-    def __init__(self, params, bins, tile=304, halo=24, half=15.0, step=5.0, nbin=64, whole_max=160_000_000):
+    def __init__(self, params, bins, tile=304, halo=24, half=15.0, step=5.0, nbin=64, whole_max=160_000_000,
+                 snap_radius=4):
         self.params = np.asarray(params, np.float32)
         self.nbin = nbin
         self.bins = np.asarray(bins, np.float64)
         self.tile, self.halo, self.whole_max = tile, halo, whole_max
         self.half, self.step = half, step
+        self.snap_radius = snap_radius
         self.timings = {}
         self._net, self._net_dims = None, None
 
@@ -219,18 +221,27 @@ class Surrogate:
         return self._net, halo
 
     def propose(self, volume, K=2000, nms_radius=12.0, threshold=0.05, aux=None):
-        """Top-K candidates (K, 5): x y z bin prob. aux: the aux_map of the
-        volume (when the network was trained with one)."""
+        """Top-K candidates (K, 5): x y z bin logit, taken from local maxima of
+        the particle logit above logit(threshold) (the logit does not saturate,
+        so a maximum sits at its peak, not on a plateau of probabilities that
+        round to 1). aux: the aux_map of the volume."""
         dims = tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume)
         net, halo = self._network(dims)
         t0 = time.perf_counter()
-        prob, b = net.infer(volume, halo, aux=aux)
+        prob, b = net.infer(volume, halo, aux=aux, logit=True)
         _sync(prob)
         t1 = time.perf_counter()
-        lm = api.local_maxima(prob, 2, threshold)
+        lm = api.local_maxima(prob, 2, float(np.log(threshold / (1 - threshold))))
         t2 = time.perf_counter()
         keep = greedy_nms(lm[:, :3], lm[:, 3], nms_radius, K) if len(lm) else np.zeros(0, np.int64)
         lm = lm[keep]
+        if aux is not None and self.snap_radius > 0 and len(lm):
+            # recentre on the matched-filter map: its maximum sits at the
+            # particle centre, where the network's may drift by a few voxels
+            xyz = api.snap_maxima(aux, lm[:, :3].astype(np.int32), self.snap_radius)
+            lm = np.concatenate([xyz.astype(np.float32), lm[:, 3:]], 1)
+            _, first = np.unique(xyz, axis=0, return_index=True)
+            lm = lm[np.sort(first)]
         bh = b.cpu().numpy() if hasattr(b, "cpu") else np.asarray(b)
         xyz = lm[:, :3].astype(np.int64)
         if hasattr(b, "cpu"):  # torch (z, y, x)

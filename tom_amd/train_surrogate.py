@@ -7,12 +7,13 @@ the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
 missing wedge of a +-60 degree tilt series), each searched exhaustively on the
 GPU at 15 degrees; the picklist of the score map (tom_os3_returnPicklist
 semantics) above --label-threshold gives the particle targets and the best
-orientation there the bin targets (a Gaussian heat map of sigma 3 voxels around
-each peak; bins within 4 voxels), on top of dense targets from the exhaustive
-maps at every voxel (the best score mapped linearly from [0.1, 0.5] to [0, 1],
-and the bin of the best orientation wherever the score exceeds 0.25). The
-network's second input channel is the FLCF map of the rotationally averaged
-template (surrogate.aux_map). The template, mask and wedge weight are
+orientation there the bin targets (a Gaussian heat map of sigma --sigma voxels
+around each peak; bins within 4 voxels); --dense adds targets from the
+exhaustive maps at every voxel (the best score mapped linearly from [0.1, 0.5]
+to [0, 1], and the bin of the best orientation wherever the score exceeds
+0.25), which trains a broader response that localises worse. The network's
+second input channel is the FLCF map of the rotationally averaged template
+(surrogate.aux_map). The template, mask and wedge weight are
 those of the 512x512x256 / 48^3 benchmark workload (seed 2), so the trained
 network proposes candidates for that search.
 """
@@ -44,7 +45,8 @@ def workload_template():
     return T, M, W, angles
 
 
-def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, seed, thr, bins):
+def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, seed, thr, bins, sigma=2.0,
+                   dense=False):
     from . import surrogate, synth
     rng = np.random.default_rng(seed)
     V, pos, eul, _ = synth.make_tomogram((size, size, size), [T], P, 40, snr, rng, seed, api, torch, dev, tilt_max=60.0)
@@ -55,11 +57,12 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
         aux = surrogate.aux_map(plan, T, M, W).cpu().numpy()
     peaks = surrogate.exhaustive_peaks(score, ang, T.shape[0], thr)
     vz = V.cpu().numpy()
-    tp_peak, tb_peak = surrogate.label_volumes(vz.shape, peaks, angles, bins)
-    tp_dense, tb_dense = surrogate.dense_targets(score.cpu().numpy(), ang.cpu().numpy(), angles, bins)
-    # dense FLCF targets everywhere, the peak heat map on top at the picks
-    tprob = np.maximum(tp_dense, tp_peak)
-    tbin = np.where(tb_peak >= 0, tb_peak, tb_dense)
+    tprob, tbin = surrogate.label_volumes(vz.shape, peaks, angles, bins, sigma=sigma)
+    if dense:
+        # dense FLCF targets everywhere, the peak heat map on top at the picks
+        tp_dense, tb_dense = surrogate.dense_targets(score.cpu().numpy(), ang.cpu().numpy(), angles, bins)
+        tprob = np.maximum(tp_dense, tprob)
+        tbin = np.where(tbin >= 0, tbin, tb_dense)
     return vz, tprob, tbin, peaks, pos, aux
 
 
@@ -72,11 +75,13 @@ def main(argv=None):
     ap.add_argument("--snr-lo", type=float, default=0.05)
     ap.add_argument("--snr-hi", type=float, default=0.3)
     ap.add_argument("--label-threshold", type=float, default=0.3)
-    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--crop", type=int, default=96)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--wpos", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=100)
+    ap.add_argument("--sigma", type=float, default=2.0, help="heat-map sigma around the exhaustive peaks (voxels)")
+    ap.add_argument("--dense", action="store_true", help="add dense targets from the exhaustive score map")
     args = ap.parse_args(argv)
 
     import torch
@@ -95,7 +100,8 @@ def main(argv=None):
     for i in range(args.tomos):
         snr = float(rng.uniform(args.snr_lo, args.snr_hi))
         vz, tp, tb, peaks, pos, aux = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
-                                                     args.particles, snr, args.seed + i, args.label_threshold, bins)
+                                                     args.particles, snr, args.seed + i, args.label_threshold, bins,
+                                                     sigma=args.sigma, dense=args.dense)
         tr.add(vz, tp, tb, aux)
         npk.append(len(peaks))
         log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold}, "
