@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch group measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (fetch scaled x2.000, write x1.000 by a 1 GiB device copy in the
 # same runs); profiles/flcf_c2_hbm_traffic.md has the commands and counts.
-MEASURED_TRAFFIC = {("c2", 4): {"passA": 1.5514e9, "passB": 2.1476e9, "passC": 1.6414e9}}
+MEASURED_TRAFFIC = {("c2", 4): {"passA": 1.5514e9, "passB": 2.1476e9, "passC": 1.6414e9},
+                    ("c2", 16): {"passA": 5.3961e9, "passB": 8.5903e9, "passC": 4.9830e9}}
 
 CONFIGS = {
     # name: volume (X,Y,Z), template edge, blobs, sigma, radius, particles,
@@ -128,7 +129,9 @@ def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exh
     recall of the exhaustive search's picklist peaks above 0.3."""
     from tom_amd import surrogate as S
     w = np.load(wpath, allow_pickle=False)
-    sg = S.Surrogate(w["params"], w["bins"])
+    # networks trained on density-centre targets carry the template centroid
+    cen = np.asarray(w["centroid"], np.float64) if "centroid" in w.files else None
+    sg = S.Surrogate(w["params"], w["bins"], centroid=cen)
     ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
     T, M, W, tn, mu_t, sig_t = plan_template
     # warm (network allocation, code objects)
@@ -148,11 +151,13 @@ def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exh
     t_sur = time.perf_counter() - t0 + pre_s
     sg.timings["aux_s"] = t_aux
     sg.timings["precompute_s"] = pre_s
-    rec_any = S.recall(ref[:, :3], pos)
     rec_hit = S.recall(ref[:, :3], pos[sc > 0.3])
     from scipy.spatial import cKDTree
-    d_cand = cKDTree(cand[:, :3]).query(ref[:, :3])[0] if len(cand) else np.full(len(ref), np.inf)
+    # candidates locate density centres when the network was trained on them
+    ref_c = S.centroid_positions(ref, angles, cen) if cen is not None else ref[:, :3]
+    d_cand = cKDTree(cand[:, :3]).query(ref_c)[0] if len(cand) else np.full(len(ref), np.inf)
     d_ref = cKDTree(pos).query(ref[:, :3])[0] if len(pos) else np.full(len(ref), np.inf)
+    rec_any = float(np.mean(d_cand <= 3.0))
     dist = {"candidate_p50_p90_max": [round(float(np.percentile(d_cand, q)), 2) for q in (50, 90, 100)],
             "refined_p50_p90_max": [round(float(np.percentile(d_ref, q)), 2) for q in (50, 90, 100)],
             "recall_refined_within_5": float(np.mean(d_ref <= 5))}

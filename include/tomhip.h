@@ -180,9 +180,14 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
  * orientation a of euler[a][g][0..2] (A per group, degrees) is scored at
  * cand[p][0..2] (0-based voxel) and its 26 neighbours. Outputs per candidate:
  * best score, best a (first wins, strict '>'), best voxel as x y z.
- * Needs an FLCF plan with its template set. volume: the searched volume. */
+ * Needs an FLCF plan with its template set. volume: the searched volume.
+ * centroid (optional, 3 floats x y z): the candidates locate the rotated
+ * template point p + R d (d in voxels from the rotation centre, e.g. the
+ * template's centre of mass) instead of the centre p; orientations are then
+ * scored in chunks of 4 around cand - round(mean R d of the chunk). */
 int tomhip_search_refine(tomhip_search* h, const float* volume, const float* euler, int G, int A,
-                         const int32_t* cand, int64_t P, float* score, int32_t* angle, int32_t* pos);
+                         const int32_t* cand, int64_t P, const float* centroid, float* score, int32_t* angle,
+                         int32_t* pos);
 
 /* ---------------------------------------------------------------- secondary scores */
 /* Peak-to-sidelobe ratio map of a correlation map (os3_psr,
@@ -213,24 +218,29 @@ int tomhip_unet_get_params(tomhip_unet* h, float* out);
 int tomhip_unet_set_params(tomhip_unet* h, const float* in);
 /* volume and the optional aux map (NULL = zeros) fill input channels 0 and 1 */
 int tomhip_unet_forward(tomhip_unet* h, const float* volume, const float* aux);
-/* logits [V][80]: channel 0 particle, 1..nbin orientation bins */
+/* logits [V][80]: channel 0 particle, 1..nbin orientation bins, nbin+1..nbin+3
+ * the offset (dx, dy, dz) / 8 from the voxel to the particle centre */
 int tomhip_unet_get_logits(tomhip_unet* h, float* out);
 /* loss: weighted BCE (positive weight wpos) of channel 0 against tprob[V],
  * averaged over voxels, plus softmax cross-entropy of the bins against
- * tbin[V] (-1 = no label) averaged over labelled voxels. Computes all
- * parameter gradients; loss (optional) = {bce, ce}. */
-int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, float wpos, double* loss);
+ * tbin[V] (-1 = no label) averaged over labelled voxels, plus (toff optional,
+ * [V][4] = dx/8, dy/8, dz/8, weight) the squared error of the offset channels
+ * averaged over voxels of weight > 0. Computes all parameter gradients; loss
+ * (optional) = {bce, ce, offset}. */
+int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, const float* toff, float wpos,
+                         double* loss);
 int tomhip_unet_get_grads(tomhip_unet* h, float* out);
 int tomhip_unet_adam(tomhip_unet* h, float lr);
-/* prob[V] = sigmoid(logit 0), bin[V] = argmax of the bin logits (0-based) */
-int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin);
+/* prob[V] = sigmoid(logit 0), bin[V] = argmax of the bin logits (0-based),
+ * offsets[V][3] = the offset channels (each output optional) */
+int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin, float* offsets);
 /* Whole-volume inference in overlapping tiles of the network's size (halo
  * voxels of context on each side, tile origins on multiples of 4). volume and
  * the optional aux map are each normalised by their global mean and std.
  * flags bit 0: write the particle logit instead of its sigmoid (no
- * saturation at 1, so local maxima stay sharp). */
+ * saturation at 1, so local maxima stay sharp). offsets (optional) [V][3]. */
 int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
-                      int flags, float* prob, int32_t* bin);
+                      int flags, float* prob, int32_t* bin, float* offsets);
 
 /* Candidate extraction for the learned proposal (no reference counterpart):
  * local maxima of map (x fastest) above threshold within the cube of

@@ -101,9 +101,12 @@ def forward(vol, theta, bf16=True, params=None, aux=None):
     return logits[0].permute(1, 2, 3, 0), P
 
 
-def loss(logits, tprob, tbin, nbin, wpos):
-    """Weighted BCE on channel 0 (mean over voxels) and softmax cross-entropy
-    over channels 1..nbin at voxels with tbin >= 0 (mean over those)."""
+def loss(logits, tprob, tbin, nbin, wpos, toff=None):
+    """Weighted BCE on channel 0 (mean over voxels), softmax cross-entropy
+    over channels 1..nbin at voxels with tbin >= 0 (mean over those) and, with
+    toff [..., 4] = (dx/8, dy/8, dz/8, weight), the squared error of channels
+    nbin+1..nbin+3 against toff[..., :3] summed over the three and averaged
+    over voxels of weight > 0. Returns (bce, ce, off)."""
     z = logits[..., 0].reshape(-1)
     t = torch.as_tensor(np.asarray(tprob), dtype=z.dtype).reshape(-1)
     bce = F.binary_cross_entropy_with_logits(z, t, pos_weight=torch.tensor(wpos, dtype=z.dtype), reduction="mean")
@@ -114,24 +117,31 @@ def loss(logits, tprob, tbin, nbin, wpos):
         ce = F.cross_entropy(L, b[sel], reduction="mean")
     else:
         ce = torch.zeros((), dtype=z.dtype)
-    return bce, ce
+    off = torch.zeros((), dtype=z.dtype)
+    if toff is not None:
+        to = torch.as_tensor(np.asarray(toff), dtype=z.dtype).reshape(-1, 4)
+        w = to[:, 3] > 0
+        if int(w.sum()) > 0:
+            O = logits.reshape(-1, HEAD)[w][:, 1 + nbin:4 + nbin]
+            off = ((O - to[w][:, :3]) ** 2).sum(1).mean()
+    return bce, ce, off
 
 
-def gradients(vol, theta, tprob, tbin, nbin, wpos, bf16=True, aux=None):
-    """(bce, ce, flat gradient of bce + ce) in FP64."""
+def gradients(vol, theta, tprob, tbin, nbin, wpos, bf16=True, aux=None, toff=None):
+    """(bce, ce, off, flat gradient of bce + ce + off) in FP64."""
     P = unpack(theta)
     for w, b in P:
         w.requires_grad_(True)
         b.requires_grad_(True)
     logits, _ = forward(vol, theta, bf16=bf16, params=P, aux=aux)
-    bce, ce = loss(logits, tprob, tbin, nbin, wpos)
-    (bce + ce).backward()
+    bce, ce, off = loss(logits, tprob, tbin, nbin, wpos, toff)
+    (bce + ce + off).backward()
     g = []
     for (w, b), (ci, co, t, _) in zip(P, SPEC):
         # torch [co][ci][kz][ky][kx] -> W[t][co][ci]
         g.append(w.grad.reshape(co, ci, t).permute(2, 0, 1).reshape(-1))
         g.append(b.grad.reshape(-1))
-    return float(bce), float(ce), torch.cat(g).detach().numpy()
+    return float(bce), float(ce), float(off), torch.cat(g).detach().numpy()
 
 
 def conv_direct(x, W, b, relu):

@@ -289,6 +289,52 @@ def test_refine_matches_exhaustive_maps(api, oracle):
             assert an[i] == ref[1] and tuple(pos[i]) == ref[2]
 
 
+def test_refine_centroid_windows(api, oracle):
+    """tomhip_search_refine with a centroid d: candidates locate the rotated
+    template point R d + p; orientations a = 4k .. 4k + 3 of group g are
+    scored around cand - round(mean R d) over the 27 neighbours (FP64 oracle
+    maps as in the test above, R of tom_rotate's convention)."""
+    say("refine with centroid")
+    from tom_amd import orient
+    rng = np.random.default_rng(22)
+    N = (64, 48, 32)
+    n = 16
+    V = rng.standard_normal(N).astype(np.float32)
+    T = _phantom(rng, n, 4, 1.5, 5)
+    V[20:36, 10:26, 8:24] += 2 * oracle.rotate(T, [30, 60, 90])
+    M = oracle.sphere_mask_gauss(n, 6, 1.5)
+    angles = np.concatenate([oracle.angle_grid(60, 60, 60)[::7], [[30, 60, 90]]])[:20]
+    G, A = 2, 10
+    ga = angles.reshape(G, A, 3).transpose(1, 0, 2)  # (A, G, 3)
+    _, _, allm = oracle.find_template(V, T, M, angles, return_all=True)
+    d = np.array([2.6, -1.7, 3.2])
+    cand = np.array([[31, 15, 19, 0], [31, 15, 19, 1], [0, 0, 0, 0], [63, 47, 31, 1], [5, 40, 30, 0]], np.int32)
+    tn, mu, sg = api.prepare_template(T, M)
+    with api.SearchPlan(V, T.shape, M, batch=4) as p:
+        p.set_template(tn, None, mu, sg)
+        sc, an, pos = p.refine(V, ga, cand, centroid=d)
+    R = orient.rotmatrix(ga.astype(np.float64))  # (A, G, 3, 3)
+    Rd = R @ d
+    for i, (x, y, z, g) in enumerate(cand):
+        vals = []
+        for a in range(A):
+            k = a // 4
+            sh = np.rint(Rd[4 * k:min(A, 4 * k + 4), g].mean(0)).astype(int)
+            cx, cy, cz = x - sh[0], y - sh[1], z - sh[2]
+            for q in range(27):
+                dx, dy, dz = q % 3 - 1, (q // 3) % 3 - 1, q // 9 - 1
+                px, py, pz = (cx + dx) % N[0], (cy + dy) % N[1], (cz + dz) % N[2]
+                vals.append((allm[g * A + a][px, py, pz], a, (px, py, pz)))
+        best = max(v[0] for v in vals)
+        srt = sorted((v[0] for v in vals), reverse=True)
+        err = abs(sc[i] - best)
+        say(f"  cand {i}: gpu {sc[i]:.6f} fp64 {best:.6f} |d| {err:.2e}")
+        assert err < SCORE_TOL
+        if srt[0] - srt[1] > MARGIN:
+            ref = next(v for v in vals if v[0] == best)
+            assert an[i] == ref[1] and tuple(pos[i]) == ref[2]
+
+
 def test_autotune_picks_a_batch_with_identical_results(api, oracle):
     """The autotuner times candidate batches and checks that every batch gives
     the same maps before it reports a choice."""

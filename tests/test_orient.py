@@ -66,3 +66,21 @@ def test_local_grid():
     assert g[:, :2].min() >= 0 and g[:, :2].max() < 360
     # phi outermost, theta innermost
     assert g[1, 2] - g[0, 2] == 5 and g[0, 0] == g[48, 0]
+
+
+def test_template_centroid_rotates_with_the_template(oracle):
+    """The centre of mass of tom_rotate(T, e) sits at R(e) d from the rotation
+    centre, d = template_centroid(T): the convention the proposal's centroid
+    targets and the refinement's windows rely on."""
+    from tom_amd import surrogate, synth
+    rng = np.random.default_rng(2)
+    T = synth.phantom(rng, 32, 6, 2.0, 9)
+    d = surrogate.template_centroid(T)
+    assert np.linalg.norm(d) > 1.0
+    for e in ([30, 50, 70], [100, 200, 40], [0, 0, 90], [250, 10, 135]):
+        got = surrogate.template_centroid(oracle.rotate(T, e))
+        np.testing.assert_allclose(got, O.rotmatrix(np.array(e, float)) @ d, atol=0.05)
+    pk = np.array([[10, 20, 30, 0.5, 1]], float)
+    ang = np.array([[0, 0, 0], [30, 50, 70]], float)
+    np.testing.assert_allclose(surrogate.centroid_positions(pk, ang, d)[0],
+                               [10, 20, 30] + O.rotmatrix(ang[1]) @ d)

@@ -34,6 +34,31 @@ def test_label_volumes():
     assert (tb >= 0).sum() == 33 + 11
 
 
+def test_label_offsets_brute_force():
+    """Offsets point from each voxel to the nearest peak (brute force over the
+    peaks), in units of OFFSET_SCALE, within offset_radius only."""
+    bins = orient.orientation_bins(64, 0)
+    angles = np.asarray([[0, 0, 0], [10, 20, 30]], np.float32)
+    peaks = np.array([[5, 6, 7, 0.5, 1], [12, 3, 2, 0.4, 0], [1, 13, 9, 0.35, 0]], float)
+    shape = (12, 14, 16)
+    tp0, tb0 = S.label_volumes(shape, peaks, angles, bins, sigma=2.0, bin_radius=2.0)
+    tp, tb, to = S.label_volumes(shape, peaks, angles, bins, sigma=2.0, bin_radius=2.0, offset_radius=5.0)
+    np.testing.assert_array_equal(tp, tp0)
+    np.testing.assert_array_equal(tb, tb0)
+    Z, Y, X = shape
+    c = peaks[:, :3]
+    for z in range(Z):
+        for y in range(Y):
+            for x in range(X):
+                d2 = ((c - [x, y, z]) ** 2).sum(1)
+                j = int(np.argmin(d2))
+                if d2[j] <= 25 and (d2 == d2[j]).sum() == 1:
+                    np.testing.assert_allclose(to[z, y, x, :3], (c[j] - [x, y, z]) / S.OFFSET_SCALE, atol=1e-7)
+                    assert to[z, y, x, 3] == 1
+                elif d2[j] > 25:
+                    assert not to[z, y, x].any()
+
+
 def test_recall():
     ref = np.array([[0, 0, 0], [10, 10, 10], [20, 20, 20]], float)
     got = np.array([[1, 1, 1], [10, 10, 14]], float)

@@ -27,10 +27,10 @@ def _theta(rng, scale=1.0):
         if li == 0:
             W[:, :, 2:] = 0
         if t == 1:
-            W[:, 1 + NBIN:, :] = 0
+            W[:, 4 + NBIN:, :] = 0
         b = rng.standard_normal(co) * 0.05
         if t == 1:
-            b[1 + NBIN:] = 0
+            b[4 + NBIN:] = 0
         out += [W.reshape(-1), b]
     return np.concatenate(out)
 
@@ -39,6 +39,14 @@ def _targets(rng, shape):
     tprob = (rng.random(shape) < 0.05).astype(np.float32)
     tbin = np.where(tprob > 0, rng.integers(0, NBIN, shape), -1).astype(np.int32)
     return tprob, tbin
+
+
+def _offsets(rng, shape):
+    """[..., 4] offset targets, supervised at about a fifth of the voxels."""
+    toff = np.zeros(shape + (4,), np.float32)
+    toff[..., :3] = rng.uniform(-1.5, 1.5, shape + (3,))
+    toff[..., 3] = rng.random(shape) < 0.2
+    return toff
 
 
 # ---------------------------------------------------------------- CPU pins
@@ -63,7 +71,8 @@ def test_losses_closed_form():
     L = torch.as_tensor(rng.standard_normal((3, 4, 5, 80)) * 3)
     tprob, tbin = _targets(rng, (3, 4, 5))
     tprob[0, 0, 0], tbin[0, 0, 0] = 1.0, 5
-    bce, ce = U.loss(L, tprob, tbin, NBIN, 7.0)
+    toff = _offsets(rng, (3, 4, 5))
+    bce, ce, off = U.loss(L, tprob, tbin, NBIN, 7.0, toff)
     z = L[..., 0].numpy().ravel()
     t = tprob.ravel()
     p = 1 / (1 + np.exp(-z))
@@ -73,6 +82,16 @@ def test_losses_closed_form():
     lse = np.log(np.exp(Lb[sel]).sum(1))
     ref_ce = np.mean(lse - Lb[sel][np.arange(sel.sum()), tbin.ravel()[sel]])
     assert abs(float(bce) - ref_bce) < 1e-12 and abs(float(ce) - ref_ce) < 1e-12
+    # offsets: channels 65..67 against the supervised voxels only (loop form)
+    Lf, tf = L.numpy().reshape(-1, 80), toff.reshape(-1, 4)
+    acc, n = 0.0, 0
+    for v in range(Lf.shape[0]):
+        if tf[v, 3] > 0:
+            n += 1
+            for c in range(3):
+                acc += (Lf[v, 65 + c] - tf[v, c]) ** 2
+    assert n > 0 and abs(float(off) - acc / n) < 1e-12
+    assert float(U.loss(L, tprob, tbin, NBIN, 7.0)[2]) == 0.0
 
 
 def test_restatement_gradients_finite_differences():
@@ -83,12 +102,13 @@ def test_restatement_gradients_finite_differences():
     theta = _theta(rng)
     tprob, tbin = _targets(rng, shape)
     tprob[4, 4, 4], tbin[4, 4, 4] = 1.0, 3
-    _, _, g = U.gradients(vol, theta, tprob, tbin, NBIN, 5.0, bf16=False)
+    toff = _offsets(rng, shape)
+    _, _, _, g = U.gradients(vol, theta, tprob, tbin, NBIN, 5.0, bf16=False, toff=toff)
 
     def L(th):
         logits, _ = U.forward(vol, th, bf16=False)
-        b, c = U.loss(logits, tprob, tbin, NBIN, 5.0)
-        return float(b + c)
+        b, c, o = U.loss(logits, tprob, tbin, NBIN, 5.0, toff)
+        return float(b + c + o)
 
     offs = np.cumsum([0] + [t * co * ci + co for ci, co, t, _ in U.SPEC])
     idx = []
@@ -98,6 +118,8 @@ def test_restatement_gradients_finite_differences():
         # a weight of a live input channel and the bias of channel 0
         idx.append(offs[li] + (rng.integers(0, nw) if li else 0))
         idx.append(offs[li] + nw)
+    # the head's bias of the z-offset channel (only the offset term reaches it)
+    idx.append(offs[-1] - 80 + NBIN + 3)
     eps = 1e-6
     for i in idx:
         tp, tm = theta.copy(), theta.copy()
@@ -156,13 +178,15 @@ def test_unet_backward_matches_restatement(api):
     aux = rng.standard_normal(shape).astype(np.float32)
     theta = _theta(rng).astype(np.float32)
     tprob, tbin = _targets(rng, shape)
+    toff = _offsets(rng, shape)
     with _gpu_net(api, shape, theta) as net:
         net.forward(_as_f(vol), _as_f(aux))
-        bce, ce = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
+        bce, ce, off = net.backward(_as_f(tprob), _as_f(tbin), 4.0, toff=toff.transpose(3, 2, 1, 0))
         g = net.get_grads()
-    rb, rc, rg = U.gradients(vol, theta.astype(np.float64), tprob, tbin, NBIN, 4.0, bf16=True, aux=aux)
-    print(f"  loss gpu ({bce:.6f}, {ce:.6f}) ref ({rb:.6f}, {rc:.6f})", flush=True)
-    assert abs(bce - rb) <= 1e-3 * abs(rb) and abs(ce - rc) <= 1e-3 * abs(rc)
+    rb, rc, ro, rg = U.gradients(vol, theta.astype(np.float64), tprob, tbin, NBIN, 4.0, bf16=True, aux=aux,
+                                 toff=toff)
+    print(f"  loss gpu ({bce:.6f}, {ce:.6f}, {off:.6f}) ref ({rb:.6f}, {rc:.6f}, {ro:.6f})", flush=True)
+    assert abs(bce - rb) <= 1e-3 * abs(rb) and abs(ce - rc) <= 1e-3 * abs(rc) and abs(off - ro) <= 1e-3 * abs(ro)
     off = 0
     for li, (ci, co, t, _) in enumerate(U.SPEC):
         n = t * co * ci + co
@@ -188,7 +212,7 @@ def test_unet_adam_descends_and_infer_tiles(api):
         losses = []
         for _ in range(12):
             net.forward(_as_f(vol))
-            b, c = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
+            b, c, _ = net.backward(_as_f(tprob), _as_f(tbin), 4.0)
             losses.append(b + c)
             net.adam(1e-3)
         print(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
@@ -197,7 +221,10 @@ def test_unet_adam_descends_and_infer_tiles(api):
         aux = (rng.random(shape) * 3 + 1).astype(np.float32)
         an = (aux - aux.astype(np.float64).mean()) / aux.astype(np.float64).std()
         net.forward(_as_f(vn.astype(np.float32)), _as_f(an.astype(np.float32)))
-        p0, b0 = net.predict()
-        p1, b1 = net.infer(_as_f(vol), halo=0, aux=_as_f(aux))
+        p0, b0, o0 = net.predict(offsets=True)
+        lg = net.logits()
+        p1, b1, o1 = net.infer(_as_f(vol), halo=0, aux=_as_f(aux), offsets=True)
     np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-6)
     assert (b1 == b0).mean() > 0.999
+    np.testing.assert_array_equal(o0, lg[..., 1 + NBIN:4 + NBIN].transpose(3, 2, 1, 0))
+    np.testing.assert_allclose(o1, o0, rtol=0, atol=1e-5)

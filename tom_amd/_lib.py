@@ -45,7 +45,8 @@ SIGNATURES = {
     "tomhip_os3_corr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "tomhip_oscar_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp]),
     "tomhip_picklist_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp, vp]),
-    "tomhip_search_refine": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64, vp, vp, vp]),
+    "tomhip_search_refine": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64, vp, vp, vp,
+                                             vp]),
     "tomhip_local_maxima_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int64, vp, vp]),
     "tomhip_psr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "tomhip_search_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
@@ -58,11 +59,11 @@ SIGNATURES = {
     "tomhip_unet_set_params": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_forward": (ctypes.c_int, [vp, vp, vp]),
     "tomhip_unet_get_logits": (ctypes.c_int, [vp, vp]),
-    "tomhip_unet_backward": (ctypes.c_int, [vp, vp, vp, ctypes.c_float, vp]),
+    "tomhip_unet_backward": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_float, vp]),
     "tomhip_unet_get_grads": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_adam": (ctypes.c_int, [vp, ctypes.c_float]),
-    "tomhip_unet_predict": (ctypes.c_int, [vp, vp, vp]),
-    "tomhip_unet_infer": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]),
+    "tomhip_unet_predict": (ctypes.c_int, [vp, vp, vp, vp]),
+    "tomhip_unet_infer": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]),
     "tomhip_em_read_header": (ctypes.c_int, [ctypes.c_char_p, vp]),
     "tomhip_em_read_f32": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int64, vp]),
     "tomhip_em_write_f32": (ctypes.c_int, [ctypes.c_char_p, vp, vp, vp]),

@@ -286,11 +286,14 @@ class SearchPlan:
         self.ctx.check(self.ctx.lib.tomhip_search_finalize(self.h, _ptr(sc), _ptr(an)))
         return sc, an
 
-    def refine(self, volume, group_angles, cand):
+    def refine(self, volume, group_angles, cand, centroid=None):
         """Exact FLCF around candidates: group_angles (A, G, 3) [phi psi theta]
         (A orientations for each of G groups), cand (P, 4) int rows
-        x y z group (0-based). Returns (score[P], angle index[P] into the
-        group's list, position[P, 3] x y z 0-based)."""
+        x y z group (0-based). centroid (3,) x y z: the candidates locate the
+        rotated template point R d + p instead of the centre p (chunks of 4
+        orientations are scored around cand - round(mean R d)). Returns
+        (score[P], angle index[P] into the group's list, position[P, 3] of
+        the best centre, x y z 0-based)."""
         vb = _Buf(volume)
         _prep_ctx(self.ctx, vb)
         ga = np.ascontiguousarray(np.asarray(group_angles, np.float32))
@@ -302,8 +305,9 @@ class SearchPlan:
         sc = np.zeros(P, np.float32)
         an = np.zeros(P, np.int32)
         pos = np.zeros((P, 3), np.int32)
-        self.ctx.check(self.ctx.lib.tomhip_search_refine(self.h, vb.ptr, _p(ga), G, A, _p(c), P, _p(sc), _p(an),
-                                                         _p(pos)))
+        cen = None if centroid is None else np.ascontiguousarray(np.asarray(centroid, np.float32).reshape(3))
+        self.ctx.check(self.ctx.lib.tomhip_search_refine(self.h, vb.ptr, _p(ga), G, A, _p(c), P,
+                                                         None if cen is None else _p(cen), _p(sc), _p(an), _p(pos)))
         return sc, an, pos
 
     def set_tiles(self, tile_z: int = 16, tile_y: int = 16):
@@ -590,13 +594,18 @@ class Unet:
         nx, ny, nz = (int(x) for x in self.dims)
         return out.reshape(nz, ny, nx, self.HEAD)
 
-    def backward(self, tprob, tbin, wpos: float = 1.0):
-        """Gradients of BCE(tprob) + CE(tbin); returns (bce, ce)."""
+    def backward(self, tprob, tbin, wpos: float = 1.0, toff=None):
+        """Gradients of BCE(tprob) + CE(tbin) [+ offset squared error against
+        toff, whose memory is [V][4] = (dx/8, dy/8, dz/8, weight): a torch
+        (nz, ny, nx, 4) tensor or a numpy (4, nx, ny, nz) F-order array];
+        returns (bce, ce, offset)."""
         pb, bb = _Buf(tprob), _Buf(tbin, np.int32)
-        _prep_ctx(self.ctx, pb, bb)
-        loss = np.zeros(2)
-        self.ctx.check(self.ctx.lib.tomhip_unet_backward(self.h, pb.ptr, bb.ptr, float(wpos), _p(loss)))
-        return float(loss[0]), float(loss[1])
+        ob = _Buf(toff) if toff is not None else None
+        _prep_ctx(self.ctx, pb, bb, ob)
+        loss = np.zeros(3)
+        self.ctx.check(self.ctx.lib.tomhip_unet_backward(self.h, pb.ptr, bb.ptr, None if ob is None else ob.ptr,
+                                                         float(wpos), _p(loss)))
+        return float(loss[0]), float(loss[1]), float(loss[2])
 
     def get_grads(self) -> np.ndarray:
         out = np.zeros(self.num_params(), np.float32)
@@ -606,8 +615,15 @@ class Unet:
     def adam(self, lr: float):
         self.ctx.check(self.ctx.lib.tomhip_unet_adam(self.h, float(lr)))
 
-    def predict(self, like=None):
-        """(prob, bin) of the last forward pass, shaped like the input."""
+    def _offsets_like(self, buf, dims):
+        if buf is not None and buf.torch:
+            return torch.empty(tuple(reversed(tuple(dims))) + (3,), dtype=torch.float32, device=buf.obj.device)
+        return np.zeros((3,) + tuple(dims), np.float32, order="F")
+
+    def predict(self, like=None, offsets: bool = False):
+        """(prob, bin[, offsets]) of the last forward pass, shaped like the
+        input; offsets: torch (nz, ny, nx, 3) or numpy (3, nx, ny, nz), the
+        predicted (dx, dy, dz) / 8 to the particle centre."""
         ref = _Buf(like) if like is not None else None
         dims = tuple(int(x) for x in self.dims)
         if ref is not None and ref.torch:
@@ -616,22 +632,25 @@ class Unet:
         else:
             prob = np.zeros(dims, np.float32, order="F")
             b = np.zeros(dims, np.int32, order="F")
-        self.ctx.check(self.ctx.lib.tomhip_unet_predict(self.h, _ptr(prob), _ptr(b)))
-        return prob, b
+        o = self._offsets_like(ref, dims) if offsets else None
+        self.ctx.check(self.ctx.lib.tomhip_unet_predict(self.h, _ptr(prob), _ptr(b), None if o is None else _ptr(o)))
+        return (prob, b, o) if offsets else (prob, b)
 
-    def infer(self, vol, halo: int = 24, aux=None, logit: bool = False):
-        """Whole-volume (prob, bin) maps in overlapping tiles of the network
-        size; the volume (and the aux map) are normalised by their global mean
-        and std. logit: return the particle logit instead of its sigmoid."""
+    def infer(self, vol, halo: int = 24, aux=None, logit: bool = False, offsets: bool = False):
+        """Whole-volume (prob, bin[, offsets]) maps in overlapping tiles of the
+        network size; the volume (and the aux map) are normalised by their
+        global mean and std. logit: return the particle logit instead of its
+        sigmoid; offsets as in predict()."""
         vb = _Buf(vol)
         ab = _Buf(aux) if aux is not None else None
         ctx = _prep_ctx(self.ctx, vb, ab)
         N = _dims3(vb.dims)
         prob = _empty_like(vb, vb.dims)
         b = _empty_like(vb, vb.dims, np.int32)
+        o = self._offsets_like(vb, vb.dims) if offsets else None
         ctx.check(ctx.lib.tomhip_unet_infer(self.h, vb.ptr, None if ab is None else ab.ptr, _p(N), int(halo),
-                                            1 if logit else 0, _ptr(prob), _ptr(b)))
-        return prob, b
+                                            1 if logit else 0, _ptr(prob), _ptr(b), None if o is None else _ptr(o)))
+        return (prob, b, o) if offsets else (prob, b)
 
     def close(self):
         if self.h:

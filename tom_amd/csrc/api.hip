@@ -453,7 +453,8 @@ void tomhip_search_destroy(tomhip_search* h) {
 }
 
 int tomhip_search_refine(tomhip_search* h, const float* volume, const float* euler, int G, int A,
-                         const int32_t* cand, int64_t P, float* score, int32_t* angle, int32_t* pos) {
+                         const int32_t* cand, int64_t P, const float* centroid, float* score, int32_t* angle,
+                         int32_t* pos) {
   tomhip_ctx* ctx = h ? h->ctx : nullptr;
   return guard(ctx, [&] {
     TOMHIP_REQUIRE(h && volume && euler && cand && score && angle && pos, ERR_WRONG_INPUT, "null pointer");
@@ -483,7 +484,14 @@ int tomhip_search_refine(tomhip_search* h, const float* volume, const float* eul
     std::vector<float> hs(P);
     std::vector<int32_t> ha(P), hp(P);
     try {
-      S.refine(v.get<float>(), rm.data(), G, A, dc.get<int32_t>(), P, bs, ba, bp);
+      float cen[3] = {0, 0, 0};
+      if (centroid) {
+        TOMHIP_CHECK(hipMemcpyAsync(cen, centroid, sizeof(cen), hipMemcpyDefault, s));
+        TOMHIP_CHECK(hipStreamSynchronize(s));
+        require_finite(cen, 3);
+        for (float q : cen) TOMHIP_REQUIRE(std::fabs(q) <= 64.f, ERR_WRONG_INPUT, "centroid beyond 64 voxels");
+      }
+      S.refine(v.get<float>(), rm.data(), G, A, dc.get<int32_t>(), P, centroid ? cen : nullptr, bs, ba, bp);
       TOMHIP_CHECK(hipMemcpyAsync(hs.data(), bs, sizeof(float) * P, hipMemcpyDeviceToHost, s));
       TOMHIP_CHECK(hipMemcpyAsync(ha.data(), ba, sizeof(int32_t) * P, hipMemcpyDeviceToHost, s));
       TOMHIP_CHECK(hipMemcpyAsync(hp.data(), bp, sizeof(int32_t) * P, hipMemcpyDeviceToHost, s));
@@ -909,23 +917,30 @@ int tomhip_unet_get_logits(tomhip_unet* h, float* out) {
   });
 }
 
-int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, float wpos, double* loss) {
+int tomhip_unet_backward(tomhip_unet* h, const float* tprob, const int32_t* tbin, const float* toff, float wpos,
+                         double* loss) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && tprob && tbin, ERR_WRONG_INPUT, "null pointer");
     TOMHIP_REQUIRE(std::isfinite(wpos) && wpos > 0, ERR_WRONG_INPUT, "wpos must be positive");
     hipStream_t s = h->ctx->stream;
     DevBuf p(s, tprob, sizeof(float) * h->V, true, false);
     DevBuf b(s, tbin, sizeof(int32_t) * h->V, true, false);
+    DevBuf o(s, toff, sizeof(float) * 4 * h->V, true, false);
     std::vector<int32_t> hb(h->V);
     TOMHIP_CHECK(hipMemcpyAsync(hb.data(), b.get<int32_t>(), sizeof(int32_t) * h->V, hipMemcpyDeviceToHost, s));
+    std::vector<float> ho(toff ? 4 * h->V : 0);
+    if (toff)
+      TOMHIP_CHECK(hipMemcpyAsync(ho.data(), o.get<float>(), sizeof(float) * 4 * h->V, hipMemcpyDeviceToHost, s));
     TOMHIP_CHECK(hipStreamSynchronize(s));
-    long npos = 0;
+    long npos = 0, noff = 0;
     for (int32_t v : hb) npos += (v >= 0);
-    double l[2];
-    h->u->backward(p.get<float>(), b.get<int32_t>(), wpos, npos, l);
+    for (long v = 0; v < (long)ho.size() / 4; ++v) noff += (ho[4 * v + 3] > 0.f);
+    double l[3];
+    h->u->backward(p.get<float>(), b.get<int32_t>(), toff ? o.get<float>() : nullptr, wpos, npos, noff, l);
     if (loss) {
       loss[0] = l[0];
       loss[1] = l[1];
+      loss[2] = l[2];
     }
   });
 }
@@ -946,23 +961,26 @@ int tomhip_unet_adam(tomhip_unet* h, float lr) {
   });
 }
 
-int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin) {
+int tomhip_unet_predict(tomhip_unet* h, float* prob, int32_t* bin, float* offsets) {
   return guard(h ? h->ctx : nullptr, [&] {
-    TOMHIP_REQUIRE(h && (prob || bin), ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(h && (prob || bin || offsets), ERR_WRONG_INPUT, "null pointer");
     hipStream_t s = h->ctx->stream;
     DevBuf p(s, prob, sizeof(float) * h->V, false, true);
     DevBuf b(s, bin, sizeof(int32_t) * h->V, false, true);
-    h->u->predict(prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr);
+    DevBuf o(s, offsets, sizeof(float) * 3 * h->V, false, true);
+    h->u->predict(prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr, false,
+                  offsets ? o.get<float>() : nullptr);
     p.commit();
     b.commit();
+    o.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }
 
 int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, const int64_t N[3], int halo,
-                      int flags, float* prob, int32_t* bin) {
+                      int flags, float* prob, int32_t* bin, float* offsets) {
   return guard(h ? h->ctx : nullptr, [&] {
-    TOMHIP_REQUIRE(h && volume && (prob || bin), ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(h && volume && (prob || bin || offsets), ERR_WRONG_INPUT, "null pointer");
     check_dims3(N, "volume");
     const long NN = N[0] * N[1] * N[2];
     hipStream_t s = h->ctx->stream;
@@ -970,10 +988,13 @@ int tomhip_unet_infer(tomhip_unet* h, const float* volume, const float* aux, con
     DevBuf a(s, aux, sizeof(float) * NN, true, false);
     DevBuf p(s, prob, sizeof(float) * NN, false, true);
     DevBuf b(s, bin, sizeof(int32_t) * NN, false, true);
+    DevBuf o(s, offsets, sizeof(float) * 3 * NN, false, true);
     h->u->infer(v.get<float>(), aux ? a.get<float>() : nullptr, (int)N[0], (int)N[1], (int)N[2], halo,
-                prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr, (flags & 1) != 0);
+                prob ? p.get<float>() : nullptr, bin ? b.get<int32_t>() : nullptr, (flags & 1) != 0,
+                offsets ? o.get<float>() : nullptr);
     p.commit();
     b.commit();
+    o.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }

@@ -55,8 +55,12 @@ class Search {
   // scored at its position and 26 neighbours for the A orientations of its
   // group. Outputs (device, [P]): best score, best orientation index (0..A-1,
   // first wins), best voxel (linear index). volume: the searched volume.
+  // centroid (host, 3 floats, optional): the candidates locate R d + p
+  // (the rotated template point d, in voxels from the rotation centre, of a
+  // particle at p) rather than p; each chunk of orientations is then scored
+  // around candidate - round(mean R d) of the chunk.
   void refine(const float* volume, const float* rm_host, int G, int A, const int32_t* cand_dev, long P,
-              float* bscore, int32_t* bangle, int32_t* bpos);
+              const float* centroid, float* bscore, int32_t* bangle, int32_t* bpos);
 
   // Column-tile widths of pass A (z FFTs) and pass B (y FFTs): 8, 16 or 32
   // columns per workgroup (default 16). Results do not depend on them.

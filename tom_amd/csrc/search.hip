@@ -537,9 +537,16 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
                                                  int sy, int sz, const int4* __restrict__ cand,
                                                  const double* __restrict__ mu, const double* __restrict__ sig,
                                                  double inv_m, double mu_t, double sig_t, int a0, int nvalid,
-                                                 float* bscore, int32_t* bangle, int32_t* bpos) {
+                                                 const int4* __restrict__ shift, float* bscore, int32_t* bangle,
+                                                 int32_t* bpos) {
   extern __shared__ float ring[];  // 3 planes of PH x PW
-  const int4 c = cand[blockIdx.x];
+  int4 c = cand[blockIdx.x];
+  if (shift) {  // window centre of this orientation chunk: candidate - shift[group]
+    const int4 h = shift[c.w];
+    c.x = wrapi(c.x - h.x, X);
+    c.y = wrapi(c.y - h.y, Y);
+    c.z = wrapi(c.z - h.z, Z);
+  }
   const long n3 = (long)nx * ny * nz;
   const int PW = nx + 2, PH = ny + 2, PS = PW * PH;
   const int C = (NT_R / ny) > 0 ? NT_R / ny : 1;  // segments per template row
@@ -1154,7 +1161,7 @@ void Search::finalize(float* score, int32_t* angle) {
 }
 
 void Search::refine(const float* volume, const float* rm_host, int G, int A, const int32_t* cand_dev, long P,
-                    float* bscore, int32_t* bangle, int32_t* bpos) {
+                    const float* centroid, float* bscore, int32_t* bangle, int32_t* bpos) {
   TOMHIP_REQUIRE(mode == MODE_FLCF, ERR_WRONG_INPUT, "refinement needs an FLCF plan");
   TOMHIP_REQUIRE(started_, ERR_WRONG_INPUT, "set the template before refining");
   TOMHIP_REQUIRE(G >= 1 && A >= 1 && P >= 1, ERR_WRONG_INPUT, "empty refinement");
@@ -1162,15 +1169,42 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
   constexpr int KS = 4;
   const long n3 = (long)nx * ny * nz;
   const long ntp = (long)KS * G;
+  const int nch = (A + KS - 1) / KS;
+  // With a centroid d (template frame, voxels from the rotation centre) the
+  // candidates locate the rotated centroid R d + p of a particle at p, so
+  // the chunk of orientations a0 .. a0 + KS - 1 of group g is scored around
+  // candidate - round(mean over the chunk of R d).
+  std::vector<int32_t> hshift;
+  if (centroid) {
+    hshift.assign((size_t)nch * G * 4, 0);
+    for (int ch = 0; ch < nch; ++ch)
+      for (int g = 0; g < G; ++g) {
+        const int a1 = std::min(A, (ch + 1) * KS);
+        double acc[3] = {0, 0, 0};
+        for (int a = ch * KS; a < a1; ++a) {
+          const float* r = rm_host + 9 * ((long)a * G + g);  // r[3 i + j] = R[j][i]
+          for (int j = 0; j < 3; ++j)
+            acc[j] += (double)r[j] * centroid[0] + (double)r[3 + j] * centroid[1] + (double)r[6 + j] * centroid[2];
+        }
+        for (int j = 0; j < 3; ++j)
+          hshift[((size_t)ch * G + g) * 4 + j] = (int32_t)std::lround(acc[j] / (a1 - ch * KS));
+      }
+  }
   float* rm = nullptr;
   float* tt = nullptr;
   c32* work = nullptr;
+  int32_t* dshift = nullptr;
   auto release = [&] {
     (void)hipFree(rm);
     (void)hipFree(tt);
     (void)hipFree(work);
+    (void)hipFree(dshift);
   };
   try {
+    if (centroid) {
+      TOMHIP_CHECK(hipMalloc(&dshift, sizeof(int32_t) * hshift.size()));
+      TOMHIP_CHECK(hipMemcpyAsync(dshift, hshift.data(), sizeof(int32_t) * hshift.size(), hipMemcpyHostToDevice, s_));
+    }
     TOMHIP_CHECK(hipMalloc(&rm, sizeof(float) * 9 * (long)G * A));
     TOMHIP_CHECK(hipMalloc(&tt, sizeof(float) * n3 * ntp));
     if (wsh_) TOMHIP_CHECK(hipMalloc(&work, sizeof(c32) * n3 * ntp));
@@ -1187,7 +1221,9 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
       const size_t lds = sizeof(float) * 3 * (nx + 2) * (ny + 2);
       hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P), dim3(NT_R), lds, s_, volume, X, Y, Z, (const float*)tt, G,
                          nx, ny, nz, sx, sy, sz, reinterpret_cast<const int4*>(cand_dev), (const double*)mu_,
-                         (const double*)sig_, inv_m, mu_t, sig_t, a0, cnt, bscore, bangle, bpos);
+                         (const double*)sig_, inv_m, mu_t, sig_t, a0, cnt,
+                         dshift ? reinterpret_cast<const int4*>(dshift) + (long)(a0 / KS) * G : nullptr, bscore,
+                         bangle, bpos);
       TOMHIP_CHECK(hipGetLastError());
     }
     TOMHIP_CHECK(hipStreamSynchronize(s_));

@@ -21,7 +21,8 @@ void conv_weight_grad(const uint16_t* X, int xs, int xo, int Cin, const uint16_t
 
 class Unet {
  public:
-  // D, H, W: input edges (multiples of 4); nbin orientation bins (<= 79).
+  // D, H, W: input edges (multiples of 4); nbin orientation bins (<= 76); the
+  // head's channels are 1 particle logit, nbin bin logits, 3 centre offsets.
   Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool train);
   ~Unet();
   long num_params() const;
@@ -33,18 +34,22 @@ class Unet {
   void forward(const float* vol, const float* aux);
   double forward_flops() const;
   // tprob: device f32 [V] particle targets in [0, 1]; tbin: device int32 [V]
-  // bin index or -1; npos = #{tbin >= 0}. loss_out (host, optional) gets
-  // {mean BCE, mean CE}.
-  void backward(const float* tprob, const int32_t* tbin, float wpos, long npos, double* loss_out);
+  // bin index or -1; npos = #{tbin >= 0}; toff (optional): device f32
+  // [V][4] offset targets (dx, dy, dz in units of 8 voxels, weight > 0 where
+  // supervised), noff = #{weight > 0}. loss_out (host, optional) gets
+  // {mean BCE, mean CE, mean offset squared error}.
+  void backward(const float* tprob, const int32_t* tbin, const float* toff, float wpos, long npos, long noff,
+                double* loss_out);
   void adam(float lr);
-  // prob = sigmoid of the particle logit (the logit itself if logit)
-  void predict(float* prob, int32_t* bin, bool logit = false);
+  // prob = sigmoid of the particle logit (the logit itself if logit); offs
+  // (optional, [V][3]) = the offset channels
+  void predict(float* prob, int32_t* bin, bool logit = false, float* offs = nullptr);
   // whole volume vol (device f32 X x Y x Z) and optional aux map of the same
   // size, each normalised here by its global mean and standard deviation, in
   // tiles of the network size with halo voxels of context on each side;
   // prob / bin (device) receive every voxel
   void infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin,
-             bool logit = false);
+             bool logit = false, float* offs = nullptr);
   const float* logits() const;  // device f32 [V][80]
   int D() const;
   int H() const;

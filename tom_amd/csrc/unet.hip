@@ -418,9 +418,13 @@ __global__ void k_split_add(const float* g, int Ctot, int off, int C, long V, fl
 // heads: logits [V][CH] f32; channel 0 particle logit, 1..nbin orientation.
 // Weighted BCE on channel 0 (mean over voxels), softmax CE over the bins at
 // voxels with a target bin (mean over those). Writes dlogits and loss sums.
-__global__ void k_loss(const float* logits, int CH, int nbin, const float* tprob, const int32_t* tbin, long V,
-                       float wpos, float inv_v, float inv_npos, float* dlog, double* loss) {
-  double l1 = 0, l2 = 0;
+// Offsets: channels 1 + nbin .. 3 + nbin regress (centre - voxel) / 8 where
+// toff[v][3] > 0 (squared error, mean over those voxels).
+__global__ void k_loss(const float* logits, int CH, int nbin, const float* tprob, const int32_t* tbin,
+                       const float* toff, long V, float wpos, float inv_v, float inv_npos, float inv_noff,
+                       float* dlog, double* loss) {
+  double l1 = 0, l2 = 0, l3 = 0;
+  const int oc = 1 + nbin;
   for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
     const float* L = logits + v * CH;
     float* G = dlog + v * CH;
@@ -441,14 +445,23 @@ __global__ void k_loss(const float* logits, int CH, int nbin, const float* tprob
       l2 += -(L[1 + b] - mx - logf(se));
       for (int c = 0; c < nbin; ++c) G[1 + c] = (expf(L[1 + c] - mx) / se - (c == b ? 1.f : 0.f)) * inv_npos;
     }
+    if (toff && toff[v * 4 + 3] > 0.f) {
+      for (int c = 0; c < 3; ++c) {
+        const float d = L[oc + c] - toff[v * 4 + c];
+        l3 += (double)d * d;
+        G[oc + c] = 2.f * d * inv_noff;
+      }
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     l1 += __shfl_xor(l1, o, 64);
     l2 += __shfl_xor(l2, o, 64);
+    l3 += __shfl_xor(l3, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&loss[0], l1);
     atomicAdd(&loss[1], l2);
+    atomicAdd(&loss[2], l3);
   }
 }
 
@@ -480,8 +493,10 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, long n, flo
   }
 }
 
-// prob = sigmoid(particle logit), or the logit itself (no saturation at 1)
-__global__ void k_predict(const float* logits, int CH, int nbin, long V, float* prob, int32_t* bin, int raw) {
+// prob = sigmoid(particle logit), or the logit itself (no saturation at 1);
+// offs (optional) = the three offset channels
+__global__ void k_predict(const float* logits, int CH, int nbin, long V, float* prob, int32_t* bin, float* offs,
+                          int raw) {
   for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
     const float* L = logits + v * CH;
     if (prob) prob[v] = raw ? L[0] : 1.f / (1.f + expf(-L[0]));
@@ -491,6 +506,8 @@ __global__ void k_predict(const float* logits, int CH, int nbin, long V, float* 
         if (L[1 + c] > L[1 + b]) b = c;
       bin[v] = b;
     }
+    if (offs)
+      for (int c = 0; c < 3; ++c) offs[v * 3 + c] = L[1 + nbin + c];
   }
 }
 
@@ -528,8 +545,9 @@ __global__ void k_tile_in(const float* vol, int X, int Y, int Z, int ox, int oy,
 }
 
 // write the tile core [halo, halo + core) clipped to the volume
-__global__ void k_tile_out(const float* tp, const int32_t* tb, Dims t, int halo, int cx, int cy, int cz, int ox,
-                           int oy, int oz, int X, int Y, int Z, float* prob, int32_t* bin) {
+__global__ void k_tile_out(const float* tp, const int32_t* tb, const float* to, Dims t, int halo, int cx, int cy,
+                           int cz, int ox, int oy, int oz, int X, int Y, int Z, float* prob, int32_t* bin,
+                           float* offs) {
   const long nc = (long)cx * cy * cz;
   for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < nc; i += (long)gridDim.x * NTU) {
     const int a = (int)(i % cx), b = (int)((i / cx) % cy), c = (int)(i / ((long)cx * cy));
@@ -539,6 +557,8 @@ __global__ void k_tile_out(const float* tp, const int32_t* tb, Dims t, int halo,
     const long dst = ((long)z * Y + y) * X + x;
     if (prob) prob[dst] = tp[src];
     if (bin) bin[dst] = tb[src];
+    if (offs)
+      for (int c = 0; c < 3; ++c) offs[dst * 3 + c] = to[src * 3 + c];
   }
 }
 
@@ -644,6 +664,7 @@ struct Unet::Impl {
   // tiled inference buffers
   float* tile_in = nullptr;
   float* tile_aux = nullptr;
+  float* tile_offs = nullptr;
   float* tile_prob = nullptr;
   int32_t* tile_bin = nullptr;
   double* sums = nullptr;
@@ -664,7 +685,7 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   Impl& I = *d_;
   TOMHIP_REQUIRE(D % 4 == 0 && H % 4 == 0 && W % 4 == 0 && D >= 8 && H >= 8 && W >= 8, ERR_UNSUPPORTED,
                  "U-Net input edges must be multiples of 4 and >= 8");
-  TOMHIP_REQUIRE(nbin >= 1 && nbin <= kHeadCh - 1, ERR_WRONG_INPUT, "nbin out of range");
+  TOMHIP_REQUIRE(nbin >= 1 && nbin <= kHeadCh - 4, ERR_WRONG_INPUT, "nbin out of range (1..76)");
   I.s = s;
   I.D = D;
   I.H = H;
@@ -700,7 +721,7 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
       const int ci = (int)(i % c.Cin);
       const int co = (int)((i / c.Cin) % c.Cout);
       if (I.L.empty() && ci > 1) continue;           // padded input channels
-      if (c.taps == 1 && co > nbin) continue;        // unused head channels
+      if (c.taps == 1 && co > nbin + 3) continue;    // unused head channels
       h[i] = nd(rng);
     }
     TOMHIP_CHECK(hipMemcpyAsync(c.p, h.data(), sizeof(float) * c.np, hipMemcpyHostToDevice, s));
@@ -723,7 +744,7 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   I.d1a = I.alloc<uint16_t>(V0 * 16);
   I.d1b = I.alloc<uint16_t>(V0 * 16);
   I.logits = I.alloc<float>(V0 * kHeadCh);
-  I.loss = I.alloc<double>(2);
+  I.loss = I.alloc<double>(3);
   I.train = train;
   if (!train) {
     refresh_weights();
@@ -834,25 +855,27 @@ double Unet::forward_flops() const {
   return f;
 }
 
-void Unet::predict(float* prob, int32_t* bin, bool logit) {
+void Unet::predict(float* prob, int32_t* bin, bool logit, float* offs) {
   Impl& I = *d_;
   hipLaunchKernelGGL(k_predict, dim3(gridv(I.V0)), dim3(NTU), 0, I.s, (const float*)I.logits, kHeadCh, I.nbin,
-                     I.V0, prob, bin, logit ? 1 : 0);
+                     I.V0, prob, bin, offs, logit ? 1 : 0);
   TOMHIP_CHECK(hipGetLastError());
 }
 
-void Unet::backward(const float* tprob, const int32_t* tbin, float wpos, long npos, double* loss_out) {
+void Unet::backward(const float* tprob, const int32_t* tbin, const float* toff, float wpos, long npos, long noff,
+                    double* loss_out) {
   Impl& I = *d_;
   TOMHIP_REQUIRE(I.train, ERR_WRONG_INPUT, "U-Net was created for inference only");
   hipStream_t s = I.s;
   const int D = I.D, H = I.H, W = I.W;
   Dims d0{D, H, W}, d1{D / 2, H / 2, W / 2}, d2{D / 4, H / 4, W / 4};
   auto& L = I.L;
-  TOMHIP_CHECK(hipMemsetAsync(I.loss, 0, sizeof(double) * 2, s));
+  TOMHIP_CHECK(hipMemsetAsync(I.loss, 0, sizeof(double) * 3, s));
   for (auto& c : L) TOMHIP_CHECK(hipMemsetAsync(c.g, 0, sizeof(float) * c.np, s));
   const float inv_v = 1.f / (float)I.V0, inv_np = npos > 0 ? 1.f / (float)npos : 0.f;
+  const float inv_no = noff > 0 ? 1.f / (float)noff : 0.f;
   hipLaunchKernelGGL(k_loss, dim3(gridv(I.V0)), dim3(NTU), 0, s, (const float*)I.logits, kHeadCh, I.nbin, tprob,
-                     tbin, I.V0, wpos, inv_v, inv_np, I.g_logits, I.loss);
+                     tbin, toff, I.V0, wpos, inv_v, inv_np, inv_no, I.g_logits, I.loss);
   // zero the accumulated data gradients
   auto zero = [&](float* p, long n) { TOMHIP_CHECK(hipMemsetAsync(p, 0, sizeof(float) * n, s)); };
   zero(I.g_c1, I.V0 * 48);
@@ -900,11 +923,12 @@ void Unet::backward(const float* tprob, const int32_t* tbin, float wpos, long np
   layer_bwd(0, I.g_a1, I.a1, 16, 0, I.x0, 16, 0, d0, nullptr);
   TOMHIP_CHECK(hipGetLastError());
   if (loss_out) {
-    double h[2];
+    double h[3];
     TOMHIP_CHECK(hipMemcpyAsync(h, I.loss, sizeof(h), hipMemcpyDeviceToHost, s));
     TOMHIP_CHECK(hipStreamSynchronize(s));
     loss_out[0] = h[0] / (double)I.V0;
     loss_out[1] = npos > 0 ? h[1] / (double)npos : 0.0;
+    loss_out[2] = noff > 0 ? h[2] / (double)noff : 0.0;
   }
 }
 
@@ -922,7 +946,7 @@ void Unet::adam(float lr) {
 }
 
 void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin,
-                 bool logit) {
+                 bool logit, float* offs) {
   Impl& I = *d_;
   hipStream_t s = I.s;
   TOMHIP_REQUIRE(halo >= 0 && halo % 4 == 0, ERR_WRONG_INPUT, "halo must be a non-negative multiple of 4");
@@ -935,6 +959,7 @@ void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int ha
     I.sums = I.alloc<double>(4);
   }
   if (aux && !I.tile_aux) I.tile_aux = I.alloc<float>(I.V0);
+  if (offs && !I.tile_offs) I.tile_offs = I.alloc<float>(I.V0 * 3);
   const long n = (long)X * Y * Z;
   TOMHIP_CHECK(hipMemsetAsync(I.sums, 0, sizeof(double) * 4, s));
   hipLaunchKernelGGL(k_vsum, dim3(gridv(n)), dim3(NTU), 0, s, vol, n, I.sums);
@@ -949,9 +974,10 @@ void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int ha
           hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, aux, X, Y, Z, ox, oy, oz, t,
                              (const double*)(I.sums + 2), n, I.tile_aux);
         forward(I.tile_in, aux ? I.tile_aux : nullptr);
-        predict(I.tile_prob, I.tile_bin, logit);
+        predict(I.tile_prob, I.tile_bin, logit, offs ? I.tile_offs : nullptr);
         hipLaunchKernelGGL(k_tile_out, dim3(gridv((long)cx * cy * cz)), dim3(NTU), 0, s, (const float*)I.tile_prob,
-                           (const int32_t*)I.tile_bin, t, halo, cx, cy, cz, ox, oy, oz, X, Y, Z, prob, bin);
+                           (const int32_t*)I.tile_bin, (const float*)I.tile_offs, t, halo, cx, cy, cz, ox, oy, oz,
+                           X, Y, Z, prob, bin, offs);
       }
   TOMHIP_CHECK(hipGetLastError());
 }

@@ -66,36 +66,76 @@ def exhaustive_peaks(score, angle, template_size: int, threshold: float, max_pea
     return pk
 
 
-def label_volumes(shape_zyx, peaks, angles, bins, sigma: float = 3.0, bin_radius: float = 4.0):
+OFFSET_SCALE = 8.0  # offset channels predict (centre - voxel) / OFFSET_SCALE
+
+
+def template_centroid(T):
+    """Centre of mass of the template's positive density, x y z in voxels from
+    the rotation centre ceil(n/2) (0-based) of tom_rotate. A particle whose
+    exhaustive peak is at p with orientation R has its density centre at
+    p + R d: the proposal network locates that point, which does not depend
+    on the orientation the way the peak does."""
+    T = np.clip(np.asarray(T, np.float64), 0, None)
+    g = np.indices(T.shape, dtype=np.float64)
+    c = np.array([(s + 1) // 2 for s in T.shape], np.float64)
+    return np.array([(g[i] * T).sum() / T.sum() for i in range(3)]) - c
+
+
+def centroid_positions(peaks, angles, d):
+    """Rows x y z of the density centres p + R(angle) d of picklist rows
+    (x y z value angle-index)."""
+    if len(peaks) == 0:
+        return np.zeros((0, 3))
+    R = orient.rotmatrix(np.asarray(angles, np.float64)[peaks[:, 4].astype(np.int64)])
+    return peaks[:, :3] + R @ np.asarray(d, np.float64)
+
+
+def label_volumes(shape_zyx, peaks, angles, bins, sigma: float = 3.0, bin_radius: float = 4.0,
+                  offset_radius: float = 0.0):
     """Targets for the proposal network: tprob = exp(-d^2 / (2 sigma^2)) of
     the distance d to the nearest peak (0 beyond 3 sigma), tbin = bin of that
     peak's best orientation within bin_radius of it (-1 elsewhere).
-    peaks: (P, 5) rows x y z value angle-index; angles: the search's list."""
+    peaks: (P, 5) rows x y z value angle-index; angles: the search's list.
+    With offset_radius > 0 also returns toff (z, y, x, 4): (dx, dy, dz) /
+    OFFSET_SCALE from the voxel to the nearest peak and weight 1 within
+    offset_radius of it (0 elsewhere)."""
     tprob = np.zeros(shape_zyx, np.float32)
     tbin = np.full(shape_zyx, -1, np.int32)
+    toff = np.zeros(tuple(shape_zyx) + (4,), np.float32) if offset_radius > 0 else None
     if len(peaks) == 0:
-        return tprob, tbin
+        return (tprob, tbin) if toff is None else (tprob, tbin, toff)
     b = orient.assign_bins(angles[peaks[:, 4].astype(np.int64)], bins)
-    r = int(np.ceil(3 * sigma))
+    r = int(np.ceil(max(3 * sigma, offset_radius)))
     g = np.arange(-r, r + 1)
     dz, dy, dx = np.meshgrid(g, g, g, indexing="ij")
     d2 = (dx * dx + dy * dy + dz * dz).ravel()
-    keep = d2 <= (3 * sigma) ** 2
+    keep = d2 <= max(3 * sigma, offset_radius) ** 2
     off = np.stack([dz.ravel(), dy.ravel(), dx.ravel()], 1)[keep]
-    val = np.exp(-d2[keep] / (2 * sigma * sigma)).astype(np.float32)
-    inb = d2[keep] <= bin_radius * bin_radius
+    d2 = d2[keep]
+    val = np.where(d2 <= (3 * sigma) ** 2, np.exp(-d2 / (2 * sigma * sigma)), 0.0).astype(np.float32)
+    inb = d2 <= bin_radius * bin_radius
+    ino = d2 <= offset_radius * offset_radius
     Z, Y, X = shape_zyx
     best = np.zeros(shape_zyx, np.float32)
+    near = np.full(shape_zyx, np.inf, np.float32)
     for (x, y, z), bb in zip(peaks[:, :3].astype(np.int64), b):
         zz, yy, xx = z + off[:, 0], y + off[:, 1], x + off[:, 2]
         ok = (zz >= 0) & (zz < Z) & (yy >= 0) & (yy < Y) & (xx >= 0) & (xx < X)
-        zz, yy, xx, v, ib = zz[ok], yy[ok], xx[ok], val[ok], inb[ok]
+        zz, yy, xx, v, ib, io, dd, o = zz[ok], yy[ok], xx[ok], val[ok], inb[ok], ino[ok], d2[ok], off[ok]
         up = v > best[zz, yy, xx]
         best[zz[up], yy[up], xx[up]] = v[up]
         sel = up & ib
         tbin[zz[sel], yy[sel], xx[sel]] = bb
+        if toff is not None:
+            nr = io & (dd < near[zz, yy, xx])
+            near[zz[nr], yy[nr], xx[nr]] = dd[nr]
+            # voxel -> centre is minus the stencil offset (centre -> voxel)
+            toff[zz[nr], yy[nr], xx[nr], 0] = -o[nr, 2] / OFFSET_SCALE
+            toff[zz[nr], yy[nr], xx[nr], 1] = -o[nr, 1] / OFFSET_SCALE
+            toff[zz[nr], yy[nr], xx[nr], 2] = -o[nr, 0] / OFFSET_SCALE
+            toff[zz[nr], yy[nr], xx[nr], 3] = 1.0
     tprob[:] = best
-    return tprob, tbin
+    return (tprob, tbin) if toff is None else (tprob, tbin, toff)
 
 
 def dense_targets(score_zyx, angle_zyx, angles, bins, s0: float = 0.1, s1: float = 0.5, bin_thr: float = 0.25):
@@ -128,19 +168,19 @@ class Trainer:
         self.data = []
         self.history = []
 
-    def add(self, volume_zyx, tprob, tbin, aux_zyx=None):
+    def add(self, volume_zyx, tprob, tbin, aux_zyx=None, toff=None):
         """volume_zyx (and the aux map): numpy (z, y, x) float32; targets of
-        the same shape."""
+        the same shape (toff: (z, y, x, 4) offset targets, optional)."""
         def norm(a):
             a = np.asarray(a, np.float64)
             return ((a - a.mean()) / a.std()).astype(np.float32)
         vn = norm(volume_zyx)
         an = norm(aux_zyx) if aux_zyx is not None else None
         pos = np.argwhere(tprob > 0.5)
-        self.data.append((vn, tprob, tbin, pos, an))
+        self.data.append((vn, tprob, tbin, pos, an, toff))
 
     def _crop(self):
-        vn, tp, tb, pos, an = self.data[self.rng.integers(len(self.data))]
+        vn, tp, tb, pos, an, to = self.data[self.rng.integers(len(self.data))]
         c = self.crop
         Z, Y, X = vn.shape
         if len(pos) and self.rng.random() < 0.7:
@@ -150,24 +190,27 @@ class Trainer:
         z, y, x = (int(np.clip(z, 0, Z - c)), int(np.clip(y, 0, Y - c)), int(np.clip(x, 0, X - c)))
         sl = (slice(z, z + c), slice(y, y + c), slice(x, x + c))
         f = lambda a: np.asfortranarray(np.ascontiguousarray(a[sl]).transpose(2, 1, 0))  # noqa: E731
-        return f(vn), f(tp), f(tb), (f(an) if an is not None else None)
+        # offsets keep their component axis fastest: (4, x, y, z) F-order
+        o = np.ascontiguousarray(to[sl]).transpose(3, 2, 1, 0) if to is not None else None
+        return f(vn), f(tp), f(tb), (f(an) if an is not None else None), o
 
     def step(self, lr=None):
-        v, tp, tb, a = self._crop()
+        v, tp, tb, a, o = self._crop()
         self.net.forward(v, a)
-        bce, ce = self.net.backward(tp, tb, self.wpos)
+        bce, ce, off = self.net.backward(tp, tb, self.wpos, toff=o)
         self.net.adam(self.lr if lr is None else lr)
-        self.history.append((bce, ce))
-        return bce, ce
+        self.history.append((bce, ce, off))
+        return bce, ce, off
 
     def train(self, steps, log=None, every=100):
         """Cosine learning-rate decay from lr to lr_min over the steps."""
         t0 = time.perf_counter()
         for i in range(steps):
             lr = self.lr_min + 0.5 * (self.lr - self.lr_min) * (1 + np.cos(np.pi * i / max(1, steps - 1)))
-            bce, ce = self.step(lr)
+            bce, ce, off = self.step(lr)
             if log and (i % every == 0 or i == steps - 1):
-                log(f"[train] step {i}: bce {bce:.4f} ce {ce:.4f} ({time.perf_counter() - t0:.1f} s)")
+                log(f"[train] step {i}: bce {bce:.4f} ce {ce:.4f} offset {off:.4f} "
+                    f"({time.perf_counter() - t0:.1f} s)")
         return time.perf_counter() - t0
 
 
@@ -196,13 +239,18 @@ class Surrogate:
     edge `tile` with `halo` voxels of context."""
 
     def __init__(self, params, bins, tile=304, halo=24, half=15.0, step=5.0, nbin=64, whole_max=160_000_000,
-                 snap_radius=4):
+                 snap_radius=0, shift_iters=2, centroid=None):
         self.params = np.asarray(params, np.float32)
         self.nbin = nbin
         self.bins = np.asarray(bins, np.float64)
         self.tile, self.halo, self.whole_max = tile, halo, whole_max
         self.half, self.step = half, step
         self.snap_radius = snap_radius
+        self.shift_iters = shift_iters
+        # template_centroid of the template when the network locates density
+        # centres (the refinement then scores each orientation around
+        # candidate - R d); None when it locates the peaks themselves
+        self.centroid = None if centroid is None else np.asarray(centroid, np.float64)
         self.timings = {}
         self._net, self._net_dims = None, None
 
@@ -228,11 +276,21 @@ class Surrogate:
         dims = tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume)
         net, halo = self._network(dims)
         t0 = time.perf_counter()
-        prob, b = net.infer(volume, halo, aux=aux, logit=True)
+        shift = self.shift_iters > 0
+        out = net.infer(volume, halo, aux=aux, logit=True, offsets=shift)
+        prob, b = out[:2]
         _sync(prob)
         t1 = time.perf_counter()
         lm = api.local_maxima(prob, 2, float(np.log(threshold / (1 - threshold))))
         t2 = time.perf_counter()
+        if shift and len(lm):
+            # move each maximum along the predicted vector to the particle
+            # centre (repeated: the vector is read again where it points)
+            xyz = lm[:, :3].astype(np.int64)
+            for _ in range(self.shift_iters):
+                o = _gather(out[2], xyz, vec=True)
+                xyz = np.clip(np.rint(xyz + OFFSET_SCALE * o), 0, np.asarray(dims) - 1).astype(np.int64)
+            lm = np.concatenate([xyz.astype(np.float32), lm[:, 3:]], 1)
         keep = greedy_nms(lm[:, :3], lm[:, 3], nms_radius, K) if len(lm) else np.zeros(0, np.int64)
         lm = lm[keep]
         if aux is not None and self.snap_radius > 0 and len(lm):
@@ -242,16 +300,11 @@ class Surrogate:
             lm = np.concatenate([xyz.astype(np.float32), lm[:, 3:]], 1)
             _, first = np.unique(xyz, axis=0, return_index=True)
             lm = lm[np.sort(first)]
-        bh = b.cpu().numpy() if hasattr(b, "cpu") else np.asarray(b)
-        xyz = lm[:, :3].astype(np.int64)
-        if hasattr(b, "cpu"):  # torch (z, y, x)
-            bins = bh[xyz[:, 2], xyz[:, 1], xyz[:, 0]]
-        else:  # numpy (x, y, z)
-            bins = bh[xyz[:, 0], xyz[:, 1], xyz[:, 2]]
+        bins = _gather(b, lm[:, :3].astype(np.int64))
         t3 = time.perf_counter()
         self.timings.update(infer_s=t1 - t0, local_max_s=t2 - t1, nms_s=t3 - t2)
         self.infer_flops = unet_forward_flops(net.dims) * _tiles(dims, net.dims, halo)
-        return np.concatenate([lm[:, :3], bins[:, None], lm[:, 3:4]], 1)
+        return np.concatenate([lm[:, :3], bins[:, None].astype(np.float32), lm[:, 3:4]], 1)
 
     def refine(self, plan, volume, cand):
         """Exact FLCF over the local grid of each candidate's bin and its 27
@@ -260,10 +313,22 @@ class Surrogate:
         ub, grp = np.unique(cand[:, 3].astype(np.int64), return_inverse=True)
         grids = np.stack([orient.local_grid(self.bins[b], self.half, self.step) for b in ub], 1)  # (A, G, 3)
         c4 = np.concatenate([cand[:, :3].astype(np.int32), grp[:, None].astype(np.int32)], 1)
-        sc, ai, pos = plan.refine(volume, grids, c4)
+        sc, ai, pos = plan.refine(volume, grids, c4, centroid=self.centroid)
         eul = grids[ai, grp]
         self.timings.update(refine_s=time.perf_counter() - t0)
         return sc, eul, pos
+
+
+def _gather(a, xyz, vec=False):
+    """Values of a map at rows x y z: a torch (z, y, x[, 3]) tensor (gathered
+    on the device) or a numpy (x, y, z) / (3, x, y, z) F-order array."""
+    if hasattr(a, "cpu"):
+        import torch
+        i = torch.as_tensor(xyz, device=a.device)
+        return a[i[:, 2], i[:, 1], i[:, 0]].cpu().numpy()
+    if vec:
+        return np.asarray(a)[:, xyz[:, 0], xyz[:, 1], xyz[:, 2]].T
+    return np.asarray(a)[xyz[:, 0], xyz[:, 1], xyz[:, 2]]
 
 
 def _tiles(dims, tdims, halo):

@@ -6,9 +6,11 @@ Training data: seeded synthetic tomograms (edge --size, --particles copies of
 the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
 missing wedge of a +-60 degree tilt series), each searched exhaustively on the
 GPU at 15 degrees; the picklist of the score map (tom_os3_returnPicklist
-semantics) above --label-threshold gives the particle targets and the best
+semantics) above --label-threshold gives the particle targets (at the density
+centre p + R d of each pick: surrogate.template_centroid) and the best
 orientation there the bin targets (a Gaussian heat map of sigma --sigma voxels
-around each peak; bins within 4 voxels); --dense adds targets from the
+around each peak; bins within 4 voxels), and voxels within --offset-radius of
+a peak learn the vector to it (the candidates are moved by it); --dense adds targets from the
 exhaustive maps at every voxel (the best score mapped linearly from [0.1, 0.5]
 to [0, 1], and the bin of the best orientation wherever the score exceeds
 0.25), which trains a broader response that localises worse. The network's
@@ -46,7 +48,7 @@ def workload_template():
 
 
 def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, seed, thr, bins, sigma=2.0,
-                   dense=False):
+                   dense=False, offset_radius=10.0):
     from . import surrogate, synth
     rng = np.random.default_rng(seed)
     V, pos, eul, _ = synth.make_tomogram((size, size, size), [T], P, 40, snr, rng, seed, api, torch, dev, tilt_max=60.0)
@@ -57,13 +59,20 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
         aux = surrogate.aux_map(plan, T, M, W).cpu().numpy()
     peaks = surrogate.exhaustive_peaks(score, ang, T.shape[0], thr)
     vz = V.cpu().numpy()
-    tprob, tbin = surrogate.label_volumes(vz.shape, peaks, angles, bins, sigma=sigma)
+    # targets sit at the density centres p + R d of the picks, which a
+    # translation-equivariant network can locate without knowing R
+    cp = peaks.copy()
+    if len(cp):
+        cp[:, :3] = np.rint(surrogate.centroid_positions(peaks, angles, surrogate.template_centroid(T)))
+        cp[:, :3] = np.clip(cp[:, :3], 0, np.asarray(vz.shape[::-1]) - 1)
+    tprob, tbin, toff = surrogate.label_volumes(vz.shape, cp, angles, bins, sigma=sigma,
+                                                offset_radius=offset_radius)
     if dense:
         # dense FLCF targets everywhere, the peak heat map on top at the picks
         tp_dense, tb_dense = surrogate.dense_targets(score.cpu().numpy(), ang.cpu().numpy(), angles, bins)
         tprob = np.maximum(tp_dense, tprob)
         tbin = np.where(tbin >= 0, tbin, tb_dense)
-    return vz, tprob, tbin, peaks, pos, aux
+    return vz, tprob, tbin, peaks, pos, aux, toff
 
 
 def main(argv=None):
@@ -82,6 +91,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=100)
     ap.add_argument("--sigma", type=float, default=2.0, help="heat-map sigma around the exhaustive peaks (voxels)")
     ap.add_argument("--dense", action="store_true", help="add dense targets from the exhaustive score map")
+    ap.add_argument("--offset-radius", type=float, default=10.0,
+                    help="voxels within this distance of a peak learn the offset to it")
     args = ap.parse_args(argv)
 
     import torch
@@ -99,10 +110,11 @@ def main(argv=None):
     npk = []
     for i in range(args.tomos):
         snr = float(rng.uniform(args.snr_lo, args.snr_hi))
-        vz, tp, tb, peaks, pos, aux = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
-                                                     args.particles, snr, args.seed + i, args.label_threshold, bins,
-                                                     sigma=args.sigma, dense=args.dense)
-        tr.add(vz, tp, tb, aux)
+        vz, tp, tb, peaks, pos, aux, to = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
+                                                         args.particles, snr, args.seed + i, args.label_threshold,
+                                                         bins, sigma=args.sigma, dense=args.dense,
+                                                         offset_radius=args.offset_radius)
+        tr.add(vz, tp, tb, aux, to)
         npk.append(len(peaks))
         log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold}, "
             f"{surrogate.recall(pos, peaks[:, :3]):.3f} of the {len(pos)} planted particles among them "
@@ -113,9 +125,9 @@ def main(argv=None):
     theta = tr.net.get_params()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     meta = dict(vars(args), label_s=t_label, train_s=t_train, peaks_per_tomogram=npk,
-                final_loss=[float(np.mean([h[0] for h in tr.history[-100:]])),
-                            float(np.mean([h[1] for h in tr.history[-100:]]))])
-    np.savez_compressed(args.out, params=theta, bins=bins, meta=json.dumps(meta))
+                final_loss=[float(np.mean([h[k] for h in tr.history[-100:]])) for k in range(3)])
+    np.savez_compressed(args.out, params=theta, bins=bins, centroid=surrogate.template_centroid(T),
+                        meta=json.dumps(meta))
     log(f"[train] saved {args.out}: label {t_label:.1f} s, train {t_train:.1f} s")
     print(json.dumps(meta))
 
