@@ -124,59 +124,67 @@ def cpu_baseline(cfg, V_host, T, M, W, angles, k, workers):
     return N * k / dt, t_pre, dt, maps
 
 
-def surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath, exhaustive_s, plan_template, pre_s):
-    """Proposal network + exact local refinement on the benchmark tomogram;
-    recall of the exhaustive search's picklist peaks above 0.3."""
+# coarse-to-fine pruned search per workload: coarse orientations, candidates
+# kept, refinement levels (+-half, step in degrees)
+PRUNED = {
+    "c1": dict(n_coarse=256, K=64, levels=((10.0, 10.0), (5.0, 5.0))),
+    "c2": dict(n_coarse=1024, K=400, levels=((10.0, 10.0), (5.0, 5.0))),
+    "c5": dict(n_coarse=1536, K=2400, levels=((15.0, 15.0), (7.5, 7.5), (3.75, 3.75))),
+}
+
+
+def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, plan_template, pre_s):
+    """Coarse-to-fine pruned search on the benchmark tomogram, judged by the
+    exhaustive run: recall of its picklist peaks above 0.3 (a refined hit
+    within 3 voxels with refined score > 0.3) and the wall-time ratio. When
+    trained weights exist, the proposal network (second input channel: the
+    coarse score map) is compared with the coarse map's own local maxima at
+    the same coarse scan, K and refinement."""
     from tom_amd import surrogate as S
-    w = np.load(wpath, allow_pickle=False)
-    # networks trained on density-centre targets carry the template centroid
-    cen = np.asarray(w["centroid"], np.float64) if "centroid" in w.files else None
-    sg = S.Surrogate(w["params"], w["bins"], centroid=cen)
-    ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
     T, M, W, tn, mu_t, sig_t = plan_template
-    # warm (network allocation, code objects)
-    aux = S.aux_map(plan, T, M, W)
-    sg.propose(V, K=16, aux=aux)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    aux = S.aux_map(plan, T, M, W)
-    plan.set_template(tn, W, mu_t, sig_t)
-    torch.cuda.synchronize()
-    t_aux = time.perf_counter() - t0
-    cand = sg.propose(V, K=args.topk, threshold=0.02, aux=aux)
-    sc, eul, pos = sg.refine(plan, V, cand)
-    torch.cuda.synchronize()
-    # the refinement needs the plan's one-off precompute (F_I, FP64 statistics)
-    # just as the exhaustive search does
-    t_sur = time.perf_counter() - t0 + pre_s
-    sg.timings["aux_s"] = t_aux
-    sg.timings["precompute_s"] = pre_s
-    rec_hit = S.recall(ref[:, :3], pos[sc > 0.3])
-    from scipy.spatial import cKDTree
-    # candidates locate density centres when the network was trained on them
-    ref_c = S.centroid_positions(ref, angles, cen) if cen is not None else ref[:, :3]
-    d_cand = cKDTree(cand[:, :3]).query(ref_c)[0] if len(cand) else np.full(len(ref), np.inf)
-    d_ref = cKDTree(pos).query(ref[:, :3])[0] if len(pos) else np.full(len(ref), np.inf)
-    rec_any = float(np.mean(d_cand <= 3.0))
-    dist = {"candidate_p50_p90_max": [round(float(np.percentile(d_cand, q)), 2) for q in (50, 90, 100)],
-            "refined_p50_p90_max": [round(float(np.percentile(d_ref, q)), 2) for q in (50, 90, 100)],
-            "recall_refined_within_5": float(np.mean(d_ref <= 5))}
-    log(f"[bench] surrogate distances to the exhaustive peaks: {dist}")
-    # exhaustive best score at the matched peaks vs the refined score there
-    log(f"[bench] surrogate: {len(cand)} candidates, {t_sur:.3f} s (exhaustive {exhaustive_s:.3f} s); "
-        f"recall of {len(ref)} exhaustive peaks > 0.3: {rec_any:.4f} (any candidate), "
-        f"{rec_hit:.4f} (refined score > 0.3); timings {sg.timings}")
-    meta = json.loads(str(w["meta"]))
-    return {"weights": os.path.relpath(wpath, ROOT), "candidates": int(len(cand)), "topk": args.topk,
-            "wall_s": round(t_sur, 4), "exhaustive_wall_s": round(exhaustive_s, 4),
-            "speedup": round(exhaustive_s / t_sur, 2), "exhaustive_peaks_gt_0.3": int(len(ref)),
-            "recall_any_candidate": rec_any, "recall_refined_gt_0.3": rec_hit, "match_distances": dist,
-            "timings_s": {k: round(v, 4) for k, v in sg.timings.items()},
-            "unet_infer": {"flops": sg.infer_flops, "tflops_per_s": round(sg.infer_flops / sg.timings["infer_s"] / 1e12, 2),
-                           "frac_of_bf16_dense_peak": round(sg.infer_flops / sg.timings["infer_s"] / 2.5e15, 4)},
-            "refine_orientations_per_candidate": 343,
-            "training": {k: meta[k] for k in ("tomos", "size", "particles", "steps", "crop", "label_s", "train_s")
-                         if k in meta}}
+    ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
+
+    def timed(ps, proposer=None):
+        ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer)  # warm: graph capture, code objects
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc, eul, pos = ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0 + pre_s  # needs the plan's precompute, as the exhaustive run does
+        rec = S.recall(ref[:, :3], pos[sc > 0.3])
+        return {"recall_gt_0.3": round(rec, 4), "wall_s": round(wall, 4), "speedup": round(exhaustive_s / wall, 2),
+                "candidates": int(ps.n_candidates), "n_coarse": ps.n_coarse, "K": ps.K,
+                "levels": [list(l) for l in ps.levels],
+                "timings_s": {k: round(v, 4) for k, v in ps.timings.items()}}
+
+    par = PRUNED.get(args.config, PRUNED["c2"])
+    ps = S.PrunedSearch(n_coarse=args.coarse or par["n_coarse"], K=par["K"], levels=par["levels"])
+    out = timed(ps)
+    out.update(exhaustive_wall_s=round(exhaustive_s, 4), exhaustive_peaks_gt_0_3=int(len(ref)),
+               precompute_s=round(pre_s, 4))
+    log(f"[bench] pruned search: recall {out['recall_gt_0.3']:.4f} of {len(ref)} exhaustive peaks > 0.3, "
+        f"{out['wall_s']:.3f} s vs exhaustive {exhaustive_s:.3f} s ({out['speedup']}x); {out['timings_s']}")
+    if wpath and os.path.exists(wpath):
+        w = np.load(wpath, allow_pickle=False)
+        meta = json.loads(str(w["meta"]))
+        if meta.get("aux") == "coarse":
+            nc = int(meta.get("coarse", 256))
+            K = par["K"]
+            lv = ((10.0, 10.0), (5.0, 5.0))
+            net = S.Surrogate(w["params"], w["bins"])
+            base = timed(S.PrunedSearch(n_coarse=nc, K=K, levels=lv))
+            withnet = timed(S.PrunedSearch(n_coarse=nc, K=K, levels=lv), proposer=net)
+            fl = net.infer_flops
+            ti = net.timings["infer_s"]
+            withnet["unet_infer"] = {"flops": fl, "s": round(ti, 4), "tflops_per_s": round(fl / ti / 1e12, 2),
+                                     "frac_of_bf16_dense_peak": round(fl / ti / 2.5e15, 4)}
+            out["network_vs_coarse_maxima"] = {
+                "weights": os.path.relpath(wpath, ROOT), "coarse_maxima": base, "network_proposer": withnet,
+                "training": {k: meta[k] for k in ("tomos", "size", "particles", "steps", "crop", "aux", "coarse",
+                                                  "targets", "label_s", "train_s") if k in meta}}
+            log(f"[bench] coarse {nc}: maxima recall {base['recall_gt_0.3']:.4f} ({base['wall_s']:.3f} s), "
+                f"network proposer recall {withnet['recall_gt_0.3']:.4f} ({withnet['wall_s']:.3f} s)")
+    return out
 
 
 C3 = dict(P=4096, edges=(32, 48, 64), A=27, step=5.0, seed=3,
@@ -303,9 +311,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="do not record per-kernel events")
     ap.add_argument("--calib-copy", action="store_true", help="run one 1 GiB device copy (PMC calibration)")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "tom_amd", "weights", "unet_c2.npz"),
-                    help="trained proposal-network weights (skipped when absent)")
-    ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--topk", type=int, default=2000)
+                    help="trained proposal-network weights (the network comparison is skipped when absent)")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the pruned-search leg")
+    ap.add_argument("--coarse", type=int, default=0, help="coarse orientations of the pruned search (0 = default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -364,7 +372,11 @@ def main():
         log(f"[bench] autotuned batch {KA}")
     t0 = time.perf_counter()
     plan = api.SearchPlan(V, (cfg["n"],) * 3, M, batch=KA)
-    plan.set_tiles(*tiles)
+    try:
+        plan.set_tiles(*tiles)
+    except api.TomHipError:
+        log(f"[bench] tiles {tiles} do not fit this volume; using the default 16,16")
+        tiles = (16, 16)
     plan.set_zchunk(zchunk)
     torch.cuda.synchronize()
     pre_ms = (time.perf_counter() - t0) * 1e3
@@ -411,14 +423,14 @@ def main():
     sc = score.cpu().numpy().transpose(2, 1, 0)
     found = float(np.mean([sc[tuple(p)] > 0.2 for p in pos])) if ntimed == A else None
 
-    # ---- learned proposal on the same tomogram, judged by the exhaustive run
+    # ---- pruned search (and the proposal network) on the same tomogram,
+    # judged by the exhaustive run
     sur = None
-    wpath = args.surrogate
-    if wpath and os.path.exists(wpath) and ntimed == A and not args.no_surrogate:
-        sur = surrogate_leg(args, cfg, api, torch, plan, V, score, ang, angles, wpath,
-                            exhaustive_s=(pre_ms / 1e3 + (graph["ms_per_step"] * groups / 1e3 if graph else dt)
-                                          + fin_ms / 1e3), plan_template=(T, M, W, tn, mu_t, sig_t),
-                            pre_s=pre_ms / 1e3)
+    if ntimed == A and not args.no_surrogate:
+        sur = pruned_leg(args, cfg, api, torch, plan, V, score, ang, args.surrogate,
+                         exhaustive_s=(pre_ms / 1e3 + (graph["ms_per_step"] * groups / 1e3 if graph else dt)
+                                       + fin_ms / 1e3), plan_template=(T, M, W, tn, mu_t, sig_t),
+                         pre_s=pre_ms / 1e3)
 
     # ---- roofline of the dominant kernel (event times over the timed region)
     names = ["rotate", "box_weight", "tmpl_fwd", "passA", "passB", "passC"]
@@ -483,7 +495,7 @@ def main():
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
                    "timed_path": "eager launches with per-kernel HIP events",
-                   "graph_replay": graph, "autotune": tuned, "surrogate": sur},
+                   "graph_replay": graph, "autotune": tuned, "pruned_search": sur},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,8 +1,19 @@
-"""Learned-proposal template matching (no reference counterpart): a 3D U-Net
-proposes particle positions and one of 64 coarse orientation bins; the exact
-FLCF (the same score as the exhaustive search) then refines each candidate over
-+-15 degrees in 5-degree steps on each Euler angle around its bin centre and
-over the 27 voxels around its position.
+"""Pruned template matching (no reference counterpart): instead of scanning
+every orientation over the whole tomogram, find candidate particles cheaply
+and score only a local orientation set at each of them with the exact FLCF
+(the same score as the exhaustive search).
+
+* PrunedSearch: the exhaustive machinery scans a few hundred to a few thousand
+  orientations spread over SO(3); local maxima of that best-score map are the
+  candidates; two or three levels of exact local refinement (Euler grids of
+  shrinking step around the best orientation so far, 27 voxels around the
+  best position so far) recover the fine orientation and score.
+* Surrogate: a 3D U-Net (BF16 MFMA kernels of this library, trained on the
+  GPU) proposes particle positions and one of 64 coarse orientation bins from
+  the tomogram and a second channel (the FLCF map of the rotationally averaged
+  template, or the coarse scan's score map); candidates are refined over
+  +-15 degrees in 5-degree steps around the bin centre at 27 voxels, or passed
+  to PrunedSearch as its proposer.
 
 Training labels come from the exhaustive search itself: peaks of the FLCF map
 (tom_os3_returnPicklist semantics) above a score threshold mark particles, and
@@ -351,3 +362,80 @@ def recall(ref_xyz, got_xyz, dist=3.0):
     from scipy.spatial import cKDTree
     d, _ = cKDTree(np.asarray(got_xyz, np.float64)).query(np.asarray(ref_xyz, np.float64))
     return float(np.mean(d <= dist))
+
+
+# ---------------------------------------------------------------- coarse-to-fine pruned search
+class PrunedSearch:
+    """Pruned search without scanning every orientation over the whole
+    tomogram: the exhaustive machinery scans `n_coarse` orientations spread
+    evenly over SO(3) (orient.orientation_bins), whose best-score map locates
+    the particles; local maxima of that map (greedy suppression within
+    `nms_radius`, the `K` best above `threshold`) are refined exactly in
+    `levels` rounds, each scoring a +-half grid of `step`-degree offsets on
+    every Euler angle around the best orientation so far, at the 27 voxels
+    around the best position so far. Scores are the exact FLCF of the search
+    (FP32 correlation, FP64 statistics). An optional `rank(volume, xyz)`
+    callable (e.g. the proposal network's particle logit at the candidates)
+    re-orders the local maxima before the K best are kept."""
+
+    def __init__(self, n_coarse=512, K=400, threshold=0.05, nms_radius=12.0, levels=((10.0, 10.0), (5.0, 5.0)),
+                 seed=0):
+        self.n_coarse, self.K, self.threshold, self.nms_radius = int(n_coarse), int(K), threshold, nms_radius
+        self.levels = tuple((float(h), float(s)) for h, s in levels)
+        self.orientations = orient.orientation_bins(self.n_coarse, seed).astype(np.float32)
+        self.timings = {}
+
+    def coarse(self, plan, tmpl, W, mu_t, sig_t):
+        """(score map, orientation index map) of the coarse scan."""
+        plan.set_template(tmpl, W, mu_t, sig_t)
+        plan.run(self.orientations, 0)
+        return plan.finalize()
+
+    def candidates(self, score, index, rank=None, volume=None):
+        """(P, 3) x y z and (P, 3) coarse Euler angles of the kept maxima."""
+        lm = api.local_maxima(score, 2, float(self.threshold))
+        if len(lm) == 0:
+            return np.zeros((0, 3), np.int64), np.zeros((0, 3), np.float32)
+        val = lm[:, 3] if rank is None else np.asarray(rank(volume, lm[:, :3].astype(np.int64)), np.float64)
+        keep = greedy_nms(lm[:, :3], val, self.nms_radius, self.K)
+        xyz = lm[keep, :3].astype(np.int64)
+        idx = _gather(index, xyz).astype(np.int64)
+        return xyz, self.orientations[idx]
+
+    def refine(self, plan, volume, xyz, eul):
+        """Levels of exact local refinement; returns (score[P], euler[P, 3],
+        pos[P, 3] x y z)."""
+        P = len(xyz)
+        if P == 0:
+            return np.zeros(0, np.float32), np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32)
+        pos = np.asarray(xyz, np.int32)
+        eul = np.asarray(eul, np.float32)
+        grp = np.arange(P, dtype=np.int32)[:, None]
+        sc = None
+        for half, step in self.levels:
+            grids = np.stack([orient.local_grid(e, half, step) for e in eul], 1)  # (A, P, 3)
+            sc, ai, pos = plan.refine(volume, grids, np.concatenate([pos, grp], 1))
+            eul = grids[ai, np.arange(P)]
+        return sc, eul, pos
+
+    def run(self, plan, volume, tmpl, W, mu_t, sig_t, rank=None, proposer=None):
+        """proposer (optional): a Surrogate whose network takes the coarse
+        score map as its second input channel; its candidates replace the
+        local maxima of the coarse map (orientations still come from the
+        coarse scan at each candidate)."""
+        t0 = time.perf_counter()
+        score, index = self.coarse(plan, tmpl, W, mu_t, sig_t)
+        _sync(score)
+        t1 = time.perf_counter()
+        if proposer is None:
+            xyz, eul = self.candidates(score, index, rank, volume)
+        else:
+            cand = proposer.propose(volume, K=self.K, nms_radius=self.nms_radius, aux=score)
+            xyz = cand[:, :3].astype(np.int64)
+            eul = self.orientations[_gather(index, xyz).astype(np.int64)]
+        t2 = time.perf_counter()
+        sc, eul, pos = self.refine(plan, volume, xyz, eul)
+        t3 = time.perf_counter()
+        self.timings = dict(coarse_s=t1 - t0, candidates_s=t2 - t1, refine_s=t3 - t2)
+        self.n_candidates = len(xyz)
+        return sc, eul, pos

@@ -48,7 +48,7 @@ def workload_template():
 
 
 def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, seed, thr, bins, sigma=2.0,
-                   dense=False, offset_radius=10.0):
+                   dense=False, offset_radius=10.0, aux_kind="sphere", n_coarse=256, targets="centroid"):
     from . import surrogate, synth
     rng = np.random.default_rng(seed)
     V, pos, eul, _ = synth.make_tomogram((size, size, size), [T], P, 40, snr, rng, seed, api, torch, dev, tilt_max=60.0)
@@ -56,13 +56,17 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
         plan.set_template(tn, W, mu, sg)
         plan.run(angles, 0)
         score, ang = plan.finalize()
-        aux = surrogate.aux_map(plan, T, M, W).cpu().numpy()
+        if aux_kind == "coarse":
+            aux = surrogate.PrunedSearch(n_coarse=n_coarse).coarse(plan, tn, W, mu, sg)[0].cpu().numpy()
+        else:
+            aux = surrogate.aux_map(plan, T, M, W).cpu().numpy()
     peaks = surrogate.exhaustive_peaks(score, ang, T.shape[0], thr)
     vz = V.cpu().numpy()
-    # targets sit at the density centres p + R d of the picks, which a
-    # translation-equivariant network can locate without knowing R
+    # centroid targets sit at the density centres p + R d of the picks, which
+    # a translation-equivariant network can locate without knowing R; peak
+    # targets at the picks themselves
     cp = peaks.copy()
-    if len(cp):
+    if len(cp) and targets == "centroid":
         cp[:, :3] = np.rint(surrogate.centroid_positions(peaks, angles, surrogate.template_centroid(T)))
         cp[:, :3] = np.clip(cp[:, :3], 0, np.asarray(vz.shape[::-1]) - 1)
     tprob, tbin, toff = surrogate.label_volumes(vz.shape, cp, angles, bins, sigma=sigma,
@@ -91,6 +95,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=100)
     ap.add_argument("--sigma", type=float, default=2.0, help="heat-map sigma around the exhaustive peaks (voxels)")
     ap.add_argument("--dense", action="store_true", help="add dense targets from the exhaustive score map")
+    ap.add_argument("--aux", choices=("sphere", "coarse"), default="sphere",
+                    help="second input channel: FLCF of the rotationally averaged template, or the best score of a "
+                         "coarse scan over --coarse orientations")
+    ap.add_argument("--coarse", type=int, default=256)
+    ap.add_argument("--targets", choices=("centroid", "peak"), default="centroid")
     ap.add_argument("--offset-radius", type=float, default=10.0,
                     help="voxels within this distance of a peak learn the offset to it")
     args = ap.parse_args(argv)
@@ -113,7 +122,8 @@ def main(argv=None):
         vz, tp, tb, peaks, pos, aux, to = label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, args.size,
                                                          args.particles, snr, args.seed + i, args.label_threshold,
                                                          bins, sigma=args.sigma, dense=args.dense,
-                                                         offset_radius=args.offset_radius)
+                                                         offset_radius=args.offset_radius, aux_kind=args.aux,
+                                                         n_coarse=args.coarse, targets=args.targets)
         tr.add(vz, tp, tb, aux, to)
         npk.append(len(peaks))
         log(f"[label] tomogram {i}: SNR {snr:.3f}, {len(peaks)} peaks > {args.label_threshold}, "
@@ -126,8 +136,8 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     meta = dict(vars(args), label_s=t_label, train_s=t_train, peaks_per_tomogram=npk,
                 final_loss=[float(np.mean([h[k] for h in tr.history[-100:]])) for k in range(3)])
-    np.savez_compressed(args.out, params=theta, bins=bins, centroid=surrogate.template_centroid(T),
-                        meta=json.dumps(meta))
+    extra = {"centroid": surrogate.template_centroid(T)} if args.targets == "centroid" else {}
+    np.savez_compressed(args.out, params=theta, bins=bins, meta=json.dumps(meta), **extra)
     log(f"[train] saved {args.out}: label {t_label:.1f} s, train {t_train:.1f} s")
     print(json.dumps(meta))
 
