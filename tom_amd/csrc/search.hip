@@ -1108,7 +1108,8 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index, const long
   timed(K_TFWD, [&] {
     dispatch_len(Y, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
-      if constexpr (L >= 1024) {
+      // the narrower tile keeps the block's LDS under 64 KB (two blocks per CU)
+      if constexpr (L >= 512) {
         hipLaunchKernelGGL((k_tmpl_fwd_xy<L, 8>), dim3(Xh / 8, nz, ka), dim3(NT), 0, s_, (const float*)t1_, Ty_,
                            Tyn_, tw_.f32(X), tw_.f32(L), X, nx, ny, nz);
       } else {
