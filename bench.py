@@ -127,16 +127,18 @@ def cpu_baseline(cfg, V_host, T, M, W, angles, k, workers):
 # coarse-to-fine pruned search per workload: coarse orientations, candidates
 # kept, refinement levels (+-half, step in degrees)
 PRUNED = {
-    "c1": dict(n_coarse=256, K=64, levels=((10.0, 10.0), (5.0, 5.0))),
-    "c2": dict(n_coarse=1024, K=400, levels=((10.0, 10.0), (5.0, 5.0))),
-    "c5": dict(n_coarse=1536, K=2400, levels=((15.0, 15.0), (7.5, 7.5), (3.75, 3.75))),
+    "c1": dict(n_coarse=256, K=64, levels=((10.0, 10.0), (5.0, 5.0)), bin=0),
+    "c2": dict(n_coarse=2048, K=400, levels=((15.0, 15.0), (5.0, 5.0)), bin=1),
+    "c5": dict(n_coarse=2048, K=2400, levels=((15.0, 15.0), (7.5, 7.5), (3.75, 3.75)), bin=1),
 }
 
 
 def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, plan_template, pre_s):
-    """Coarse-to-fine pruned search on the benchmark tomogram, judged by the
-    exhaustive run: recall of its picklist peaks above 0.3 (a refined hit
-    within 3 voxels with refined score > 0.3) and the wall-time ratio. When
+    """Coarse-to-fine pruned search on the benchmark tomogram (coarse scan of
+    the tom_bin-ned tomogram and template, exact refinement at full
+    resolution), judged by the exhaustive run: recall of its picklist peaks
+    above 0.3 (a refined hit within 3 voxels with refined score > 0.3) and the
+    wall-time ratio. When
     trained weights exist, the proposal network (second input channel: the
     coarse score map) is compared with the coarse map's own local maxima at
     the same coarse scan, K and refinement."""
@@ -158,8 +160,11 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
                 "timings_s": {k: round(v, 4) for k, v in ps.timings.items()}}
 
     par = PRUNED.get(args.config, PRUNED["c2"])
-    ps = S.PrunedSearch(n_coarse=args.coarse or par["n_coarse"], K=par["K"], levels=par["levels"])
+    ps = S.PrunedSearch(n_coarse=args.coarse or par["n_coarse"], K=par["K"], levels=par["levels"], bin=par["bin"])
+    if ps.bin:
+        ps.set_binned_template(T, M, W)
     out = timed(ps)
+    out["bin"] = ps.bin
     out.update(exhaustive_wall_s=round(exhaustive_s, 4), exhaustive_peaks_gt_0_3=int(len(ref)),
                precompute_s=round(pre_s, 4))
     log(f"[bench] pruned search: recall {out['recall_gt_0.3']:.4f} of {len(ref)} exhaustive peaks > 0.3, "

@@ -199,6 +199,12 @@ int tomhip_search_refine(tomhip_search* h, const float* volume, const float* eul
  * of the xcf map with the template's autocorrelation. */
 int tomhip_psr_f32(tomhip_ctx* ctx, const float* xcf, const int64_t N[3], const int64_t n[3], float* out);
 
+/* tom_bin(im, nbin) of a 3D volume (Sptrans/tom_bin.m:38-51): nbin times,
+ * each output voxel = the single-precision sum of its 2x2x2 block in
+ * tom_bin's term order, divided by 8; odd trailing samples are dropped.
+ * out: (N >> nbin) per axis. */
+int tomhip_bin_f32(tomhip_ctx* ctx, const float* in, const int64_t N[3], int nbin, float* out);
+
 /* ---------------------------------------------------------------- learned proposal */
 /* No counterpart in the reference: a 3-level 3D U-Net (16/32/64 channels, two
  * 3x3x3 convolutions + ReLU per level, 2x2x2 average pool, trilinear x2

@@ -50,6 +50,9 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
                         Analysis/tom_peakc.m:39-86 -> separated planted peaks come
                         back in descending order with their erase spheres
                         (test_picklist_planted_peaks)
+  tom_bin               Sptrans/tom_bin.m:38-51 -> constant and linear-ramp
+                        volumes (closed forms), odd-edge cropping, one 2x2x2
+                        block summed by hand (test_tom_bin_closed_forms)
 
 Deliberate deviations shared with the product (documented, not bugs):
   * Local variance is formed from the globally mean-shifted volume and shifted
@@ -558,6 +561,24 @@ def psr(xcf: np.ndarray, n: int) -> np.ndarray:
     nz = sig > 0
     out[nz] = (x[nz] - mu[nz]) / sig[nz]
     return out
+
+
+def tom_bin(im: np.ndarray, nbin: int = 1) -> np.ndarray:
+    """tom_bin for a 3D volume (Sptrans/tom_bin.m:38-51): nbin rounds of the
+    single-precision sum of the eight 2x2x2 samples in the order the .m file
+    writes them ((1,1,1) (1,2,1) (2,1,1) (2,2,1), then the same at z + 1,
+    MATLAB (x, y, z) indices), divided by 8; odd trailing samples dropped."""
+    a = np.asarray(im, np.float32)
+    for _ in range(int(nbin)):
+        X, Y, Z = (d // 2 * 2 for d in a.shape)
+        a = a[:X, :Y, :Z]
+        terms = [a[0::2, 0::2, 0::2], a[0::2, 1::2, 0::2], a[1::2, 0::2, 0::2], a[1::2, 1::2, 0::2],
+                 a[0::2, 0::2, 1::2], a[0::2, 1::2, 1::2], a[1::2, 0::2, 1::2], a[1::2, 1::2, 1::2]]
+        acc = terms[0].copy()
+        for t in terms[1:]:
+            acc = (acc + t).astype(np.float32)
+        a = (acc / np.float32(8)).astype(np.float32)
+    return a
 
 
 # ----------------------------------------------------------------------------

@@ -335,6 +335,20 @@ def test_refine_centroid_windows(api, oracle):
             assert an[i] == ref[1] and tuple(pos[i]) == ref[2]
 
 
+def test_tom_bin_bit_exact(api, oracle):
+    """tomhip_bin_f32 equals the tom_bin restatement bit for bit (odd edges,
+    one and two rounds, numpy and device tensors)."""
+    say("tom_bin")
+    import torch
+    rng = np.random.default_rng(23)
+    V = rng.standard_normal((37, 22, 19)).astype(np.float32) * 100
+    for nb in (1, 2):
+        np.testing.assert_array_equal(api.tom_bin(V, nb), oracle.tom_bin(V, nb))
+    Vt = torch.from_numpy(np.ascontiguousarray(V.transpose(2, 1, 0))).cuda()
+    got = api.tom_bin(Vt, 1).cpu().numpy().transpose(2, 1, 0)
+    np.testing.assert_array_equal(got, oracle.tom_bin(V, 1))
+
+
 def test_autotune_picks_a_batch_with_identical_results(api, oracle):
     """The autotuner times candidate batches and checks that every batch gives
     the same maps before it reports a choice."""

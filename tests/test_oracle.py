@@ -343,3 +343,25 @@ def test_picklist_planted_peaks(oracle):
     # the first pick erased its sphere (radius 3): a neighbour at distance 2 is 0
     _, _, after = oracle.peakc(oracle.erase_borders(a, (n, n, n)).astype(np.float32), 3)
     assert after[28, 12, 22] == 0 and after[28, 12, 20] == 0 and after[28, 12, 24] != 0
+
+
+def test_tom_bin_closed_forms(oracle):
+    """tom_bin restatement: a constant stays constant, a linear ramp
+    a + bx + cy + dz bins to its value at the block centres, odd trailing
+    samples are dropped, two rounds halve twice, and one block equals its
+    eight samples summed by hand in the .m file's order."""
+    c = np.full((6, 4, 8), 3.25, np.float32)
+    np.testing.assert_array_equal(oracle.tom_bin(c), np.full((3, 2, 4), 3.25, np.float32))
+    x, y, z = np.meshgrid(np.arange(8.0), np.arange(6.0), np.arange(4.0), indexing="ij")
+    ramp = (1.0 + 0.5 * x + 0.25 * y + 2.0 * z).astype(np.float32)
+    xb, yb, zb = np.meshgrid(np.arange(4) * 2 + 0.5, np.arange(3) * 2 + 0.5, np.arange(2) * 2 + 0.5, indexing="ij")
+    np.testing.assert_array_equal(oracle.tom_bin(ramp), (1.0 + 0.5 * xb + 0.25 * yb + 2.0 * zb).astype(np.float32))
+    odd = np.random.default_rng(3).standard_normal((7, 5, 9)).astype(np.float32)
+    assert oracle.tom_bin(odd).shape == (3, 2, 4)
+    np.testing.assert_array_equal(oracle.tom_bin(odd), oracle.tom_bin(odd[:6, :4, :8]))
+    assert oracle.tom_bin(np.ones((16, 8, 12), np.float32), 2).shape == (4, 2, 3)
+    v = odd
+    acc = np.float32(v[0, 0, 0])
+    for (i, j, k) in ((0, 1, 0), (1, 0, 0), (1, 1, 0), (0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 1)):
+        acc = np.float32(acc + v[i, j, k])
+    assert oracle.tom_bin(v)[0, 0, 0] == np.float32(acc / np.float32(8))

@@ -147,22 +147,24 @@ def test_snap_maxima_brute_force(api):
 
 @pytest.mark.gpu
 def test_pruned_search_finds_exhaustive_peaks_with_exact_scores(api, oracle):
-    """Coarse-to-fine search (256 coarse orientations, two refinement levels)
-    finds every exhaustive peak above 0.3 of a seeded volume with three
-    particles planted at orientations of the exhaustive 30-degree grid, and
-    each reported (score, orientation, position) equals the FP64 oracle's FLCF
-    of that orientation at that voxel within 1e-4."""
+    """Coarse-to-fine search (1,024 coarse orientations, three refinement
+    levels; coarse scan at full resolution and on tom_bin-ned volume and
+    template) finds every exhaustive peak above 0.3 of a seeded volume with
+    three 24^3 particles planted at orientations of the exhaustive 30-degree
+    grid, and each reported (score, orientation, position) of the full-resolution
+    run equals the FP64 oracle's FLCF of that orientation at that voxel
+    within 1e-4."""
     print("pruned search", flush=True)
     from tom_amd import synth
     rng = np.random.default_rng(41)
-    N = (96, 96, 64)
-    n = 16
+    N = (128, 96, 64)
+    n = 24
     V = rng.standard_normal(N).astype(np.float32)
-    T = synth.phantom(rng, n, 8, 1.2, 6)
-    for corner, e in (((10, 12, 8), (30, 60, 90)), ((60, 50, 30), (210, 120, 120)), ((40, 70, 40), (90, 300, 30))):
+    T = synth.phantom(np.random.default_rng(42), n, 10, 1.5, 8)
+    for corner, e in (((10, 12, 8), (30, 60, 90)), ((60, 50, 28), (210, 120, 120)), ((40, 66, 36), (90, 300, 30))):
         x, y, z = corner
-        V[x:x + n, y:y + n, z:z + n] += 2.0 * oracle.rotate(T, e)
-    M = oracle.sphere_mask_binary(n, 7)
+        V[x:x + n, y:y + n, z:z + n] += 3.0 * oracle.rotate(T, e)
+    M = oracle.sphere_mask_binary(n, 10)
     tn, mu, sg = api.prepare_template(T, M)
     angles = oracle.angle_grid(30, 30, 30).astype(np.float32)
     with api.SearchPlan(V, T.shape, M, batch=4) as plan:
@@ -170,11 +172,22 @@ def test_pruned_search_finds_exhaustive_peaks_with_exact_scores(api, oracle):
         plan.run(angles, 0)
         score, ang = plan.finalize()
         ref = S.exhaustive_peaks(score, ang, n, 0.3)
-        ps = S.PrunedSearch(n_coarse=512, K=30, threshold=0.05, nms_radius=6.0, levels=((30.0, 15.0), (10.0, 5.0)))
+        levels = ((30.0, 15.0), (15.0, 7.5), (7.5, 5.0))
+        ps = S.PrunedSearch(n_coarse=1024, K=30, threshold=0.05, nms_radius=6.0, levels=levels)
         sc, eul, pos = ps.run(plan, V, tn, None, mu, sg)
+        # the same with the coarse scan on tom_bin(volume) and tom_bin(template)
+        pb = S.PrunedSearch(n_coarse=1024, K=30, threshold=0.05, nms_radius=6.0, levels=levels, bin=1)
+        pb.set_binned_template(T, M)
+        scb, _, posb = pb.run(plan, V, tn, None, mu, sg)
     print(f"  {len(ref)} exhaustive peaks > 0.3, {ps.n_candidates} candidates, timings {ps.timings}", flush=True)
+    for name, a, b in (("full", sc, pos), ("binned", scb, posb)):
+        for r in ref:
+            d = np.linalg.norm(b - r[:3], axis=1)
+            j = int(np.argmin(d))
+            print(f"  {name}: exhaustive {r[:4]} nearest refined {b[j]} score {a[j]:.4f}", flush=True)
     assert len(ref) >= 3
     assert S.recall(ref[:, :3], pos[sc > 0.3]) == 1.0
+    assert S.recall(ref[:, :3], posb[scb > 0.3]) == 1.0
     Vd = V.astype(np.float64)
     Md = M.astype(np.float64)
     m = float((Md > 0).sum())

@@ -49,6 +49,7 @@ SIGNATURES = {
                                              vp]),
     "tomhip_local_maxima_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int64, vp, vp]),
     "tomhip_psr_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "tomhip_bin_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp]),
     "tomhip_search_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
     "tomhip_search_set_zchunk": (ctypes.c_int, [vp, ctypes.c_int]),
     "tomhip_snap_maxima_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int]),

@@ -702,6 +702,18 @@ def tom_os3_psr(xcf, template_size, ctx: Optional[Context] = None):
     return out
 
 
+def tom_bin(im, nbin: int = 1, ctx: Optional[Context] = None):
+    """tom_bin(im, nbin) of a 3D volume: nbin rounds of 2x2x2 averaging
+    (odd trailing samples dropped), shaped like the input's kind."""
+    ib = _Buf(im)
+    ctx = _prep_ctx(ctx, ib)
+    N = _dims3(ib.dims)
+    out_dims = tuple(int(d) >> int(nbin) for d in N)
+    out = _empty_like(ib, out_dims)
+    ctx.check(ctx.lib.tomhip_bin_f32(ctx.h, ib.ptr, _p(N), int(nbin), _ptr(out)))
+    return out
+
+
 def snap_maxima(peaks, cand_xyz, radius: int, ctx: Optional[Context] = None):
     """Move each (x, y, z) row to the largest map value within the cube of
     half-edge radius around it; returns the new (P, 3) int32 rows."""

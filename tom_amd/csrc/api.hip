@@ -670,6 +670,53 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
   });
 }
 
+int tomhip_bin_f32(tomhip_ctx* ctx, const float* in, const int64_t N[3], int nbin, float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && in && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(N, "volume");
+    TOMHIP_REQUIRE(N[0] > 1 && N[1] > 1 && N[2] > 1, ERR_UNSUPPORTED, "tom_bin here takes 3D volumes (edges > 1)");
+    TOMHIP_REQUIRE(nbin >= 0 && nbin <= 8, ERR_WRONG_INPUT, "nbin must be in [0, 8]");
+    long X = N[0], Y = N[1], Z = N[2];
+    for (int i = 0; i < nbin; ++i) {
+      TOMHIP_REQUIRE(X / 2 >= 1 && Y / 2 >= 1 && Z / 2 >= 1, ERR_WRONG_INPUT, "volume too small for nbin");
+      X /= 2, Y /= 2, Z /= 2;
+    }
+    hipStream_t s = ctx->stream;
+    const long NN = N[0] * N[1] * N[2], NO = X * Y * Z;
+    DevBuf v(s, in, sizeof(float) * NN, true, false);
+    DevBuf o(s, out, sizeof(float) * NO, false, true);
+    if (nbin == 0) {
+      TOMHIP_CHECK(hipMemcpyAsync(o.get<float>(), v.get<float>(), sizeof(float) * NN, hipMemcpyDeviceToDevice, s));
+    } else {
+      // ping-pong buffers sized by the first two outputs (later ones shrink)
+      float* tmp[2] = {nullptr, nullptr};
+      auto release = [&] {
+        (void)hipFree(tmp[0]);
+        (void)hipFree(tmp[1]);
+      };
+      try {
+        long x = N[0], y = N[1], z = N[2];
+        if (nbin > 1) TOMHIP_CHECK(hipMalloc(&tmp[0], sizeof(float) * (x / 2) * (y / 2) * (z / 2)));
+        if (nbin > 2) TOMHIP_CHECK(hipMalloc(&tmp[1], sizeof(float) * (x / 4) * (y / 4) * (z / 4)));
+        const float* src = v.get<float>();
+        for (int i = 0; i < nbin; ++i) {
+          float* dst = (i == nbin - 1) ? o.get<float>() : tmp[i & 1];
+          launch_bin2(src, (int)x, (int)y, (int)z, dst, s);
+          src = dst;
+          x /= 2, y /= 2, z /= 2;
+        }
+        TOMHIP_CHECK(hipStreamSynchronize(s));
+      } catch (...) {
+        release();
+        throw;
+      }
+      release();
+    }
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int tomhip_psr_f32(tomhip_ctx* ctx, const float* xcf, const int64_t N[3], const int64_t n[3], float* out) {
   return guard(ctx, [&] {
     TOMHIP_REQUIRE(ctx && xcf && out, ERR_WRONG_INPUT, "null pointer");

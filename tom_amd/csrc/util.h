@@ -22,4 +22,7 @@ void global_max(const float* a, long n, float* val, long* idx, hipStream_t s);
 void launch_psr_ring(int nx, int ny, int nz, float* out, hipStream_t s);
 // out = (x - mu) / sig where sig > 0, else 0
 void launch_psr(const float* x, const double* mu, const double* sig, float* out, long n, hipStream_t s);
+// One tom_bin step of a 3D volume (edges > 1): out (X/2, Y/2, Z/2) = the
+// FP32 sum of each 2x2x2 block in tom_bin's term order, divided by 8.
+void launch_bin2(const float* in, int X, int Y, int Z, float* out, hipStream_t s);
 }  // namespace tomhip

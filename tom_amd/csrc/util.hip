@@ -344,4 +344,37 @@ void launch_psr(const float* x, const double* mu, const double* sig, float* out,
   TOMHIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- tom_bin
+// Sptrans/tom_bin.m:47-51: (im(1:2:,1:2:,1:2:) + im(1:2:,2:2:,1:2:) +
+// im(2:2:,1:2:,1:2:) + im(2:2:,2:2:,1:2:) + [the same four at z + 1]) / 8,
+// summed left to right in single precision; odd trailing samples dropped.
+__global__ void k_bin2(const float* __restrict__ in, int X, int Y, float* __restrict__ out, int Xo, int Yo,
+                       long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const int x = (int)(i % Xo);
+    const long r = i / Xo;
+    const int y = (int)(r % Yo), z = (int)(r / Yo);
+    const long b0 = ((long)(2 * z) * Y + 2 * y) * X + 2 * x, b1 = b0 + (long)X * Y;
+    float acc = in[b0];
+    acc = acc + in[b0 + X];          // (x0, y1, z0)
+    acc = acc + in[b0 + 1];          // (x1, y0, z0)
+    acc = acc + in[b0 + X + 1];      // (x1, y1, z0)
+    acc = acc + in[b1];              // (x0, y0, z1)
+    acc = acc + in[b1 + X];          // (x0, y1, z1)
+    acc = acc + in[b1 + 1];          // (x1, y0, z1)
+    acc = acc + in[b1 + X + 1];      // (x1, y1, z1)
+    out[i] = acc / 8.f;
+  }
+}
+
+void launch_bin2(const float* in, int X, int Y, int Z, float* out, hipStream_t s) {
+  const int Xo = X / 2, Yo = Y / 2, Zo = Z / 2;
+  const long n = (long)Xo * Yo * Zo;
+  if (n == 0) return;
+  long b = (n + NT - 1) / NT;
+  if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(k_bin2, dim3((unsigned)b), dim3(NT), 0, s, in, X, Y, out, Xo, Yo, n);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace tomhip

@@ -121,6 +121,21 @@ def orientation_bins(n: int = 64, seed: int = 0, steps: int = 400) -> np.ndarray
     return euler_from_matrix(matrix_from_quat(q))
 
 
+def super_fibonacci(n: int) -> np.ndarray:
+    """(n, 3) [phi psi theta] of n orientations from the super-Fibonacci
+    spiral (M. Alexa, "Super-Fibonacci Spirals: Fast, Low-Discrepancy
+    Sampling of SO(3)", CVPR 2022): quaternion i = (r sin a, r cos a,
+    R sin b, R cos b) with s = i + 1/2, r = sqrt(s/n), R = sqrt(1 - s/n),
+    a = 2 pi s / sqrt(2), b = 2 pi s / psi, psi^4 = psi + 4. O(n), so
+    thousands of near-uniform orientations come without an iterative spread."""
+    psi = 1.533751168755204288118041
+    s = np.arange(int(n), dtype=np.float64) + 0.5
+    r, R = np.sqrt(s / n), np.sqrt(1.0 - s / n)
+    a, b = 2.0 * np.pi * s / np.sqrt(2.0), 2.0 * np.pi * s / psi
+    q = np.stack([r * np.sin(a), r * np.cos(a), R * np.sin(b), R * np.cos(b)], 1)
+    return euler_from_matrix(matrix_from_quat(q)).astype(np.float64)
+
+
 def assign_bins(euler, bins) -> np.ndarray:
     """Index of the nearest bin (largest |q . q_bin|) for each [phi psi theta]."""
     qa = quat_from_matrix(rotmatrix(np.asarray(euler, np.float64).reshape(-1, 3)))

@@ -368,7 +368,7 @@ def recall(ref_xyz, got_xyz, dist=3.0):
 class PrunedSearch:
     """Pruned search without scanning every orientation over the whole
     tomogram: the exhaustive machinery scans `n_coarse` orientations spread
-    evenly over SO(3) (orient.orientation_bins), whose best-score map locates
+    evenly over SO(3) (orient.super_fibonacci), whose best-score map locates
     the particles; local maxima of that map (greedy suppression within
     `nms_radius`, the `K` best above `threshold`) are refined exactly in
     `levels` rounds, each scoring a +-half grid of `step`-degree offsets on
@@ -379,11 +379,49 @@ class PrunedSearch:
     re-orders the local maxima before the K best are kept."""
 
     def __init__(self, n_coarse=512, K=400, threshold=0.05, nms_radius=12.0, levels=((10.0, 10.0), (5.0, 5.0)),
-                 seed=0):
+                 bin=0):
         self.n_coarse, self.K, self.threshold, self.nms_radius = int(n_coarse), int(K), threshold, nms_radius
         self.levels = tuple((float(h), float(s)) for h, s in levels)
-        self.orientations = orient.orientation_bins(self.n_coarse, seed).astype(np.float32)
+        self.orientations = orient.super_fibonacci(self.n_coarse).astype(np.float32)
+        self.bin = int(bin)
+        self._binned = None
         self.timings = {}
+
+    def set_binned_template(self, T, M, W=None):
+        """Template for a coarse scan on tom_bin(volume, bin): tom_bin of the
+        template and of the mask (a binary mask stays binary: binned values
+        above 0.5), and the central (n >> bin)^3 of the centred weight W,
+        i.e. the same frequencies on the binned grid."""
+        b = self.bin
+        Tb = np.asarray(api.tom_bin(np.asarray(T, np.float32), b), np.float32)
+        M = np.asarray(M, np.float32)
+        Mb = np.asarray(api.tom_bin(M, b), np.float32)
+        if np.all((M == 0) | (M == 1)):
+            Mb = (Mb > 0.5).astype(np.float32)
+        Wb = None
+        if W is not None:
+            Wn = np.asarray(W, np.float32)
+            lo = [d // 2 - (d >> b) // 2 for d in Wn.shape]
+            Wb = np.ascontiguousarray(Wn[lo[0]:lo[0] + (Wn.shape[0] >> b), lo[1]:lo[1] + (Wn.shape[1] >> b),
+                                         lo[2]:lo[2] + (Wn.shape[2] >> b)])
+        tn, mu, sg = api.prepare_template(Tb, Mb, Wb)
+        self._binned = (tn, Mb, Wb, mu, sg)
+
+    def coarse_binned(self, volume):
+        """(score, orientation index) maps of the coarse scan of tom_bin(volume,
+        bin), in binned voxels; includes the binned plan's precompute."""
+        if self._binned is None:
+            raise RuntimeError("set_binned_template first")
+        tn, Mb, Wb, mu, sg = self._binned
+        Vb = api.tom_bin(volume, self.bin)
+        with api.SearchPlan(Vb, Mb.shape, Mb, batch=16) as pb:
+            try:
+                pb.set_tiles(16, 32)
+            except api.TomHipError:
+                pass
+            pb.set_template(tn, Wb, mu, sg)
+            pb.run(self.orientations, 0)
+            return pb.finalize()
 
     def coarse(self, plan, tmpl, W, mu_t, sig_t):
         """(score map, orientation index map) of the coarse scan."""
@@ -397,7 +435,7 @@ class PrunedSearch:
         if len(lm) == 0:
             return np.zeros((0, 3), np.int64), np.zeros((0, 3), np.float32)
         val = lm[:, 3] if rank is None else np.asarray(rank(volume, lm[:, :3].astype(np.int64)), np.float64)
-        keep = greedy_nms(lm[:, :3], val, self.nms_radius, self.K)
+        keep = greedy_nms(lm[:, :3], val, self.nms_radius / (1 << self.bin), self.K)
         xyz = lm[keep, :3].astype(np.int64)
         idx = _gather(index, xyz).astype(np.int64)
         return xyz, self.orientations[idx]
@@ -424,10 +462,20 @@ class PrunedSearch:
         local maxima of the coarse map (orientations still come from the
         coarse scan at each candidate)."""
         t0 = time.perf_counter()
-        score, index = self.coarse(plan, tmpl, W, mu_t, sig_t)
+        if self.bin:
+            score, index = self.coarse_binned(volume)
+            plan.set_template(tmpl, W, mu_t, sig_t)  # the refinement scores at full resolution
+        else:
+            score, index = self.coarse(plan, tmpl, W, mu_t, sig_t)
         _sync(score)
         t1 = time.perf_counter()
-        if proposer is None:
+        if self.bin:
+            # binned peak i <-> template centre at i * 2^bin (the binned voxel
+            # i starts at full-resolution voxel i * 2^bin)
+            xyz, eul = self.candidates(score, index, rank, volume)
+            dims = np.asarray(tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume))
+            xyz = np.minimum(xyz << self.bin, dims - 1)
+        elif proposer is None:
             xyz, eul = self.candidates(score, index, rank, volume)
         else:
             cand = proposer.propose(volume, K=self.K, nms_radius=self.nms_radius, aux=score)
