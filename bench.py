@@ -310,8 +310,8 @@ def main():
     ap.add_argument("--tiles", default="16,32", help="column tiles of the z- and y-FFT passes (8/16/32)")
     ap.add_argument("--zchunk", type=int, default=0, help="z-chunk planes of the y-FFT and c2r passes (0 = whole)")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
-    ap.add_argument("--cpu-particles", type=int, default=6, help="c3: particles in the oracle sample")
-    ap.add_argument("--cpu-angles", type=int, default=4, help="orientations in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-particles", type=int, default=24, help="c3: particles in the oracle sample")
+    ap.add_argument("--cpu-angles", type=int, default=12, help="orientations in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--no-profile", action="store_true", help="do not record per-kernel events")
     ap.add_argument("--calib-copy", action="store_true", help="run one 1 GiB device copy (PMC calibration)")

@@ -191,6 +191,41 @@ __global__ __launch_bounds__(NT) void k_box_x_inv(const c32* in, float* t, const
   }
 }
 
+// Both in-plane transforms of one xy plane per block (square planes up to
+// 64 x 64): the plane sits in LDS with row pitch L + 1, x lines are rows,
+// y lines are columns. Same butterflies as the separate x and y passes, one
+// HBM round trip instead of two.
+template <int L>
+__global__ __launch_bounds__(NT) void k_box_xy_fwd(const float* t, c32* out, const c32* twg) {
+  constexpr int P = L + 1;
+  __shared__ c32 buf[L * P];
+  __shared__ c32 tw[L];
+  load_table<float, L, NT>(tw, twg);
+  const long base = (long)blockIdx.x * L * L;  // plane index over templates x planes
+  for (int i = threadIdx.x; i < L * L; i += NT) buf[(i / L) * P + i % L] = mk(t[base + i], 0.f);
+  __syncthreads();
+  lds_fft<float, L, L, NT, -1, Contig<P>>(buf, tw);
+  lds_fft<float, L, L, NT, -1, Inter<P>>(buf, tw);
+  for (int i = threadIdx.x; i < L * L; i += NT) out[base + i] = buf[(i / L) * P + i % L];
+}
+
+// inverse of the above, real part / n3, times mask -> float
+template <int L>
+__global__ __launch_bounds__(NT) void k_box_xy_inv(const c32* in, float* t, const float* M, int nz, float inv_n3,
+                                                   const c32* twg) {
+  constexpr int P = L + 1;
+  __shared__ c32 buf[L * P];
+  __shared__ c32 tw[L];
+  load_table<float, L, NT>(tw, twg);
+  const long base = (long)blockIdx.x * L * L;
+  const float* m = M + (long)(blockIdx.x % nz) * L * L;
+  for (int i = threadIdx.x; i < L * L; i += NT) buf[(i / L) * P + i % L] = in[base + i];
+  __syncthreads();
+  lds_fft<float, L, L, NT, 1, Inter<P>>(buf, tw);
+  lds_fft<float, L, L, NT, 1, Contig<P>>(buf, tw);
+  for (int i = threadIdx.x; i < L * L; i += NT) t[base + i] = buf[(i / L) * P + i % L].x * inv_n3 * m[i];
+}
+
 // ---------------------------------------------------------------- template fwd
 // grid (Xh/TW, nz, ka). Output Ty[a][uz][ky][kx] for kx < Xh, Tyn[a][uz][ky]
 // for kx = Xh (tile 0 only).
@@ -1001,21 +1036,32 @@ void Search::box_filter(int count, float* t, c32* work) {
   // followed by the masking inside tom_os3_corr)
   const long n3 = (long)nx * ny * nz;
   const long lpv = (long)ny * nz;  // x lines per template
-  dispatch_len(nx, [&](auto Lc) {
-    constexpr int L = decltype(Lc)::value;
-    constexpr int NL = contig_lines<float, L>();
-    hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)(count * ((lpv + NL - 1) / NL))), dim3(NT), 0, s_,
-                       (const float*)t, work, lpv, tw_.f32(L));
-  });
-  TOMHIP_CHECK(hipGetLastError());
   const long XY = (long)nx * ny;
-  dispatch_len(ny, [&](auto Lc) {
-    constexpr int L = decltype(Lc)::value;
-    constexpr int TW = strided_tw<float, L>();
-    hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, nz, count), dim3(NT), 0, s_, work, nx,
-                       (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
-  });
-  TOMHIP_CHECK(hipGetLastError());
+  const bool plane = (nx == ny && nx <= 64);
+  if (plane) {
+    dispatch_len(nx, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if constexpr (L <= 64)
+        hipLaunchKernelGGL(k_box_xy_fwd<L>, dim3((unsigned)(count * nz)), dim3(NT), 0, s_, (const float*)t, work,
+                           tw_.f32(L));
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  } else {
+    dispatch_len(nx, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      constexpr int NL = contig_lines<float, L>();
+      hipLaunchKernelGGL(k_box_x_fwd<L>, dim3((unsigned)(count * ((lpv + NL - 1) / NL))), dim3(NT), 0, s_,
+                         (const float*)t, work, lpv, tw_.f32(L));
+    });
+    TOMHIP_CHECK(hipGetLastError());
+    dispatch_len(ny, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      constexpr int TW = strided_tw<float, L>();
+      hipLaunchKernelGGL((k_c2c_strided<float, L, -1>), dim3((nx + TW - 1) / TW, nz, count), dim3(NT), 0, s_, work,
+                         nx, (long)nx, XY, n3, tw_.f32(L), (const float*)nullptr);
+    });
+    TOMHIP_CHECK(hipGetLastError());
+  }
   dispatch_len(nz, [&](auto Lc) {
     constexpr int L = decltype(Lc)::value;
     constexpr int TW = strided_tw<float, L>();
@@ -1023,6 +1069,16 @@ void Search::box_filter(int count, float* t, c32* work) {
                        XY, (long)nx, n3, tw_.f32(L), (const float*)wsh_);
   });
   TOMHIP_CHECK(hipGetLastError());
+  if (plane) {
+    dispatch_len(nx, [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if constexpr (L <= 64)
+        hipLaunchKernelGGL(k_box_xy_inv<L>, dim3((unsigned)(count * nz)), dim3(NT), 0, s_, (const c32*)work, t,
+                           mask_, nz, (float)(1.0 / (double)n3), tw_.f32(L));
+    });
+    TOMHIP_CHECK(hipGetLastError());
+    return;
+  }
   dispatch_len(ny, [&](auto Lc) {
     constexpr int L = decltype(Lc)::value;
     constexpr int TW = strided_tw<float, L>();
