@@ -227,4 +227,7 @@ def test_unet_adam_descends_and_infer_tiles(api):
     np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-6)
     assert (b1 == b0).mean() > 0.999
     np.testing.assert_array_equal(o0, lg[..., 1 + NBIN:4 + NBIN].transpose(3, 2, 1, 0))
+    # bins: first index of the largest bin logit; prob: sigmoid of channel 0
+    np.testing.assert_array_equal(b0, np.argmax(lg[..., 1:1 + NBIN], -1).transpose(2, 1, 0))
+    np.testing.assert_allclose(p0, (1 / (1 + np.exp(-lg[..., 0].astype(np.float64)))).transpose(2, 1, 0), atol=1e-6)
     np.testing.assert_allclose(o1, o0, rtol=0, atol=1e-5)

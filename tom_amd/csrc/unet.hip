@@ -14,6 +14,7 @@
 //               = the forward with W'[26-t][ci][co] = W[t][co][ci]
 //   grad weight dW[t][co][ci] = sum_v dY[v][co] X[v + d_t][ci]  (K = voxels)
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -494,20 +495,51 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, long n, flo
 }
 
 // prob = sigmoid(particle logit), or the logit itself (no saturation at 1);
-// offs (optional) = the three offset channels
-__global__ void k_predict(const float* logits, int CH, int nbin, long V, float* prob, int32_t* bin, float* offs,
-                          int raw) {
-  for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
-    const float* L = logits + v * CH;
-    if (prob) prob[v] = raw ? L[0] : 1.f / (1.f + expf(-L[0]));
-    if (bin) {
-      int b = 0;
-      for (int c = 1; c < nbin; ++c)
-        if (L[1 + c] > L[1 + b]) b = c;
-      bin[v] = b;
+// bin = first index of the largest bin logit; offs (optional) = the three
+// offset channels. The block stages 64 voxels' logits through LDS with
+// coalesced loads; four lanes scan a quarter of the bins each and combine
+// (larger value, then lower index: the first maximum, as a sequential scan).
+constexpr int PRED_VB = NTU / 4;
+__global__ __launch_bounds__(NTU) void k_predict(const float* logits, int CH, int nbin, long V, float* prob,
+                                                 int32_t* bin, float* offs, int raw) {
+  __shared__ float tile[PRED_VB * 81];
+  const int P = CH + 1;
+  const int v = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int per = (nbin + 3) / 4;
+  for (long v0 = (long)blockIdx.x * PRED_VB; v0 < V; v0 += (long)gridDim.x * PRED_VB) {
+    const int nv = (int)((V - v0 < PRED_VB) ? (V - v0) : PRED_VB);
+    const float* src = logits + v0 * CH;
+    for (int i = threadIdx.x; i < nv * CH; i += NTU) tile[(i / CH) * P + i % CH] = src[i];
+    __syncthreads();
+    float best = -FLT_MAX;
+    int bi = INT_MAX;
+    if (v < nv) {
+      const int c1 = min(nbin, (q + 1) * per);
+      for (int c = q * per; c < c1; ++c) {
+        const float x = tile[v * P + 1 + c];
+        if (bi == INT_MAX || x > best) {
+          best = x;
+          bi = c;
+        }
+      }
     }
-    if (offs)
-      for (int c = 0; c < 3; ++c) offs[v * 3 + c] = L[1 + nbin + c];
+    for (int o = 1; o < 4; o <<= 1) {
+      const float ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (q == 0 && v < nv) {
+      const long g = v0 + v;
+      const float z = tile[v * P];
+      if (prob) prob[g] = raw ? z : 1.f / (1.f + expf(-z));
+      if (bin) bin[g] = (bi == INT_MAX) ? 0 : bi;
+      if (offs)
+        for (int c = 0; c < 3; ++c) offs[g * 3 + c] = tile[v * P + 1 + nbin + c];
+    }
+    __syncthreads();
   }
 }
 
