@@ -404,6 +404,7 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
   using Lay = Contig<XH>;
   using IN = StageInfo<XH, S - 1, NL, NT_C, Lay>;
   __shared__ c32 buf[NL * XH];
+  __shared__ __align__(16) c32 raw[NL * XH];
   __shared__ c32 twH[XH];
   __shared__ c32 twR[XH];
   load_table<float, XH, NT_C>(twH, twHg);
@@ -414,15 +415,38 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
   static_assert(IN::R <= 8, "batch indices are packed 4 bits per output");
   float bx[IN::PER][IN::R], by[IN::PER][IN::R];
   uint32_t ix[IN::PER], iy[IN::PER];  // 4-bit batch index per output r
+  // The block's lines of one angle are contiguous in Q: all threads load them
+  // as 16-byte vectors (the next angle's while this one runs its LDS stages)
+  // and park them in `raw`, from where stage 0 reads each butterfly and its
+  // mirror.
+  constexpr int NV = NL * XH / 2;                 // float4 per angle and block
+  constexpr int PF = (NV + NT_C - 1) / NT_C;      // per thread
+  const int nvv = nlv * XH / 2;
+  float4 pf[PF];
+  auto fetch = [&](int a) {
+    const float4* src4 = reinterpret_cast<const float4*>(Q + (long)a * qstride + l0 * XH);
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int i = threadIdx.x + k * NT_C;
+      pf[k] = (i < nvv) ? src4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  fetch(0);
   for (int a = 0; a < ka; ++a) {
-    const c32* src = Q + (long)a * qstride + l0 * XH;
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int i = threadIdx.x + k * NT_C;
+      if (i < NV) reinterpret_cast<float4*>(raw)[i] = pf[k];
+    }
+    if (a + 1 < ka) fetch(a + 1);
+    __syncthreads();
 #pragma unroll 1
     for (int p = 0; p < PERT; ++p) {
       const int t = threadIdx.x + p * NT_C;
       if ((TT % NT_C == 0) || t < TT) {
         const int line = t / NTASK, j = t % NTASK;
         const bool ok = line < nlv;
-        const c32* g = src + (long)line * XH;
+        const c32* g = raw + line * XH;
         c32* row = buf + line * XH;
         c32 va[R0], vb[R0], z1[R0], z2[R0];
 #pragma unroll
