@@ -325,22 +325,61 @@ __global__ __launch_bounds__(NT_A) void k_passA(const c32* Ty, const c32* Tyn, c
   __syncthreads();
   const int hz = nz / 2;
   const int sZ = 2 * (Z / 2) - Z;
-  for (int a = 0; a < ka; ++a) {
+  // stage-0 inputs (the template's non-zero planes) of the next angle are
+  // loaded into registers while the current angle runs its LDS stages, where
+  // that costs at most 8 complex registers per thread (else stage 0 loads
+  // them itself)
+  using I0 = StageInfo<Z, 0, NL, NT_A, Lay>;
+  constexpr bool PRE = I0::PER * I0::R <= 8;
+  c32 pre[PRE ? I0::PER : 1][PRE ? I0::R : 1];
+  auto ld_at = [&](int a, int c, int z) -> c32 {
+    int uz = z + hz - sZ;
+    if (uz >= Z) uz -= Z;
+    if (uz >= nz) return mk(0.f, 0.f);
     const c32* ty = Ty + (long)a * nz * Y * Xh + (long)ky * Xh + tile * TW;
     const c32* tyn = Tyn + (long)a * nz * Y + ky;
-    auto ld = [&](int c, int z) -> c32 {
-      int uz = z + hz - sZ;
-      if (uz >= Z) uz -= Z;
-      if (uz >= nz) return mk(0.f, 0.f);
-      return (!NYQ || c < TW) ? ty[(long)uz * Y * Xh + c] : tyn[(long)uz * Y];
-    };
+    return (!NYQ || c < TW) ? ty[(long)uz * Y * Xh + c] : tyn[(long)uz * Y];
+  };
+  auto fetch = [&](int a) {
+    if constexpr (PRE) {
+#pragma unroll
+      for (int p = 0; p < I0::PER; ++p) {
+        if (I0::valid(p)) {
+          int line, j;
+          I0::idx(p, line, j);
+#pragma unroll
+          for (int r = 0; r < I0::R; ++r) pre[p][r] = ld_at(a, line, j + r * I0::NJ);
+        }
+      }
+    }
+  };
+  fetch(0);
+  for (int a = 0; a < ka; ++a) {
     auto prod = [&](int c, int kz, c32 v) -> c32 {
       if constexpr (FL)
         return cmulc(fis[kz * P + c], v);
       else
         return cmulc(fi_at(c, kz), v);
     };
-    stage0_g2l<float, Z, NL, NT_A, -1, Lay>(buf, ld);
+    if constexpr (PRE) {
+#pragma unroll
+      for (int p = 0; p < I0::PER; ++p) {
+        if (I0::valid(p)) {
+          int line, j;
+          I0::idx(p, line, j);
+          c32 v[I0::R];
+#pragma unroll
+          for (int r = 0; r < I0::R; ++r) v[r] = pre[p][r];
+          Bfly<float, I0::R, -1>::run(v);
+#pragma unroll
+          for (int r = 0; r < I0::R; ++r) buf[Lay::addr(line, j * I0::R + r)] = v[r];
+        }
+      }
+      if (a + 1 < ka) fetch(a + 1);
+      __syncthreads();
+    } else {
+      stage0_g2l<float, Z, NL, NT_A, -1, Lay>(buf, [&](int c, int z) { return ld_at(a, c, z); });
+    }
     StagesL2L<float, Z, 1, S - 1, NL, NT_A, -1, Lay>::run(buf, twZ);
     stage_l2l<float, Z, S - 1, NL, NT_A, -1, Lay>(buf, twZ, prod);
     if constexpr (NYQ) {
