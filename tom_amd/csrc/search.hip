@@ -237,6 +237,7 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c3
   __shared__ c32 twY[Y];
   __shared__ c32 twX[1024];
   __shared__ float plane[64 * 64];
+  __shared__ c32 xd[64 * P];  // x-DFT of the ny template rows, [uy][c]
   const int Xh = X / 2;
   const int tile = blockIdx.x, uz = blockIdx.y, a = blockIdx.z;
   const bool nyq = (tile == 0);
@@ -245,7 +246,6 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c3
   for (int i = threadIdx.x; i < X; i += NT) twX[i] = twXg[i];
   const float* src = t2 + ((long)a * nz + uz) * ny * nx;
   for (int i = threadIdx.x; i < nx * ny; i += NT) plane[i] = src[i];
-  for (int i = threadIdx.x; i < Y * P; i += NT) buf[i] = mk(0.f, 0.f);
   __syncthreads();
   const int hx = nx / 2, hy = ny / 2;
   const int sY = 2 * (Y / 2) - Y;
@@ -265,24 +265,29 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c3
       m += kx;
       if (m >= X) m -= X;
     }
-    int ry = (uy - hy + sY) % Y;
-    if (ry < 0) ry += Y;
-    buf[ry * P + c] = mk(re, im);
+    xd[uy * P + c] = mk(re, im);
   }
   __syncthreads();
-  if (nyq)
-    lds_fft<float, Y, P, NT, -1, Inter<P>>(buf, twY);
-  else
-    lds_fft<float, Y, TW, NT, -1, Inter<P>>(buf, twY);
+  // y FFT: row ry of the wrapped placement holds template row
+  // uy = (ry + hy - sY) mod Y when that is < ny, zero otherwise; stage 0 reads
+  // those rows from xd, the last stage writes Ty (and Tyn) directly
+  auto ld = [&](int c, int ry) -> c32 {
+    int uy = ry + hy - sY;
+    if (uy >= Y) uy -= Y;
+    return (uy < ny) ? xd[uy * P + c] : mk(0.f, 0.f);
+  };
   c32* dst = Ty + ((long)a * nz + uz) * Y * Xh + tile * TW;
-  for (int i = threadIdx.x; i < Y * TW; i += NT) {
-    const int ky = i / TW, c = i % TW;
-    dst[(long)ky * Xh + c] = buf[ky * P + c];
-  }
-  if (nyq) {
-    c32* dn = Tyn + ((long)a * nz + uz) * Y;
-    for (int ky = threadIdx.x; ky < Y; ky += NT) dn[ky] = buf[ky * P + TW];
-  }
+  c32* dn = Tyn + ((long)a * nz + uz) * Y;
+  auto st = [&](int, int, int c, int ky, c32 v) {
+    if (c < TW)
+      dst[(long)ky * Xh + c] = v;
+    else
+      dn[ky] = v;
+  };
+  if (nyq)
+    fft_g2g<float, Y, P, NT, -1, Inter<P>>(buf, twY, ld, st);
+  else
+    fft_g2g<float, Y, TW, NT, -1, Inter<P>>(buf, twY, ld, st);
 }
 
 // ---------------------------------------------------------------- pass A
