@@ -455,6 +455,16 @@ def main():
                 "traffic": MEASURED_TRAFFIC.get((args.config, KA), {}).get(dom),
                 "bytes_per_launch": int(model[dom]), "avg_launch_ms": round(avg_s * 1e3, 4),
                 "batch_full": full}
+        # whole-pipeline view: the unfused per-orientation byte model (a
+        # separate product pass, a three-pass c2r inverse and a reduction
+        # pass: 64 N_c + 16 N bytes, N_c = X Y (Z/2 + 1)) at the measured
+        # rate, against 8 TB/s; above 1 means fewer bytes than that model moves
+        Nc = X * Y * (Z // 2 + 1)
+        b_unfused = 64 * Nc + 16 * N
+        roof["unfused_model"] = {"bytes_per_orientation": int(b_unfused),
+                                 "equivalent_GBps": round(b_unfused * ntimed / dt / 1e9, 1),
+                                 "frac": round(b_unfused * ntimed / dt / 1e9 / HBM_PEAK_GBS, 4),
+                                 "fused_bytes_per_orientation": int(sum(model.values()) / KA)}
 
     # ---- GPU error against FP64 on the CPU sample + CPU baseline
     cpu = None
