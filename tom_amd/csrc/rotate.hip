@@ -5,14 +5,14 @@
 // arithmetic below follows the reference's float evaluation order exactly and
 // is compiled with FP contraction off, so results are bit-identical to it.
 //
-// Design: one workgroup produces one 8x8x8 output brick of one rotation. It
-// first computes the source-space bounding box of the brick from its 8
-// corners, loads that box (at most 18^3 floats) into LDS with coalesced,
-// wavefront-cooperative row loads, then interpolates every output voxel from
-// LDS. Boxes that do not fit (non-rotation 3x3 matrices with scaling) fall
-// back to gathering straight from global memory. A batch of rotations of the
-// same volume runs in one launch (grid.z = rotation index); an optional mask
-// multiplies the result (t = rot(T) .* M of the template-matching loop).
+// Design: one thread per output voxel gathers its eight taps from the source
+// volume, which for templates (up to 64^3) stays resident in L2; consecutive
+// threads own consecutive x, so the taps of a wavefront fall in a few cache
+// lines. A batch of rotations of the same volume runs in one launch (grid.y
+// strides over rotations); an optional mask multiplies the result
+// (t = rot(T) .* M of the template-matching loop).
+#include <algorithm>
+
 #include "common.h"
 #include "rotate.h"
 
@@ -22,8 +22,6 @@ namespace tomhip {
 
 namespace {
 
-constexpr int BR = 8;            // output brick edge
-constexpr int BOX = 18;          // max LDS box edge (floats)
 constexpr int NT_ROT = 256;
 
 struct RotParams {
@@ -36,6 +34,7 @@ struct RotParams {
   long dst_stride;      // elements between consecutive rotations in dst
   const long* desc;     // optional graph batch descriptor {batch, first}: rm += batch*rm_step
   long rm_step;
+  int nrot;
 };
 
 __device__ __forceinline__ float interp(const float* img, int b, int sx, int sxy, int ex, int ey,
@@ -56,98 +55,44 @@ __device__ __forceinline__ float interp(const float* img, int b, int sx, int sxy
   return (AA * vy1 + BB * vy2) * vz1 + (CC * vy1 + DD * vy2) * vz2;
 }
 
-__global__ __launch_bounds__(NT_ROT) void k_rotate_brick(RotParams P) {
-  __shared__ float box[BOX * BOX * BOX];
-  __shared__ int sh_lo[3], sh_hi[3], sh_fit;
-
-  const int nbx = (P.sx + BR - 1) / BR, nby = (P.sy + BR - 1) / BR;
-  const int bid = blockIdx.x;
-  const int bx0 = (bid % nbx) * BR, by0 = ((bid / nbx) % nby) * BR, bz0 = (bid / (nbx * nby)) * BR;
-  const int rot = blockIdx.y;
-  const float* rmp = P.rm + rot * 9 + (P.desc ? P.desc[0] * P.rm_step : 0);
-  const float rm00 = rmp[0], rm10 = rmp[1], rm20 = rmp[2];
-  const float rm01 = rmp[3], rm11 = rmp[4], rm21 = rmp[5];
-  const float rm02 = rmp[6], rm12 = rmp[7], rm22 = rmp[8];
-  const float px = P.px, py = P.py, pz = P.pz;
+// One thread per output voxel, its eight taps gathered straight from the
+// source (a template box is L2-resident, so staging it in LDS brick by brick
+// measured slower). Rotations stride over grid.y.
+__global__ __launch_bounds__(NT_ROT) void k_rotate_direct(RotParams P) {
   const int sx = P.sx, sy = P.sy, sz = P.sz, sxy = sx * sy;
-
-  if (threadIdx.x == 0) {
-    // Source bounding box of the brick: the mapped coordinate is affine in
-    // the truncated offsets (pi, pj, pk), which are monotone in (i, j, k), so
-    // the extremes sit at the brick's corners. One voxel of margin each side
-    // covers float rounding; +1 more on the high side for the second tap.
-    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-    const int ie = min(bx0 + BR, sx) - 1, je = min(by0 + BR, sy) - 1, ke = min(bz0 + BR, sz) - 1;
-    for (int c = 0; c < 8; ++c) {
-      const int i = (c & 1) ? ie : bx0, j = (c & 2) ? je : by0, k = (c & 4) ? ke : bz0;
-      const float pi = (float)(int)((float)i - px), pj = (float)(int)((float)j - py),
-                  pk = (float)(int)((float)k - pz);
-      const float r[3] = {px + rm00 * pi + rm10 * pj + rm20 * pk, py + rm01 * pi + rm11 * pj + rm21 * pk,
-                          pz + rm02 * pi + rm12 * pj + rm22 * pk};
-      for (int d = 0; d < 3; ++d) {
-        lo[d] = fminf(lo[d], r[d]);
-        hi[d] = fmaxf(hi[d], r[d]);
-      }
-    }
-    const int n[3] = {sx, sy, sz};
-    int fit = 1;
-    for (int d = 0; d < 3; ++d) {
-      int l = (int)floorf(lo[d]) - 1, h = (int)floorf(hi[d]) + 2;
-      l = max(l, 0);
-      h = min(h, n[d] - 1);
-      if (h < l) h = l;  // brick maps entirely outside: nothing is read
-      sh_lo[d] = l;
-      sh_hi[d] = h;
-      if (h - l + 1 > BOX) fit = 0;
-    }
-    sh_fit = fit;
-  }
-  __syncthreads();
-  const int fit = sh_fit;
-  const int lx = sh_lo[0], ly = sh_lo[1], lz = sh_lo[2];
-  const int ex_ = sh_hi[0] - lx + 1, ey_ = sh_hi[1] - ly + 1, ez_ = sh_hi[2] - lz + 1;
+  const long n3 = (long)sxy * sz;
+  const float px = P.px, py = P.py, pz = P.pz;
   const float* img = P.src;
-  if (fit) {
-    const int tot = ex_ * ey_ * ez_;
-    for (int t = threadIdx.x; t < tot; t += NT_ROT) {
-      const int x = t % ex_, y = (t / ex_) % ey_, z = t / (ex_ * ey_);
-      box[(z * BOX + y) * BOX + x] = img[(long)(lz + z) * sxy + (long)(ly + y) * sx + (lx + x)];
-    }
-  }
-  __syncthreads();
-
-  float* out = P.dst + (long)rot * P.dst_stride;
-  for (int v = threadIdx.x; v < BR * BR * BR; v += NT_ROT) {
-    const int i = bx0 + (v % BR), j = by0 + ((v / BR) % BR), k = bz0 + v / (BR * BR);
-    if (i >= sx || j >= sy || k >= sz) continue;
-    const long o = (long)k * sxy + (long)j * sx + i;
-    const float pi = (float)(int)((float)i - px);
-    const float pj = (float)(int)((float)j - py);
-    const float pk = (float)(int)((float)k - pz);
-    float val = 0.f;
-    const float rx = px + rm00 * pi + rm10 * pj + rm20 * pk;
-    if (!(rx < 0 || rx > (float)(sx - 1))) {
-      const float ry = py + rm01 * pi + rm11 * pj + rm21 * pk;
-      if (!(ry < 0 || ry > (float)(sy - 1))) {
-        const float rz = pz + rm02 * pi + rm12 * pj + rm22 * pk;
-        if (!(rz < 0 || rz > (float)(sz - 1))) {
-          const int fx = (int)rx, fy = (int)ry, fz = (int)rz;
-          const float vx2 = rx - (float)fx, vx1 = 1 - vx2;
-          const float vy2 = ry - (float)fy, vy1 = 1 - vy2;
-          const float vz2 = rz - (float)fz, vz1 = 1 - vz2;
-          const int ex = (fx == sx - 1), ey = (fy == sy - 1), ez = (fz == sz - 1);
-          if (fit) {
-            const int b = ((fz - lz) * BOX + (fy - ly)) * BOX + (fx - lx);
-            val = interp(box, b, BOX, BOX * BOX, ex, ey, ez, vx2, vx1, vy2, vy1, vz2, vz1);
-          } else {
-            const int b = fx + fy * sx + fz * sxy;
-            val = interp(img, b, sx, sxy, ex, ey, ez, vx2, vx1, vy2, vy1, vz2, vz1);
+  for (int rot = blockIdx.y; rot < P.nrot; rot += gridDim.y) {
+    const float* rmp = P.rm + (long)rot * 9 + (P.desc ? P.desc[0] * P.rm_step : 0);
+    const float rm00 = rmp[0], rm10 = rmp[1], rm20 = rmp[2];
+    const float rm01 = rmp[3], rm11 = rmp[4], rm21 = rmp[5];
+    const float rm02 = rmp[6], rm12 = rmp[7], rm22 = rmp[8];
+    float* out = P.dst + (long)rot * P.dst_stride;
+    for (long o = (long)blockIdx.x * NT_ROT + threadIdx.x; o < n3; o += (long)gridDim.x * NT_ROT) {
+      const int i = (int)(o % sx), j = (int)((o / sx) % sy), k = (int)(o / sxy);
+      const float pi = (float)(int)((float)i - px);
+      const float pj = (float)(int)((float)j - py);
+      const float pk = (float)(int)((float)k - pz);
+      float val = 0.f;
+      const float rx = px + rm00 * pi + rm10 * pj + rm20 * pk;
+      if (!(rx < 0 || rx > (float)(sx - 1))) {
+        const float ry = py + rm01 * pi + rm11 * pj + rm21 * pk;
+        if (!(ry < 0 || ry > (float)(sy - 1))) {
+          const float rz = pz + rm02 * pi + rm12 * pj + rm22 * pk;
+          if (!(rz < 0 || rz > (float)(sz - 1))) {
+            const int fx = (int)rx, fy = (int)ry, fz = (int)rz;
+            const float vx2 = rx - (float)fx, vx1 = 1 - vx2;
+            const float vy2 = ry - (float)fy, vy1 = 1 - vy2;
+            const float vz2 = rz - (float)fz, vz1 = 1 - vz2;
+            const int ex = (fx == sx - 1), ey = (fy == sy - 1), ez = (fz == sz - 1);
+            val = interp(img, fx + fy * sx + fz * sxy, sx, sxy, ex, ey, ez, vx2, vx1, vy2, vy1, vz2, vz1);
           }
         }
       }
+      if (P.mask) val = val * P.mask[o];
+      out[o] = val;
     }
-    if (P.mask) val = val * P.mask[o];
-    out[o] = val;
   }
 }
 
@@ -215,9 +160,11 @@ void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, 
   P.dst_stride = dst_stride;
   P.desc = desc;
   P.rm_step = rm_step;
-  const int nb = ((sx + BR - 1) / BR) * ((sy + BR - 1) / BR) * ((sz + BR - 1) / BR);
-  dim3 grid(nb, nrot);
-  hipLaunchKernelGGL(k_rotate_brick, grid, dim3(NT_ROT), 0, stream, P);
+  P.nrot = nrot;
+  const long n3 = (long)sx * sy * sz;
+  const long bx = std::min((n3 + NT_ROT - 1) / NT_ROT, 65535L);
+  dim3 grid((unsigned)bx, (unsigned)std::min(nrot, 65535));
+  hipLaunchKernelGGL(k_rotate_direct, grid, dim3(NT_ROT), 0, stream, P);
   TOMHIP_CHECK(hipGetLastError());
 }
 

@@ -639,18 +639,25 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
                                                  const float* __restrict__ t, int G, int nx, int ny, int nz, int sx,
                                                  int sy, int sz, const int4* __restrict__ cand,
                                                  const double* __restrict__ mu, const double* __restrict__ sig,
-                                                 double inv_m, double mu_t, double sig_t, int a0, int nvalid,
-                                                 const int4* __restrict__ shift, float* bscore, int32_t* bangle,
-                                                 int32_t* bpos) {
+                                                 double inv_m, double mu_t, double sig_t, int a0, int A,
+                                                 const int4* __restrict__ shift, int chunk0, float* cscore,
+                                                 int32_t* cangle, int32_t* cpos) {
+  // blockIdx.x: candidate; blockIdx.y: orientation chunk of this launch
+  // (orientations a0 + KS y .. a0 + KS y + KS - 1, templates t + KS y G n3);
+  // the chunk's best goes to row chunk0 + y of the per-chunk results
   extern __shared__ float ring[];  // 3 planes of PH x PW
+  const int P = gridDim.x;
+  const int ac = a0 + KS * (int)blockIdx.y;
+  const int nvalid = min(KS, A - ac);
+  const long n3 = (long)nx * ny * nz;
+  t += (long)blockIdx.y * KS * G * n3;
   int4 c = cand[blockIdx.x];
   if (shift) {  // window centre of this orientation chunk: candidate - shift[group]
-    const int4 h = shift[c.w];
+    const int4 h = shift[(long)blockIdx.y * G + c.w];
     c.x = wrapi(c.x - h.x, X);
     c.y = wrapi(c.y - h.y, Y);
     c.z = wrapi(c.z - h.z, Z);
   }
-  const long n3 = (long)nx * ny * nz;
   const int PW = nx + 2, PH = ny + 2, PS = PW * PH;
   const int C = (NT_R / ny) > 0 ? NT_R / ny : 1;  // segments per template row
   const int SEG = (nx + C - 1) / C;
@@ -726,8 +733,8 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
     }
   __syncthreads();
   if (threadIdx.x == 0) {
-    float best = bscore[blockIdx.x];
-    int32_t ba = bangle[blockIdx.x], bp = bpos[blockIdx.x];
+    float best = -FLT_MAX;
+    int32_t ba = -1, bp = -1;
     for (int k = 0; k < nvalid; ++k)
       for (int q = 0; q < 27; ++q) {
         float cc = 0.f;
@@ -738,21 +745,35 @@ __global__ __launch_bounds__(NT_R) void k_refine(const float* __restrict__ V, in
         const float sc = den > 0 ? (float)(((double)cc * inv_m - mu[idx] * mu_t) / den) : 0.f;
         if (sc > best) {
           best = sc;
-          ba = a0 + k;
+          ba = ac + k;
           bp = (int32_t)idx;
         }
       }
-    bscore[blockIdx.x] = best;
-    bangle[blockIdx.x] = ba;
-    bpos[blockIdx.x] = bp;
+    const long o = (long)(chunk0 + blockIdx.y) * P + blockIdx.x;
+    cscore[o] = best;
+    cangle[o] = ba;
+    cpos[o] = bp;
   }
 }
 
-__global__ void k_refine_init(float* s, int32_t* a, int32_t* p, long n) {
-  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
-    s[i] = -FLT_MAX;
-    a[i] = -1;
-    p[i] = -1;
+// Best over the chunks in orientation order (strict '>': the first chunk wins
+// a tie, as a sequential scan over all orientations would).
+__global__ void k_refine_reduce(const float* cscore, const int32_t* cangle, const int32_t* cpos, int nch, long P,
+                                float* s, int32_t* a, int32_t* p) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < P; i += (long)gridDim.x * NT) {
+    float best = -FLT_MAX;
+    int32_t ba = -1, bp = -1;
+    for (int ch = 0; ch < nch; ++ch) {
+      const float v = cscore[(long)ch * P + i];
+      if (v > best) {
+        best = v;
+        ba = cangle[(long)ch * P + i];
+        bp = cpos[(long)ch * P + i];
+      }
+    }
+    s[i] = best;
+    a[i] = ba;
+    p[i] = bp;
   }
 }
 
@@ -1293,7 +1314,6 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
   TOMHIP_REQUIRE((long)G * A <= (1L << 24) && P <= (1L << 30), ERR_WRONG_INPUT, "refinement too large");
   constexpr int KS = 4;
   const long n3 = (long)nx * ny * nz;
-  const long ntp = (long)KS * G;
   const int nch = (A + KS - 1) / KS;
   // With a centroid d (template frame, voxels from the rotation centre) the
   // candidates locate the rotated centroid R d + p of a particle at p, so
@@ -1315,15 +1335,27 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
           hshift[((size_t)ch * G + g) * 4 + j] = (int32_t)std::lround(acc[j] / (a1 - ch * KS));
       }
   }
+  // templates of several chunks are prepared together and scored in one
+  // launch (candidates x chunks), within a memory budget; each chunk's best
+  // lands in a per-chunk row, reduced in orientation order at the end
+  const long per_chunk = (long)KS * G * n3;
+  const long bytes_chunk = per_chunk * (long)(sizeof(float) + (wsh_ ? sizeof(c32) : 0));
+  const long budget = 16L << 30;
+  const int cb = (int)std::max(1L, std::min((long)nch, budget / bytes_chunk));
   float* rm = nullptr;
   float* tt = nullptr;
   c32* work = nullptr;
   int32_t* dshift = nullptr;
+  float* cs = nullptr;
+  int32_t *ca = nullptr, *cp = nullptr;
   auto release = [&] {
     (void)hipFree(rm);
     (void)hipFree(tt);
     (void)hipFree(work);
     (void)hipFree(dshift);
+    (void)hipFree(cs);
+    (void)hipFree(ca);
+    (void)hipFree(cp);
   };
   try {
     if (centroid) {
@@ -1331,26 +1363,31 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
       TOMHIP_CHECK(hipMemcpyAsync(dshift, hshift.data(), sizeof(int32_t) * hshift.size(), hipMemcpyHostToDevice, s_));
     }
     TOMHIP_CHECK(hipMalloc(&rm, sizeof(float) * 9 * (long)G * A));
-    TOMHIP_CHECK(hipMalloc(&tt, sizeof(float) * n3 * ntp));
-    if (wsh_) TOMHIP_CHECK(hipMalloc(&work, sizeof(c32) * n3 * ntp));
+    TOMHIP_CHECK(hipMalloc(&tt, sizeof(float) * per_chunk * cb));
+    if (wsh_) TOMHIP_CHECK(hipMalloc(&work, sizeof(c32) * per_chunk * cb));
+    TOMHIP_CHECK(hipMalloc(&cs, sizeof(float) * (long)nch * P));
+    TOMHIP_CHECK(hipMalloc(&ca, sizeof(int32_t) * (long)nch * P));
+    TOMHIP_CHECK(hipMalloc(&cp, sizeof(int32_t) * (long)nch * P));
     TOMHIP_CHECK(hipMemcpyAsync(rm, rm_host, sizeof(float) * 9 * (long)G * A, hipMemcpyHostToDevice, s_));
-    hipLaunchKernelGGL(k_refine_init, dim3(grid_for(P)), dim3(NT), 0, s_, bscore, bangle, bpos, P);
-    TOMHIP_CHECK(hipGetLastError());
     // per-axis s - h of c[k] = sum_u t[u] V[k + u + s - h]
     const int sx = 2 * (X / 2) - X - nx / 2, sy = 2 * (Y / 2) - Y - ny / 2, sz = 2 * (Z / 2) - Z - nz / 2;
     const double inv_m = 1.0 / m;
-    for (int a0 = 0; a0 < A; a0 += KS) {
-      const int cnt = std::min(KS, A - a0);
+    const size_t lds = sizeof(float) * 3 * (nx + 2) * (ny + 2);
+    for (int c0 = 0; c0 < nch; c0 += cb) {
+      const int nb = std::min(cb, nch - c0);
+      const int a0 = c0 * KS;
+      const int cnt = std::min(nb * KS, A - a0);
       rotate_templates(rm + 9L * a0 * G, cnt * G, tt, nullptr, 0);
       if (wsh_) box_filter(cnt * G, tt, work);
-      const size_t lds = sizeof(float) * 3 * (nx + 2) * (ny + 2);
-      hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P), dim3(NT_R), lds, s_, volume, X, Y, Z, (const float*)tt, G,
-                         nx, ny, nz, sx, sy, sz, reinterpret_cast<const int4*>(cand_dev), (const double*)mu_,
-                         (const double*)sig_, inv_m, mu_t, sig_t, a0, cnt,
-                         dshift ? reinterpret_cast<const int4*>(dshift) + (long)(a0 / KS) * G : nullptr, bscore,
-                         bangle, bpos);
+      hipLaunchKernelGGL(k_refine<KS>, dim3((unsigned)P, (unsigned)nb), dim3(NT_R), lds, s_, volume, X, Y, Z,
+                         (const float*)tt, G, nx, ny, nz, sx, sy, sz, reinterpret_cast<const int4*>(cand_dev),
+                         (const double*)mu_, (const double*)sig_, inv_m, mu_t, sig_t, a0, A,
+                         dshift ? reinterpret_cast<const int4*>(dshift) + (long)c0 * G : nullptr, c0, cs, ca, cp);
       TOMHIP_CHECK(hipGetLastError());
     }
+    hipLaunchKernelGGL(k_refine_reduce, dim3(grid_for(P)), dim3(NT), 0, s_, (const float*)cs, (const int32_t*)ca,
+                       (const int32_t*)cp, nch, P, bscore, bangle, bpos);
+    TOMHIP_CHECK(hipGetLastError());
     TOMHIP_CHECK(hipStreamSynchronize(s_));
   } catch (...) {
     release();
