@@ -46,10 +46,10 @@ CONFIGS = {
                mask=("gauss", 10.0, 2.0), bp=(2.0, 20.0), step=(30, 30, 30), seed=1,
                desc="C1 plain template matching 64^3, 24^3 template, 30 deg (1,008 orientations)"),
     "c2": dict(N=(512, 512, 256), n=48, k=12, sigma=2.0, R=18, P=300, spacing=40, snr=0.1, wedge=30.0, tilt=60.0,
-               mask=("binary", 22.0, 0.0), bp=None, step=(15, 15, 15), seed=2,
+               mask=("binary", 22.0, 0.0), bp=None, step=(15, 15, 15), seed=2, tiles="16,32",
                desc="C2 plain exhaustive FLCF 512x512x256, 48^3 template, 15 deg (7,488 orientations)"),
     "c5": dict(N=(1024, 1024, 384), n=48, k=12, sigma=2.0, R=18, P=2000, spacing=48, snr=0.05, wedge=40.0, tilt=50.0,
-               mask=("binary", 22.0, 0.0), bp=None, step=(10, 10, 10), seed=5,
+               mask=("binary", 22.0, 0.0), bp=None, step=(10, 10, 10), seed=5, tiles="8,16",
                desc="C5 exhaustive FLCF 1024x1024x384, 48^3 template, 10 deg (24,624 orientations)"),
 }
 
@@ -136,7 +136,7 @@ PRUNED = {
 def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, plan_template, pre_s):
     """Coarse-to-fine pruned search on the benchmark tomogram (coarse scan of
     the tom_bin-ned tomogram and template, exact refinement at full
-    resolution), judged by the exhaustive run: recall of its picklist peaks
+    resolution), checked against the exhaustive run: recall of its picklist peaks
     above 0.3 (a refined hit within 3 voxels with refined score > 0.3) and the
     wall-time ratio. When
     trained weights exist, the proposal network (second input channel: the
@@ -307,7 +307,8 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="launch groups to time (0 = the full angle set)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--batch", default="16", help="orientations per launch group, or 'auto' (autotuner)")
-    ap.add_argument("--tiles", default="16,32", help="column tiles of the z- and y-FFT passes (8/16/32)")
+    ap.add_argument("--tiles", default="", help="column tiles of the z- and y-FFT passes (8/16/32); default: the "
+                                              "workload's measured best")
     ap.add_argument("--zchunk", type=int, default=0, help="z-chunk planes of the y-FFT and c2r passes (0 = whole)")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--cpu-particles", type=int, default=24, help="c3: particles in the oracle sample")
@@ -323,7 +324,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the search would shard by angle subsets with one max/argmax merge; see DESIGN.md)",
+              "(the search would shard by angle subsets with one max/argmax merge of the maps)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -343,7 +344,7 @@ def main():
     angles = angle_grid(cfg["step"])
     A = angles.shape[0]
     KA = 16 if args.batch == "auto" else int(args.batch)
-    tiles = tuple(int(v) for v in args.tiles.split(","))
+    tiles = tuple(int(v) for v in (args.tiles or cfg.get("tiles", "16,16")).split(","))
     zchunk = args.zchunk
     groups = (A + KA - 1) // KA
     steps = args.steps if args.steps > 0 else groups
@@ -429,7 +430,7 @@ def main():
     found = float(np.mean([sc[tuple(p)] > 0.2 for p in pos])) if ntimed == A else None
 
     # ---- pruned search (and the proposal network) on the same tomogram,
-    # judged by the exhaustive run
+    # checked against the exhaustive run
     sur = None
     if ntimed == A and not args.no_surrogate:
         sur = pruned_leg(args, cfg, api, torch, plan, V, score, ang, args.surrogate,
