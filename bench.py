@@ -493,14 +493,18 @@ def main():
         log(f"[bench] max |score(GPU FP32) - score(FP64 oracle)| over the sample: {err:.3e}")
 
     plan.close()
+    # the production path is the hipGraph replay of the angle batches; the
+    # eager pass with per-kernel events (same orientations) feeds `roofline`
+    eager = {"value": value, "ms_per_step": dt / max(1, math.ceil(ntimed / KA)) * 1e3}
+    head = graph if graph else eager
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "voxel-orientations/s",
         "n_gpus": 1,
         "steps": int(ntimed // KA + (ntimed % KA > 0)),
         "warmup": args.warmup,
-        "ms_per_step": dt / max(1, math.ceil(ntimed / KA)) * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -510,8 +514,8 @@ def main():
                    "orientations_timed": int(ntimed), "orientations_total": int(A), "batch": KA, "tiles_zy": list(tiles), "zchunk": zchunk,
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
-                   "timed_path": "eager launches with per-kernel HIP events",
-                   "graph_replay": graph, "autotune": tuned, "pruned_search": sur},
+                   "timed_path": "hipGraph replay" if graph else "eager launches with per-kernel HIP events",
+                   "graph_replay": graph, "eager_with_events": eager, "autotune": tuned, "pruned_search": sur},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
