@@ -348,24 +348,42 @@ __device__ __forceinline__ void up_taps(int o, int n, int* i0, int* i1, float* w
 }
 
 // X [Vlow][C] bf16 -> Y rows (stride ys, offset yo) at twice the size
+// One thread per output voxel and 8 channels: the taps and weights are
+// computed once for the 8 channels, which move as 16-byte vectors (C, ys and
+// yo multiples of 8).
 __global__ void k_up(const uint16_t* X, int C, Dims lo, uint16_t* Y, int ys, int yo) {
   const Dims hi{lo.D * 2, lo.H * 2, lo.W * 2};
-  const long n = hi.V() * C;
+  const int C8 = C / 8;
+  const long n = hi.V() * C8;
   for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
-    const int c = (int)(i % C);
-    const long v = i / C;
+    const int c = (int)(i % C8) * 8;
+    const long v = i / C8;
     const int x = (int)(v % hi.W), y = (int)((v / hi.W) % hi.H), z = (int)(v / ((long)hi.W * hi.H));
     int xi[2], yi[2], zi[2];
     float xw[2], yw[2], zw[2];
     up_taps(x, lo.W, &xi[0], &xi[1], &xw[0], &xw[1]);
     up_taps(y, lo.H, &yi[0], &yi[1], &yw[0], &yw[1]);
     up_taps(z, lo.D, &zi[0], &zi[1], &zw[0], &zw[1]);
-    float s = 0;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b)
-        for (int e = 0; e < 2; ++e)
-          s += zw[a] * yw[b] * xw[e] * bf2f(X[(((long)zi[a] * lo.H + yi[b]) * lo.W + xi[e]) * C + c]);
-    Y[v * ys + yo + c] = f2bf(s);
+        for (int e = 0; e < 2; ++e) {
+          const float w = zw[a] * yw[b] * xw[e];
+          const uint4 q =
+              *reinterpret_cast<const uint4*>(X + (((long)zi[a] * lo.H + yi[b]) * lo.W + xi[e]) * C + c);
+          const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            s[2 * k] += w * bf2f((uint16_t)(u[k] & 0xffffu));
+            s[2 * k + 1] += w * bf2f((uint16_t)(u[k] >> 16));
+          }
+        }
+    uint4 o;
+    o.x = (uint32_t)f2bf(s[0]) | ((uint32_t)f2bf(s[1]) << 16);
+    o.y = (uint32_t)f2bf(s[2]) | ((uint32_t)f2bf(s[3]) << 16);
+    o.z = (uint32_t)f2bf(s[4]) | ((uint32_t)f2bf(s[5]) << 16);
+    o.w = (uint32_t)f2bf(s[6]) | ((uint32_t)f2bf(s[7]) << 16);
+    *reinterpret_cast<uint4*>(Y + v * ys + yo + c) = o;
   }
 }
 
@@ -869,10 +887,10 @@ void Unet::forward(const float* vol, const float* aux) {
   hipLaunchKernelGGL(k_pool, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, I.c2, 96, 64, 32, d1, I.p2);
   conv(4, I.p2, 32, 0, d2.D, d2.H, d2.W, I.b1, 64, 0, nullptr);
   conv(5, I.b1, 64, 0, d2.D, d2.H, d2.W, I.b2, 64, 0, nullptr);
-  hipLaunchKernelGGL(k_up, dim3(gridv(I.V1 * 64)), dim3(NTU), 0, s, I.b2, 64, d2, I.c2, 96, 0);
+  hipLaunchKernelGGL(k_up, dim3(gridv(I.V1 * 8)), dim3(NTU), 0, s, I.b2, 64, d2, I.c2, 96, 0);
   conv(6, I.c2, 96, 0, d1.D, d1.H, d1.W, I.d2a, 32, 0, nullptr);
   conv(7, I.d2a, 32, 0, d1.D, d1.H, d1.W, I.d2b, 32, 0, nullptr);
-  hipLaunchKernelGGL(k_up, dim3(gridv(I.V0 * 32)), dim3(NTU), 0, s, I.d2b, 32, d1, I.c1, 48, 0);
+  hipLaunchKernelGGL(k_up, dim3(gridv(I.V0 * 4)), dim3(NTU), 0, s, I.d2b, 32, d1, I.c1, 48, 0);
   conv(8, I.c1, 48, 0, D, H, W, I.d1a, 16, 0, nullptr);
   conv(9, I.d1a, 16, 0, D, H, W, I.d1b, 16, 0, nullptr);
   conv(10, I.d1b, 16, 0, D, H, W, nullptr, 0, 0, I.logits);
