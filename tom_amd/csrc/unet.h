@@ -31,7 +31,9 @@ class Unet {
   void grads(float* host);
   // vol: device f32 [D][H][W] (x fastest), already normalised; aux (optional,
   // same layout, normalised): a second input channel
-  void forward(const float* vol, const float* aux);
+  // head = false stops before the 1x1x1 head (no FP32 logits written); the
+  // tiled inference then runs head_predict
+  void forward(const float* vol, const float* aux, bool head = true);
   double forward_flops() const;
   // tprob: device f32 [V] particle targets in [0, 1]; tbin: device int32 [V]
   // bin index or -1; npos = #{tbin >= 0}; toff (optional): device f32
@@ -44,6 +46,9 @@ class Unet {
   // prob = sigmoid of the particle logit (the logit itself if logit); offs
   // (optional, [V][3]) = the offset channels
   void predict(float* prob, int32_t* bin, bool logit = false, float* offs = nullptr);
+  // predict's outputs computed from the last decoder activations directly
+  // (the head fused with the prediction, FP32 sums on the vector units)
+  void head_predict(float* prob, int32_t* bin, bool logit = false, float* offs = nullptr);
   // whole volume vol (device f32 X x Y x Z) and optional aux map of the same
   // size, each normalised here by its global mean and standard deviation, in
   // tiles of the network size with halo voxels of context on each side;

@@ -561,6 +561,54 @@ __global__ __launch_bounds__(NTU) void k_predict(const float* logits, int CH, in
   }
 }
 
+// Inference head fused with the prediction: the 1x1x1 head (16 -> 1 + nbin
+// + 3 channels, bf16 weights, FP32 sums over the 16 inputs in order) at one
+// voxel per thread, writing only prob (or the logit), the first argmax bin
+// and the offsets instead of the FP32 logits of every head channel.
+__global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
+                                                      const float* __restrict__ bias, int nbin, long V, float* prob,
+                                                      int32_t* bin, float* offs, int raw) {
+  __shared__ float w[80 * 16];
+  __shared__ float bb[80];
+  const int nc = 1 + nbin + 3;
+  for (int i = threadIdx.x; i < nc * 16; i += NTU) w[i] = bf2f(Wt[i]);
+  for (int i = threadIdx.x; i < nc; i += NTU) bb[i] = bias[i];
+  __syncthreads();
+  for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
+    const uint4* src = reinterpret_cast<const uint4*>(X + v * 16);
+    const uint4 q0 = src[0], q1 = src[1];
+    const uint32_t u[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    float x[16];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      x[2 * k] = bf2f((uint16_t)(u[k] & 0xffffu));
+      x[2 * k + 1] = bf2f((uint16_t)(u[k] >> 16));
+    }
+    auto dot = [&](int c) {
+      float a = 0.f;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) a = fmaf(x[k], w[c * 16 + k], a);
+      return a + bb[c];
+    };
+    const float z = dot(0);
+    if (prob) prob[v] = raw ? z : 1.f / (1.f + expf(-z));
+    if (bin) {
+      float best = dot(1);
+      int bi = 0;
+      for (int c = 1; c < nbin; ++c) {
+        const float l = dot(1 + c);
+        if (l > best) {
+          best = l;
+          bi = c;
+        }
+      }
+      bin[v] = bi;
+    }
+    if (offs)
+      for (int c = 0; c < 3; ++c) offs[v * 3 + c] = dot(1 + nbin + c);
+  }
+}
+
 // sums of a volume (for the global normalisation of the input)
 __global__ void k_vsum(const float* a, long n, double* out) {
   double s = 0, q = 0;
@@ -866,7 +914,7 @@ void Unet::grads(float* out) {
   TOMHIP_CHECK(hipStreamSynchronize(I.s));
 }
 
-void Unet::forward(const float* vol, const float* aux) {
+void Unet::forward(const float* vol, const float* aux, bool head) {
   Impl& I = *d_;
   hipStream_t s = I.s;
   const int D = I.D, H = I.H, W = I.W;
@@ -893,7 +941,16 @@ void Unet::forward(const float* vol, const float* aux) {
   hipLaunchKernelGGL(k_up, dim3(gridv(I.V0 * 4)), dim3(NTU), 0, s, I.d2b, 32, d1, I.c1, 48, 0);
   conv(8, I.c1, 48, 0, D, H, W, I.d1a, 16, 0, nullptr);
   conv(9, I.d1a, 16, 0, D, H, W, I.d1b, 16, 0, nullptr);
-  conv(10, I.d1b, 16, 0, D, H, W, nullptr, 0, 0, I.logits);
+  if (head) conv(10, I.d1b, 16, 0, D, H, W, nullptr, 0, 0, I.logits);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+void Unet::head_predict(float* prob, int32_t* bin, bool logit, float* offs) {
+  Impl& I = *d_;
+  ConvLayer& c = I.L[10];
+  hipLaunchKernelGGL(k_head_predict, dim3(gridv(I.V0)), dim3(NTU), 0, I.s, (const uint16_t*)I.d1b,
+                     (const uint16_t*)c.wb, (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.nbin, I.V0, prob,
+                     bin, offs, logit ? 1 : 0);
   TOMHIP_CHECK(hipGetLastError());
 }
 
@@ -1023,8 +1080,8 @@ void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int ha
         if (aux)
           hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, aux, X, Y, Z, ox, oy, oz, t,
                              (const double*)(I.sums + 2), n, I.tile_aux);
-        forward(I.tile_in, aux ? I.tile_aux : nullptr);
-        predict(I.tile_prob, I.tile_bin, logit, offs ? I.tile_offs : nullptr);
+        forward(I.tile_in, aux ? I.tile_aux : nullptr, false);
+        head_predict(I.tile_prob, I.tile_bin, logit, offs ? I.tile_offs : nullptr);
         hipLaunchKernelGGL(k_tile_out, dim3(gridv((long)cx * cy * cz)), dim3(NTU), 0, s, (const float*)I.tile_prob,
                            (const int32_t*)I.tile_bin, (const float*)I.tile_offs, t, halo, cx, cy, cz, ox, oy, oz,
                            X, Y, Z, prob, bin, offs);
