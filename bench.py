@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (fetch scaled x2.000, write x1.000 by a 1 GiB device copy in the
 # same runs); profiles/flcf_c2_hbm_traffic.md has the commands and counts.
 MEASURED_TRAFFIC = {("c2", 4): {"passA": 1.5514e9, "passB": 2.1476e9, "passC": 1.6414e9},
-                    ("c2", 16): {"passA": 5.3961e9, "passB": 8.5903e9, "passC": 4.9830e9}}
+                    ("c2", 16): {"passA": 5.3959e9, "passB": 8.5902e9, "passC": 5.0465e9}}
 
 CONFIGS = {
     # name: volume (X,Y,Z), template edge, blobs, sigma, radius, particles,
