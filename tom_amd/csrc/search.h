@@ -126,6 +126,11 @@ class Search {
   long rm_cap_ = 0;
   float* t1_ = nullptr;      // [KA][n3] rotated (masked, filtered) templates
   c32* tbox_ = nullptr;      // [KA][n3] box filter workspace
+  // refinement workspaces (templates, box filter), kept between refine calls
+  // and grown on demand; freed with the plan
+  float* ref_tt_ = nullptr;
+  c32* ref_work_ = nullptr;
+  long ref_cap_ = 0;         // templates of n3 voxels each buffer holds
   c32* Ty_ = nullptr;        // [KA][nz][Y][Xh]
   c32* Tyn_ = nullptr;       // [KA][nz][Y]
   c32* Q_ = nullptr;         // [KA][Z][Y][Xh]

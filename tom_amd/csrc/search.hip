@@ -879,6 +879,8 @@ Search::~Search() {
   (void)hipFree(sig_);
   (void)hipFree(wsh_);
   (void)hipFree(rm_);
+  (void)hipFree(ref_tt_);
+  (void)hipFree(ref_work_);
   (void)hipFree(const_cast<float*>(mask_));
   drop_graph();
   (void)hipFree(desc_);
@@ -1343,15 +1345,11 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
   const long budget = 16L << 30;
   const int cb = (int)std::max(1L, std::min((long)nch, budget / bytes_chunk));
   float* rm = nullptr;
-  float* tt = nullptr;
-  c32* work = nullptr;
   int32_t* dshift = nullptr;
   float* cs = nullptr;
   int32_t *ca = nullptr, *cp = nullptr;
   auto release = [&] {
     (void)hipFree(rm);
-    (void)hipFree(tt);
-    (void)hipFree(work);
     (void)hipFree(dshift);
     (void)hipFree(cs);
     (void)hipFree(ca);
@@ -1363,8 +1361,20 @@ void Search::refine(const float* volume, const float* rm_host, int G, int A, con
       TOMHIP_CHECK(hipMemcpyAsync(dshift, hshift.data(), sizeof(int32_t) * hshift.size(), hipMemcpyHostToDevice, s_));
     }
     TOMHIP_CHECK(hipMalloc(&rm, sizeof(float) * 9 * (long)G * A));
-    TOMHIP_CHECK(hipMalloc(&tt, sizeof(float) * per_chunk * cb));
-    if (wsh_) TOMHIP_CHECK(hipMalloc(&work, sizeof(c32) * per_chunk * cb));
+    const long need = (long)KS * G * cb;  // templates per batch of chunks
+    if (need > ref_cap_ || (wsh_ && !ref_work_)) {
+      TOMHIP_CHECK(hipStreamSynchronize(s_));
+      (void)hipFree(ref_tt_);
+      (void)hipFree(ref_work_);
+      ref_tt_ = nullptr;
+      ref_work_ = nullptr;
+      ref_cap_ = 0;
+      TOMHIP_CHECK(hipMalloc(&ref_tt_, sizeof(float) * n3 * need));
+      if (wsh_) TOMHIP_CHECK(hipMalloc(&ref_work_, sizeof(c32) * n3 * need));
+      ref_cap_ = need;
+    }
+    float* tt = ref_tt_;
+    c32* work = ref_work_;
     TOMHIP_CHECK(hipMalloc(&cs, sizeof(float) * (long)nch * P));
     TOMHIP_CHECK(hipMalloc(&ca, sizeof(int32_t) * (long)nch * P));
     TOMHIP_CHECK(hipMalloc(&cp, sizeof(int32_t) * (long)nch * P));

@@ -146,8 +146,14 @@ def assign_bins(euler, bins) -> np.ndarray:
 def local_grid(center, half: float = 15.0, step: float = 5.0) -> np.ndarray:
     """[phi psi theta] grid center +- half in steps of `step` on each Euler
     angle (phi outermost, theta innermost, as tom_os3_angleList orders)."""
+    return local_grids(np.asarray(center, np.float64).reshape(1, 3), half, step)[:, 0]
+
+
+def local_grids(centers, half: float = 15.0, step: float = 5.0) -> np.ndarray:
+    """local_grid of every row of centers (P, 3), as one (A, P, 3) array."""
     off = np.arange(-half, half + 1e-9, step)
-    c = np.asarray(center, np.float64).ravel()
-    g = np.array([(c[0] + a, c[1] + b, c[2] + t) for a in off for b in off for t in off])
-    g[:, :2] = np.mod(g[:, :2], 360.0)
+    a, b, t = np.meshgrid(off, off, off, indexing="ij")  # phi outermost, theta innermost
+    d = np.stack([a.ravel(), b.ravel(), t.ravel()], 1)  # (A, 3)
+    g = np.asarray(centers, np.float64).reshape(1, -1, 3) + d[:, None, :]
+    g[..., :2] = np.mod(g[..., :2], 360.0)
     return g.astype(np.float32)

@@ -322,7 +322,7 @@ class Surrogate:
         voxels. Returns (score[P], euler[P, 3], pos[P, 3] x y z)."""
         t0 = time.perf_counter()
         ub, grp = np.unique(cand[:, 3].astype(np.int64), return_inverse=True)
-        grids = np.stack([orient.local_grid(self.bins[b], self.half, self.step) for b in ub], 1)  # (A, G, 3)
+        grids = orient.local_grids(self.bins[ub], self.half, self.step)  # (A, G, 3)
         c4 = np.concatenate([cand[:, :3].astype(np.int32), grp[:, None].astype(np.int32)], 1)
         sc, ai, pos = plan.refine(volume, grids, c4, centroid=self.centroid)
         eul = grids[ai, grp]
@@ -451,7 +451,7 @@ class PrunedSearch:
         grp = np.arange(P, dtype=np.int32)[:, None]
         sc = None
         for half, step in self.levels:
-            grids = np.stack([orient.local_grid(e, half, step) for e in eul], 1)  # (A, P, 3)
+            grids = orient.local_grids(eul, half, step)  # (A, P, 3)
             sc, ai, pos = plan.refine(volume, grids, np.concatenate([pos, grp], 1))
             eul = grids[ai, np.arange(P)]
         return sc, eul, pos
