@@ -2,22 +2,26 @@
 
     python -m tom_amd.train_surrogate --out tom_amd/weights/unet_c2.npz
 
+(the defaults reproduce the shipped weights: 48 tomograms, the 256-orientation
+coarse score map as the second input channel, targets at the picks).
+
 Training data: seeded synthetic tomograms (edge --size, --particles copies of
 the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
 missing wedge of a +-60 degree tilt series), each searched exhaustively on the
 GPU at 15 degrees; the picklist of the score map (tom_os3_returnPicklist
-semantics) above --label-threshold gives the particle targets (at the density
-centre p + R d of each pick: surrogate.template_centroid) and the best
-orientation there the bin targets (a Gaussian heat map of sigma --sigma voxels
-around each peak; bins within 4 voxels), and voxels within --offset-radius of
-a peak learn the vector to it (the candidates are moved by it); --dense adds targets from the
-exhaustive maps at every voxel (the best score mapped linearly from [0.1, 0.5]
-to [0, 1], and the bin of the best orientation wherever the score exceeds
-0.25), which trains a broader response that localises worse. The network's
-second input channel is the FLCF map of the rotationally averaged template
-(surrogate.aux_map). The template, mask and wedge weight are
-those of the 512x512x256 / 48^3 benchmark workload (seed 2), so the trained
-network proposes candidates for that search.
+semantics) above --label-threshold gives the particle targets (a Gaussian heat
+map of sigma --sigma voxels at each pick, or, with --targets centroid, at the
+density centre p + R d of each pick: surrogate.template_centroid) and the best
+orientation there the bin targets (bins within 4 voxels); voxels within
+--offset-radius of a target learn the vector to it (the candidates are moved
+by it). --dense adds targets from the exhaustive maps at every voxel (the best
+score mapped linearly from [0.1, 0.5] to [0, 1], and the bin of the best
+orientation wherever the score exceeds 0.25), which trains a broader response
+that localises worse. The network's second input channel (--aux) is the best
+score of a coarse scan over --coarse orientations (PrunedSearch.coarse) or the
+FLCF map of the rotationally averaged template (surrogate.aux_map). The
+template, mask and wedge weight are those of the 512x512x256 / 48^3 benchmark
+workload (seed 2), so the trained network proposes candidates for that search.
 """
 from __future__ import annotations
 
@@ -82,7 +86,7 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "unet_c2.npz"))
-    ap.add_argument("--tomos", type=int, default=24)
+    ap.add_argument("--tomos", type=int, default=48)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--particles", type=int, default=80)
     ap.add_argument("--snr-lo", type=float, default=0.05)
@@ -95,11 +99,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=100)
     ap.add_argument("--sigma", type=float, default=2.0, help="heat-map sigma around the exhaustive peaks (voxels)")
     ap.add_argument("--dense", action="store_true", help="add dense targets from the exhaustive score map")
-    ap.add_argument("--aux", choices=("sphere", "coarse"), default="sphere",
+    ap.add_argument("--aux", choices=("sphere", "coarse"), default="coarse",
                     help="second input channel: FLCF of the rotationally averaged template, or the best score of a "
                          "coarse scan over --coarse orientations")
     ap.add_argument("--coarse", type=int, default=256)
-    ap.add_argument("--targets", choices=("centroid", "peak"), default="centroid")
+    ap.add_argument("--targets", choices=("centroid", "peak"), default="peak")
     ap.add_argument("--offset-radius", type=float, default=10.0,
                     help="voxels within this distance of a peak learn the offset to it")
     args = ap.parse_args(argv)
