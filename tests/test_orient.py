@@ -82,6 +82,21 @@ def test_super_fibonacci_covers_so3():
     assert cover.max() < 0.8 * cover_rand.max()
 
 
+def test_local_grids_order_and_wrap():
+    """(A, P, 3) grids: phi outermost, theta innermost, phi and psi wrapped
+    into [0, 360), theta not wrapped (loops over the offsets)."""
+    rng = np.random.default_rng(7)
+    c = _rand_euler(rng, 9)
+    c[0] = [355.0, 2.0, 90.0]
+    g = O.local_grids(c, 15, 7.5)
+    off = [-15.0, -7.5, 0.0, 7.5, 15.0]
+    assert g.shape == (125, 9, 3)
+    for p in range(9):
+        ref = [((c[p][0] + a) % 360, (c[p][1] + b) % 360, c[p][2] + t) for a in off for b in off for t in off]
+        np.testing.assert_allclose(g[:, p], np.asarray(ref, np.float32), atol=1e-4)
+        np.testing.assert_array_equal(g[:, p], O.local_grid(c[p], 15, 7.5))
+
+
 def test_local_grid():
     g = O.local_grid([350, 10, 90], 15, 5)
     assert g.shape == (343, 3)
