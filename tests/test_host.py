@@ -41,3 +41,34 @@ def test_angle_list_wrap_range():
     vals = api._angle_list_1d(350, 20, 10)
     assert vals[0] == 351 % 360 and vals[-1] == (351 + 10 * (len(vals) - 1)) % 360
     assert len(vals) == len(range(351, 20 + 360, 10))
+
+
+def test_bench_one_gpu_only():
+    """bench.py measures one GPU: any other --gpus exits with status 2 and a
+    message, before touching a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_bench_byte_model_matches_the_unfused_reference_model():
+    """The per-launch algorithmic bytes of the three FLCF passes at C2 and the
+    unfused per-orientation model (64 N_c + 16 N) the roofline also reports."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    m = bench.byte_model(bench.CONFIGS["c2"], 16)
+    q = 256 * 512 * 256 * 8
+    assert m["passB"] == 2 * 16 * q == 8589934592
+    assert m["passA"] == 16 * 48 * 512 * 257 * 8 + 256 * 512 * 257 * 8 + 16 * q
+    assert m["passC"] == 16 * q + 512 * 512 * 256 * 4
+    X, Y, Z = bench.CONFIGS["c2"]["N"]
+    assert 64 * X * Y * (Z // 2 + 1) + 16 * X * Y * Z == 3238002688
