@@ -2,8 +2,9 @@
 
     python -m tom_amd.train_surrogate --out tom_amd/weights/unet_c2.npz
 
-(the defaults reproduce the shipped weights: 48 tomograms, the 256-orientation
-coarse score map as the second input channel, targets at the picks).
+(the defaults reproduce the shipped weights: 96 tomograms, 24,000 Adam steps,
+the 256-orientation coarse score map as the second input channel, targets at
+the picks with a heat-map sigma of 1.5 voxels).
 
 Training data: seeded synthetic tomograms (edge --size, --particles copies of
 the template phantom at uniform random orientations, SNR U(--snr-lo, --snr-hi),
@@ -86,18 +87,18 @@ def label_tomogram(api, torch, dev, T, M, W, tn, mu, sg, angles, size, P, snr, s
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "unet_c2.npz"))
-    ap.add_argument("--tomos", type=int, default=48)
+    ap.add_argument("--tomos", type=int, default=96)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--particles", type=int, default=80)
     ap.add_argument("--snr-lo", type=float, default=0.05)
     ap.add_argument("--snr-hi", type=float, default=0.3)
     ap.add_argument("--label-threshold", type=float, default=0.3)
-    ap.add_argument("--steps", type=int, default=16000)
+    ap.add_argument("--steps", type=int, default=24000)
     ap.add_argument("--crop", type=int, default=96)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--wpos", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=100)
-    ap.add_argument("--sigma", type=float, default=2.0, help="heat-map sigma around the exhaustive peaks (voxels)")
+    ap.add_argument("--sigma", type=float, default=1.5, help="heat-map sigma around the exhaustive peaks (voxels)")
     ap.add_argument("--dense", action="store_true", help="add dense targets from the exhaustive score map")
     ap.add_argument("--aux", choices=("sphere", "coarse"), default="coarse",
                     help="second input channel: FLCF of the rotationally averaged template, or the best score of a "
