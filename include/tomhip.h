@@ -152,17 +152,24 @@ int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], c
  *   templates, masks     packed per box (box k at offset sum_{j<k} B_j^3);
  *                        masks_cc likewise (peak-search masks in unshifted
  *                        correlation coordinates) or NULL for everywhere
+ *   wedge_ref_semi_deg[nbox]  WEDGE_REF: tom_wedge semi-angle of each box
+ *                        template's missing wedge, turned with the template and
+ *                        applied to the particle (corr.cpp:1086-1111); NULL or
+ *                        a negative / NaN entry: none
  *   particle_box[P]      box index of each particle; particles packed with
  *                        element offsets particle_offset[P]
- *   wedge_semi_deg[P]    tom_wedge semi-angle of each particle's missing wedge
- *                        (<= 0: none); SimpleWedge with cutoff (B-1)/2
+ *   wedge_semi_deg[P]    WEDGE_PART: semi-angle of each particle's missing
+ *                        wedge (negative / NaN: none; 0: only the cutoff
+ *                        sphere, as a scalar 0 does in the gateway);
+ *                        SimpleWedge with cutoff (B-1)/2
  *   angles[P][A][3]      [phi psi theta] degrees
  * Outputs: best_angle[P] (0-based, -1 none), peak[P][3] (0-based, unshifted),
  * ccc[P] (NaN when none). */
 int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
-                               const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
-                               const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
-                               const float* angles, int64_t A, int32_t* best_angle, int32_t* peak, float* ccc);
+                               const float* masks, const float* masks_cc, const float* wedge_ref_semi_deg, int64_t P,
+                               const int32_t* particle_box, const float* particles, const int64_t* particle_offset,
+                               const float* wedge_semi_deg, const float* angles, int64_t A, int32_t* best_angle,
+                               int32_t* peak, float* ccc);
 
 /* ---------------------------------------------------------------- picklist */
 /* tom_os3_returnPicklist(peaks, angles, job, n) without binning

@@ -24,7 +24,8 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
   apply_weight          Filtrans/tom_apply_weight_function.m:56-62 -> W=1 identity,
                         Parseval, linearity
   tom_norm_mean0_1std   Sptrans/tom_norm.m:83-119 + Analysis/tom_dev.m:55-60
-                        -> mean 0 / population std 1
+                        -> closed form (T - mean)/std_pop; constant -> 0
+                        (test_tom_norm_mean0_1std_closed_form)
   norm_under_mask       statistics/tom_os3_normUnderMask.m:1-8 -> direct sums
   local_stats           statistics/tom_os3_mean.m:44-63, tom_os3_std.m:43-68
                         -> brute-force direct sums (orc_flcf_direct)
@@ -38,11 +39,22 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
                         tom_peak.m:27-50 (value and spline position)
   find_max              Analysis/tom_peakinc.c:23-52 -> numpy argmax
   fourier               Filtrans/tom_fourier.m:42-50 -> the 4x4 KAT tom_fourier.m:15-22
-  norm_mask             src/tom/core/volume_fcn.cpp:168-247 -> direct sums
-  simple_wedge_half     src/tom/core/wedge.cpp:149-243 -> grid predicate
+  norm_mask             src/tom/core/volume_fcn.cpp:168-247 -> closed forms:
+                        ramp under a binary mask, (-1)^x pattern under a soft
+                        mask, constants (tests/test_subtomo.py)
+  simple_wedge          src/tom/core/wedge.cpp:149-243 (cutoff branch, optional
+                        rotation by the template angle, transform.c:99-157)
+                        -> hand-checked frequencies, k -> -k symmetry, float64
+                        definition; rotated: W(R^T k) in float64 away from
+                        boundaries, 45-degree sense check
   subtomo_score         src/tom/corr/corr.cpp:969-1400, correlation_handler.cpp:96-120
-                        -> planted copy scores 1; brute-force circular correlation
-  mask_cc_sphere        peak-search mask of the 'peak' handler -> closed form
+                        -> planted copy scores 1 (no wedge; particle wedge with
+                        the particle built from numpy FFTs); brute-force
+                        circular correlation; template wedge: swap symmetry with
+                        the particle-wedge branch and covariance under a
+                        90-degree turn, each with a planted-mistake control
+  mask_cc_sphere        CorrelationHandlerPeaks mask (init_spheremask + inverse
+                        fftshift) -> lattice-point counts of the ball
   psr_ring / psr        src/tom/os3/os3_correlation.cpp:183-262 -> direct sums
                         under the ring (test_psr_direct_sums)
   erase_borders, peakc, return_picklist
@@ -460,46 +472,123 @@ def norm_mask(v: np.ndarray, M: np.ndarray, boolean: bool):
     return out, var
 
 
-def simple_wedge_half(shape, semi_angle_deg: float):
-    """tom::SimpleWedge(angle, cutoff=(min(size)-1)/2) weight on the full
-    unshifted frequency grid (src/tom/core/wedge.cpp:149-243, cutoff branch;
-    float arithmetic): keep |k| <= cutoff and (|kz| < 1e-4 or
-    tan(angle) <= |kx|/|kz|). Returned for every (kx, ky, kz) index; the
-    predicate is symmetric, so any half-spectrum slice of it is the reference's."""
+# mex angle conversion of tom_mex_corr3d.cpp:247-249 (degrees times PI/180.)
+_MEX_PI = 3.141592653589793238512808959406186204433
+
+
+def _init_rotmat_zxz(phi, theta, psi):
+    """tom_transf_init_rotmat_3d(P, 3, 3, {2, 0, 2}, {phi, theta, psi})
+    (Oscar3/c++Picker/src/tom/core/transform.c:99-157) in double: starting
+    from the identity, each rotation updates the columns of every row."""
+    P = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]
+    import math
+    for axis, ang in ((2, phi), (0, theta), (2, psi)):
+        if ang - math.floor(ang / (_MEX_PI / 2.0)) * (2.0 * _MEX_PI) == 0.0:
+            continue
+        sa, ca = math.sin(ang), math.cos(ang)
+        i0, i1 = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[axis]
+        for r in P:
+            p0, p1 = r[i0], r[i1]
+            if axis == 1:
+                r[i0] = p0 * ca + p1 * sa
+                r[i1] = -p0 * sa + p1 * ca
+            else:
+                r[i0] = p0 * ca - p1 * sa
+                r[i1] = p0 * sa + p1 * ca
+    return P
+
+
+def simple_wedge(shape, semi_angle_deg: float, euler_deg=None):
+    """tom::SimpleWedge(angle, cutoff = (min(size)-1)/2), the wedge that
+    tom_mex_corr3d builds from a scalar WEDGE_REF / WEDGE_PART
+    (tom_mex_corr3d.cpp:334,410), optionally rotated by [phi psi theta]
+    degrees as corr3d_single does for the template's wedge (corr.cpp:1089,
+    SimpleWedge::rotate). Restates init_wedge_volume's cutoff branch
+    (src/tom/core/wedge.cpp:149-224) in its float arithmetic and order:
+    P = init_rotmat(z,x,z; phi,theta,psi) in double with the offset
+    -P.centre, cast to float; grid index x -> xs = (x + ceil(n/2)) mod n;
+    keep = |P(xs)| <= cutoff and (|z'| < 1e-4 or tan(angle) <= |x'|/|z'|).
+    The reference evaluates it on the r2c half (z index <= n/2); the full
+    unshifted grid returned here takes the other half from the Hermitian
+    partner, which is what the half-spectrum product means."""
     f = np.float32
-    cutoff = f((min(shape) - 1) // 2)
-    tan_a = f(np.tan(semi_angle_deg * np.pi / 180.0))
-    ax = [np.fft.fftfreq(s, 1.0 / s).astype(np.float32) for s in shape]
-    X, Y, Z = np.meshgrid(*ax, indexing="ij")
-    X = np.where(X == -(shape[0] // 2), f(shape[0] // 2), X) if shape[0] % 2 == 0 else X
-    r = np.sqrt((X * X + Y * Y + Z * Z).astype(np.float32))
-    az = np.abs(Z)
+    sx, sy, sz = shape
+    if euler_deg is None:
+        phi = psi = theta = 0.0
+    else:
+        phi, psi, theta = (float(a) * (_MEX_PI / 180.0) for a in euler_deg)
+    cx, cy, cz = (float(np.ceil((s - 1) / 2.0)) for s in shape)
+    Pl = _init_rotmat_zxz(phi, theta, psi)
+    off = [-(r[0] * cx + r[1] * cy + r[2] * cz) for r in Pl]
+    P = [[f(r[0]), f(r[1]), f(r[2]), f(o)] for r, o in zip(Pl, off)]
+    maxdist = np.sqrt(f(cx * cx + cy * cy + cz * cz))
+    tan_a = f(np.tan(semi_angle_deg * (_MEX_PI / 180.0)))
+    c0 = float((min(shape) - 1) // 2)
+    cutoff = f(c0) if (c0 > 0 and c0 < maxdist) else f(0)
+    ox, oy, oz = (s // 2 + s % 2 for s in shape)
+    xs = ((np.arange(sx) + ox) % sx).astype(np.float32)[:, None, None]
+    ys = ((np.arange(sy) + oy) % sy).astype(np.float32)[None, :, None]
+    zs = ((np.arange(sz // 2 + 1) + oz) % sz).astype(np.float32)[None, None, :]
+    xt = P[0][2] * zs + P[0][3]
+    yt = P[1][2] * zs + P[1][3]
+    zt = P[2][2] * zs + P[2][3]
+    xt = P[0][1] * ys + xt
+    yt = P[1][1] * ys + yt
+    zt = P[2][1] * ys + zt
+    x_ = P[0][0] * xs + xt
+    y_ = P[1][0] * xs + yt
+    z_ = np.abs(P[2][0] * xs + zt)
     with np.errstate(divide="ignore", invalid="ignore"):
-        keep = (r <= cutoff) & ((az < f(1e-4)) | (tan_a <= (np.abs(X) / az).astype(np.float32)))
-    return keep.astype(np.float64)
+        ratio = (np.abs(x_) / z_).astype(np.float32)
+        r = np.sqrt((x_ * x_ + y_ * y_) + z_ * z_).astype(np.float32)
+        if cutoff > 0:
+            half = (r <= cutoff) & ((z_ < f(1e-4)) | (tan_a <= ratio))
+        else:
+            half = (z_ < f(1e-4)) | (tan_a <= ratio)
+    full = np.zeros(shape, np.float64)
+    full[:, :, :sz // 2 + 1] = half
+    for z in range(sz // 2 + 1, sz):
+        full[:, :, z] = np.roll(half[::-1, ::-1, sz - z], 1, axis=(0, 1))
+    return full
 
 
-def subtomo_score(particle, template, M, mask_cc, semi_angle_deg, angles):
+def simple_wedge_half(shape, semi_angle_deg: float):
+    """The unrotated SimpleWedge weight on the full unshifted grid."""
+    return simple_wedge(shape, semi_angle_deg)
+
+
+def subtomo_score(particle, template, M, mask_cc, semi_angle_deg, angles, wedge_ref_deg=None):
     """One particle against one template over an angle list, the way
     corr3d_batch -> corr3d_single (reapply_mask_after_wedge = true) ->
     CorrelationHandlerPeaks do it (src/tom/corr/corr.cpp:969-1210, 1217-1400,
     src/tom/corr/correlation_handler.cpp:96-120), FP64:
       template -= mean(template); particle: norm_mask (soft), FFT, DC = 0
       per angle: t = rotate(T, a); norm_mask(t); Tf = FFT(t), DC = 0
-                 particle wedge (semi_angle > 0): Tw = Tf.W, DC = 0,
-                 tw = c2r(Tw) (unnormalised), norm_mask_bool(tw) -> var_t,
-                 Tw = FFT(tw)
-                 C = conj(Tw).P, DC = 0; c = c2r(C)/(sqrt(var_p var_t) numel^2)
+                 template wedge (wedge_ref_deg not None, corr.cpp:1086-1111):
+                   Pw = P . W_ref rotated by a, DC = 0, pw = c2r(Pw)
+                   (unnormalised), norm_mask_bool(pw) -> var_p, Pw = FFT(pw)
+                 particle wedge (semi_angle_deg not None, corr.cpp:1146-1171):
+                   Tw = Tf . W_p, DC = 0, tw = c2r(Tw) (unnormalised),
+                   norm_mask_bool(tw) -> var_t, Tw = FFT(tw)
+                 C = conj(Tw).Pw, DC = 0; c = c2r(C)/(sqrt(var_p var_t) numel^2)
                  peak of c over mask_cc != 0 (first in x-fastest order)
       keep the strictly better value (first angle wins).
+    A wedge semi-angle of None (or negative / NaN) is NoWedge ([] in the
+    gateway); 0 is SimpleWedge(0), i.e. only the cutoff sphere.
     Returns (angle index or -1, peak [x y z] unshifted 0-based, value, maps)."""
     n = particle.size
     T = np.asarray(template, np.float64)
     T = T - T.mean()
-    p, var_p = norm_mask(particle, M, False)
+    p, var_p0 = norm_mask(particle, M, False)
     P = np.fft.fftn(p)
     P.flat[0] = 0
-    W = simple_wedge_half(particle.shape, semi_angle_deg) if semi_angle_deg > 0 else None
+
+    def active(a):
+        return a is not None and np.isfinite(a) and a >= 0
+
+    W = simple_wedge(particle.shape, semi_angle_deg) if active(semi_angle_deg) else None
+    if mask_cc is None:
+        mask_cc = np.ones(particle.shape)
     best = (-1, np.zeros(3, int), 0.0)
     vals = []
     for ia, a in enumerate(np.asarray(angles).reshape(-1, 3)):
@@ -507,6 +596,15 @@ def subtomo_score(particle, template, M, mask_cc, semi_angle_deg, angles):
         t, var_t = norm_mask(t, M, False)
         Tf = np.fft.fftn(t)
         Tf.flat[0] = 0
+        var_p = var_p0
+        if active(wedge_ref_deg):
+            Pw = P * simple_wedge(particle.shape, wedge_ref_deg, a)
+            Pw.flat[0] = 0
+            pw = np.real(np.fft.ifftn(Pw)) * n
+            pw, var_p = norm_mask(pw, M, True)
+            Pw = np.fft.fftn(pw)
+        else:
+            Pw = P
         if W is not None:
             Tw = Tf * W
             Tw.flat[0] = 0
@@ -515,7 +613,7 @@ def subtomo_score(particle, template, M, mask_cc, semi_angle_deg, angles):
             Tw = np.fft.fftn(tw)
         else:
             Tw = Tf
-        C = np.conj(Tw) * P
+        C = np.conj(Tw) * Pw
         C.flat[0] = 0
         c = np.real(np.fft.ifftn(C)) * n / (np.sqrt(var_p * var_t) * n * n)
         sel = np.asfortranarray(mask_cc != 0)

@@ -37,6 +37,26 @@ def test_em_read_types_and_endianness(tmp_path, machine, code, dt):
     np.testing.assert_array_equal(w, vals.astype(np.float32))
 
 
+def test_em_read_refuses_malformed_headers(tmp_path):
+    """dims of 0xFFFFFFFF (their product overflows 64 bits) and a header that
+    claims more data than the file holds are refused with -1 and a message,
+    not a crash or an exception across the C ABI."""
+    import ctypes
+
+    from tom_amd import api
+    lib = api.load()
+    out = np.zeros(64, np.float32)
+    for dims, extra in (([0xFFFFFFFF] * 3, 16), ([4, 4, 4], 100), ([4, 4, 4], 252)):
+        hdr = bytearray(512)
+        hdr[0], hdr[3] = 6, 5
+        hdr[4:16] = np.asarray(dims, "<u4").tobytes()
+        path = str(tmp_path / "bad.em")
+        open(path, "wb").write(bytes(hdr) + bytes(extra))
+        rc = lib.tomhip_em_read_f32(path.encode(), ctypes.c_void_p(out.ctypes.data), out.size, None)
+        assert rc == -1, dims
+        assert lib.tomhip_em_last_error()
+
+
 @pytest.mark.gpu
 def test_picklist_matches_oracle(oracle):
     from tom_amd import api

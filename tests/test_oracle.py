@@ -147,6 +147,38 @@ def test_fourier_kat(oracle):
 
 
 # ---------------------------------------------------------------- masks/weights
+def test_tom_norm_mean0_1std_closed_form(oracle):
+    """tom_norm(1e5*(T+1), 'mean0+1std') (Sptrans/tom_norm.m:83-119 with
+    tom_dev's population std, Analysis/tom_dev.m:55-60) is affine-invariant:
+    it equals (T - mean T) / std_pop(T). The sample std (n - 1) fails it;
+    a constant template gives zeros (the st == 0 branch)."""
+    rng = np.random.default_rng(12)
+    T = (3.0 * rng.standard_normal((10, 9, 8)) + 0.4).astype(np.float32)
+    Td = T.astype(np.float64)
+    out = oracle.tom_norm_mean0_1std(T)
+    np.testing.assert_allclose(out, (Td - Td.mean()) / Td.std(), atol=1e-9)
+    assert abs(out.mean()) < 1e-12 and abs(out.std() - 1) < 1e-12
+    assert np.abs(out - (Td - Td.mean()) / Td.std(ddof=1)).max() > 1e-4
+    np.testing.assert_array_equal(oracle.tom_norm_mean0_1std(np.full((4, 4, 4), 2.5)), 0.0)
+
+
+def test_mask_cc_sphere_lattice_counts(oracle):
+    """The peak-search mask (init_spheremask radius r, sigma 0, then the
+    inverse fftshift of CorrelationHandlerPeaks, correlation_handler.cpp:
+    65-80): shifts s with |s| <= r around the origin of the unshifted
+    correlation volume. Its size is the number of integer points in the ball
+    of radius r (1, 7, 33, 123, 257, 515 for r = 0..5; 19 and 27 for
+    r^2 = 2 and 3), it holds the zero shift and wraps to negative shifts."""
+    n = 16
+    for r, cnt in ((0, 1), (1, 7), (2, 33), (3, 123), (4, 257), (5, 515), (np.sqrt(2), 19), (np.sqrt(3), 27)):
+        m = oracle.mask_cc_sphere((n, n, n), r + 1e-9)
+        assert int(m.sum()) == cnt, r
+    m = oracle.mask_cc_sphere((n, n, n), 3)
+    assert m[0, 0, 0] == 1 and m[n - 3, 0, 0] == 1 and m[3, 0, 0] == 1 and m[4, 0, 0] == 0
+    assert m[n // 2, n // 2, n // 2] == 0
+    np.testing.assert_array_equal(m, np.roll(m[::-1, ::-1, ::-1], 1, axis=(0, 1, 2)))
+
+
 def test_masks_closed_form(oracle):
     m, cnt = oracle.sphere_mask_os3(24)
     x = np.arange(24) - 12
@@ -365,3 +397,31 @@ def test_tom_bin_closed_forms(oracle):
     for (i, j, k) in ((0, 1, 0), (1, 0, 0), (1, 1, 0), (0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 1)):
         acc = np.float32(acc + v[i, j, k])
     assert oracle.tom_bin(v)[0, 0, 0] == np.float32(acc / np.float32(8))
+
+
+def test_cpu_fp32_baseline_matches_oracle(oracle):
+    """The FP32 CPU baseline (the reference's loop in single precision) agrees
+    with the FP64 oracle's find_template to FP32 rounding on a small case,
+    best angles identical where the margin exceeds 1e-3."""
+    from oracle.cpu_fp32 import CpuFlcfFP32
+    rng = np.random.default_rng(4)
+    n, N = 12, (32, 24, 28)
+    x = np.arange(n) - n // 2
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    T = (np.exp(-((X - 2) ** 2 + Y ** 2 + (Z + 1) ** 2) / 4.0) + 0.6 * np.exp(-((X + 2) ** 2 + (Y - 2) ** 2) / 3.0))
+    T = T.astype(np.float32)
+    V = rng.standard_normal(N).astype(np.float32)
+    V[6:18, 5:17, 8:20] += 3 * oracle.rotate(T, [30, 60, 90])
+    M = oracle.sphere_mask_binary(n, 5)
+    W = oracle.wedge((n, n, n), 30) * oracle.bandpass_weight((n, n, n), 1, 5)
+    angles = np.array([[0, 0, 0], [30, 60, 90], [90, 30, 45]], float)
+    ref, ref_ang, allm = oracle.find_template(V, T, M, angles, W=W, return_all=True)
+    cpu = CpuFlcfFP32(V, T, M, W)
+    for a in angles:
+        cpu.step(a)
+    got = cpu.best.numpy()
+    assert np.abs(got - ref).max() < 2e-3
+    srt = np.sort(allm, axis=0)
+    sel = (srt[-1] - srt[-2]) > 1e-3
+    assert np.all(cpu.arg.numpy()[sel] == ref_ang[sel])
+    assert cpu.arg.numpy()[12, 11, 14] == 2

@@ -195,7 +195,9 @@ def test_unet_backward_matches_restatement(api):
         cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
         rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-300)
         print(f"  layer {li}: cos {cos:.5f} rel {rel:.3e}", flush=True)
-        assert cos >= 0.995 and rel <= 0.1, li
+        # measured on gfx950: rel <= 2.2e-3 in every layer (BF16 operands,
+        # FP32 sums in a different order than the restatement)
+        assert cos >= 0.9999 and rel <= 1e-2, li
 
 
 @pytest.mark.gpu

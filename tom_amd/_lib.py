@@ -69,8 +69,8 @@ SIGNATURES = {
     "tomhip_em_read_f32": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int64, vp]),
     "tomhip_em_write_f32": (ctypes.c_int, [ctypes.c_char_p, vp, vp, vp]),
     "tomhip_em_last_error": (ctypes.c_char_p, []),
-    "tomhip_subtomo_score_batch": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp,
-                                                  vp, ctypes.c_int64, vp, vp, vp]),
+    "tomhip_subtomo_score_batch": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp, vp, vp,
+                                                  vp, vp, ctypes.c_int64, vp, vp, vp]),
 }
 
 _lib = None

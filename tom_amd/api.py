@@ -419,10 +419,13 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
     corr3d_batch semantics, one persistent launch).
 
     boxes: list of dicts {"template": (B,B,B), "mask": (B,B,B), "mask_cc":
-    (B,B,B) or None}, one per box edge B in {32, 48, 64} (MATLAB indexing).
+    (B,B,B) or None, "wedge_ref": semi-angle of the template's missing wedge in
+    degrees or None}, one per box edge B in {32, 48, 64} (MATLAB indexing).
     particles: list of (B,B,B) arrays, or one contiguous float32 torch tensor
     holding them packed in order (x fastest); particle_box[p] indexes boxes;
-    wedge_semi_deg[p] (<= 0: no wedge); angles: (P, A, 3) degrees.
+    wedge_semi_deg[p]: the particle's wedge semi-angle (negative or NaN: none,
+    0: the cutoff sphere only, as tom_mex_corr3d treats a scalar 0);
+    angles: (P, A, 3) degrees.
     Returns (best angle index 0-based (-1 none), peak (P,3) 0-based unshifted,
     ccc (P,))."""
     ctx = ctx or context()
@@ -435,6 +438,9 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
                               for b in boxes])
     else:
         Mcc = None
+    wref = None
+    if any(b.get("wedge_ref") is not None for b in boxes):
+        wref = np.asarray([np.nan if b.get("wedge_ref") is None else float(b["wedge_ref"]) for b in boxes], np.float32)
     pb = np.ascontiguousarray(particle_box, np.int32)
     P = pb.shape[0]
     sizes = [int(edges[i]) ** 3 for i in pb]
@@ -455,8 +461,9 @@ def subtomo_score_batch(boxes, particles, particle_box, wedge_semi_deg, angles, 
     peak = np.zeros((P, 3), np.int32)
     ccc = np.zeros(P, np.float32)
     ctx.check(ctx.lib.tomhip_subtomo_score_batch(ctx.h, len(boxes), _p(edges), _p(T), _p(M),
-                                                 None if Mcc is None else _p(Mcc), P, _p(pb), _ptr(packed), _p(off),
-                                                 _p(semi), _p(ang), A, _p(best), _p(peak), _p(ccc)))
+                                                 None if Mcc is None else _p(Mcc),
+                                                 None if wref is None else _p(wref), P, _p(pb), _ptr(packed),
+                                                 _p(off), _p(semi), _p(ang), A, _p(best), _p(peak), _p(ccc)))
     return best, peak, ccc
 
 

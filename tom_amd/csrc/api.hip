@@ -135,7 +135,9 @@ int tomhip_init(int device, tomhip_ctx** out) {
     TOMHIP_CHECK(hipSetDevice(device));
     auto c = std::make_unique<tomhip_ctx>();
     c->device = device;
-    TOMHIP_CHECK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    // a blocking stream: ordered with work on the null stream (torch's default
+    // stream), so caller buffers written there are complete before we read them
+    TOMHIP_CHECK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
     c->stream = c->own;
     c->tw = std::make_unique<Twiddles>();
     *out = c.release();
@@ -363,6 +365,7 @@ int tomhip_search_set_template(tomhip_search* h, const float* tmpl, const float*
   return guard(ctx, [&] {
     TOMHIP_REQUIRE(h && tmpl, ERR_WRONG_INPUT, "null pointer");
     Search& S = *h->s;
+    S.bind_stream(ctx->stream);
     const int64_t n[3] = {S.nx, S.ny, S.nz};
     const long n3 = n[0] * n[1] * n[2];
     hipStream_t s = ctx->stream;
@@ -386,6 +389,7 @@ int tomhip_search_run(tomhip_search* h, const float* euler, int64_t count, int64
     TOMHIP_REQUIRE(h && (euler || count == 0), ERR_WRONG_INPUT, "null pointer");
     TOMHIP_REQUIRE(count >= 0, ERR_WRONG_INPUT, "negative count");
     std::vector<float> rm = rotmats(euler, count);
+    h->s->bind_stream(ctx->stream);
     h->s->run(rm.data(), count, first_index);
   });
 }
@@ -394,6 +398,7 @@ int tomhip_search_finalize(tomhip_search* h, float* score, int32_t* angle) {
   tomhip_ctx* ctx = h ? h->ctx : nullptr;
   return guard(ctx, [&] {
     TOMHIP_REQUIRE(h && score && angle, ERR_WRONG_INPUT, "null pointer");
+    h->s->bind_stream(ctx->stream);
     const long N = h->s->N;
     hipStream_t s = ctx->stream;
     DevBuf sc(s, score, sizeof(float) * N, false, true);
@@ -461,6 +466,7 @@ int tomhip_search_refine(tomhip_search* h, const float* volume, const float* eul
     TOMHIP_REQUIRE(G >= 1 && A >= 1 && P >= 1 && P < (1L << 30) && (long)G * A <= (1L << 24), ERR_WRONG_INPUT,
                    "bad refinement sizes");
     Search& S = *h->s;
+    S.bind_stream(ctx->stream);
     hipStream_t s = ctx->stream;
     std::vector<float> e((size_t)G * A * 3);
     TOMHIP_CHECK(hipMemcpyAsync(e.data(), euler, sizeof(float) * e.size(), hipMemcpyDefault, s));
@@ -796,9 +802,10 @@ int tomhip_snap_maxima_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3]
 }
 
 int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edge, const float* templates,
-                               const float* masks, const float* masks_cc, int64_t P, const int32_t* particle_box,
-                               const float* particles, const int64_t* particle_offset, const float* wedge_semi_deg,
-                               const float* angles, int64_t A, int32_t* best_angle, int32_t* peak, float* ccc) {
+                               const float* masks, const float* masks_cc, const float* wedge_ref_semi_deg, int64_t P,
+                               const int32_t* particle_box, const float* particles, const int64_t* particle_offset,
+                               const float* wedge_semi_deg, const float* angles, int64_t A, int32_t* best_angle,
+                               int32_t* peak, float* ccc) {
   return guard(ctx, [&] {
     TOMHIP_REQUIRE(ctx && box_edge && templates && masks && particle_box && particles && particle_offset &&
                        wedge_semi_deg && angles && best_angle && peak && ccc,
@@ -865,13 +872,32 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
     for (int64_t i = 0; i < P; ++i) order[i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t b) { return edges[pb[a]] > edges[pb[b]]; });
+    // SimpleWedge(angle * PI / 180., cutoff) of a scalar wedge argument
+    // (tom_mex_corr3d.cpp:334,410); a negative or NaN angle means no wedge
+    // ([] in the gateway), 0 keeps only the cutoff sphere as the reference does
+    const double kPi = 3.141592653589793238512808959406186204433;
+    auto tan_of = [&](float deg) { return deg >= 0.f ? (float)tan((double)deg * kPi / 180.) : -1.f; };
     std::vector<float> tanw(P);
-    const double kPi = 3.14159265358979323846264338327950288419716939937510;
-    for (int64_t i = 0; i < P; ++i) tanw[i] = semi[i] > 0 ? (float)tan(semi[i] * kPi / 180.0) : -1.f;
+    for (int64_t i = 0; i < P; ++i) tanw[i] = tan_of(semi[i]);
+    std::vector<float> tanr(nbox, -1.f);
+    if (wedge_ref_semi_deg) {
+      host_copy(wedge_ref_semi_deg, sizeof(float) * nbox, tanr.data());
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+      for (int k = 0; k < nbox; ++k) tanr[k] = tan_of(tanr[k]);
+    }
+    bool any_ref = false;
+    for (int k = 0; k < nbox; ++k) any_ref = any_ref || tanr[k] >= 0.f;
     std::vector<float> ang((size_t)P * A * 3);
     host_copy(angles, sizeof(float) * ang.size(), ang.data());
     TOMHIP_CHECK(hipStreamSynchronize(s));
     std::vector<float> rms = rotmats(ang.data(), P * A);
+    std::vector<float> wmap;
+    if (any_ref) {
+      wmap.resize((size_t)P * A * 12);
+      for (int64_t i = 0; i < P; ++i)
+        for (int64_t a = 0; a < A; ++a)
+          simple_wedge_map(&ang[((size_t)i * A + a) * 3], edges[pb[i]], &wmap[((size_t)i * A + a) * 12]);
+    }
     std::vector<long> poffl(poff.begin(), poff.end());
     std::vector<void*> tmp;
     auto up = [&](const void* h, size_t bytes) {
@@ -887,8 +913,10 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
       const long* d_poff = (const long*)up(poffl.data(), sizeof(long) * P);
       const float* d_tan = (const float*)up(tanw.data(), sizeof(float) * P);
       const float* d_rm = (const float*)up(rms.data(), sizeof(float) * rms.size());
+      const float* d_tanr = (const float*)up(tanr.data(), sizeof(float) * nbox);
+      const float* d_wmap = any_ref ? (const float*)up(wmap.data(), sizeof(float) * wmap.size()) : nullptr;
       subtomo_run(*ctx->tw, s, boxes, dT.get<float>(), toff.data(), P, d_pb, d_order, dP.get<float>(), d_poff, d_tan,
-                  d_rm, A, oA.get<int32_t>(), oP.get<int32_t>(), oC.get<float>());
+                  d_rm, d_tanr, d_wmap, A, oA.get<int32_t>(), oP.get<int32_t>(), oC.get<float>());
       oA.commit();
       oP.commit();
       oC.commit();

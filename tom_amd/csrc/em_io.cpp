@@ -10,6 +10,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -79,7 +81,20 @@ int tomhip_em_read_header(const char* path, tomhip_em_header* hdr) {
   return rc;
 }
 
+static int em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em_header* hdr_out);
+
 int tomhip_em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em_header* hdr_out) {
+  // no C++ exception crosses the C ABI
+  try {
+    return em_read_f32(path, out, capacity, hdr_out);
+  } catch (const std::bad_alloc&) {
+    return fail(-2, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(-1, e.what());
+  }
+}
+
+static int em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em_header* hdr_out) {
   if (!path || !out) return fail(-1, "null pointer");
   FILE* f = fopen(path, "rb");
   if (!f) return fail(-1, std::string("cannot open ") + path);
@@ -94,10 +109,27 @@ int tomhip_em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em
     fclose(f);
     return fail(-3, "EM data type " + std::to_string((int)h.type) + " is not a real type");
   }
-  const int64_t n = (int64_t)h.dims[0] * h.dims[1] * h.dims[2];
-  if (n > capacity) {
+  // element count without overflow (three 32-bit dims), checked against
+  // the caller's buffer and against the bytes the file holds
+  const unsigned __int128 n128 = (unsigned __int128)h.dims[0] * h.dims[1] * h.dims[2];
+  if (n128 > (unsigned __int128)(capacity < 0 ? 0 : capacity)) {
     fclose(f);
-    return fail(-1, "output buffer too small");
+    return fail(-1, "output buffer too small for the header's dims");
+  }
+  const int64_t n = (int64_t)n128;
+  long here = ftell(f);
+  if (here < 0 || fseek(f, 0, SEEK_END) != 0) {
+    fclose(f);
+    return fail(-1, "cannot size the EM file");
+  }
+  const long end = ftell(f);
+  if (end < 0 || fseek(f, here, SEEK_SET) != 0) {
+    fclose(f);
+    return fail(-1, "cannot size the EM file");
+  }
+  if ((unsigned __int128)(end - here) < n128 * (unsigned)es) {
+    fclose(f);
+    return fail(-1, "EM file shorter than its header says");
   }
   std::vector<uint8_t> raw((size_t)n * es);
   if (fread(raw.data(), es, (size_t)n, f) != (size_t)n) {
@@ -120,7 +152,19 @@ int tomhip_em_read_f32(const char* path, float* out, int64_t capacity, tomhip_em
   return 0;
 }
 
+static int em_write_f32(const char* path, const float* data, const int64_t dims[3], const tomhip_em_header* hdr);
+
 int tomhip_em_write_f32(const char* path, const float* data, const int64_t dims[3], const tomhip_em_header* hdr) {
+  try {
+    return em_write_f32(path, data, dims, hdr);
+  } catch (const std::bad_alloc&) {
+    return fail(-2, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(-1, e.what());
+  }
+}
+
+static int em_write_f32(const char* path, const float* data, const int64_t dims[3], const tomhip_em_header* hdr) {
   if (!path || !data || !dims) return fail(-1, "null pointer");
   tomhip_em_header h;
   std::memset(&h, 0, sizeof(h));
@@ -131,6 +175,8 @@ int tomhip_em_write_f32(const char* path, const float* data, const int64_t dims[
     if (dims[d] < 1 || dims[d] > 0xffffffffLL) return fail(-1, "bad dims");
     h.dims[d] = (uint32_t)dims[d];
   }
+  if ((unsigned __int128)dims[0] * dims[1] * dims[2] > (unsigned __int128)(INT64_MAX / 8))
+    return fail(-1, "bad dims");
   const int64_t n = dims[0] * dims[1] * dims[2];
   FILE* f = fopen(path, "wb");
   if (!f) return fail(-1, std::string("cannot create ") + path);

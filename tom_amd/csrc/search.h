@@ -71,6 +71,9 @@ class Search {
   void set_zchunk(int planes);
   bool tiles_ok(int tile_a, int tile_b) const;
 
+  // Queue the plan's later work on stream s, after what it has queued so far.
+  void bind_stream(hipStream_t s);
+
   void set_profile(bool on) { profile_ = on; }
   // Replay full angle batches through one captured hipGraph (default on;
   // profiling runs eager so every launch can be timed).
@@ -89,6 +92,7 @@ class Search {
 
  private:
   void precompute_stats(const float* volume);
+  void release();
   void run_batch(const float* rm_dev, int ka, long first_index, const long* desc);
   void rotate_templates(const float* rm_dev, int count, float* tout, const long* desc, long rm_step);
   void box_filter(int count, float* t, c32* work);
@@ -102,6 +106,7 @@ class Search {
 
   Twiddles& tw_;
   hipStream_t s_;
+  hipEvent_t swap_ev_ = nullptr;
   int KA_;
   bool profile_ = false;
   KernelTimes times_;

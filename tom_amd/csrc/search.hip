@@ -827,27 +827,8 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
   TOMHIP_REQUIRE(mode == MODE_FLCF || mode == MODE_XCF, ERR_WRONG_INPUT, "unknown mode");
   TOMHIP_REQUIRE(mode == MODE_XCF || mask != nullptr, ERR_WRONG_INPUT, "FLCF needs a mask");
   const long n3 = (long)nx * ny * nz;
-
-  TOMHIP_CHECK(hipMalloc(&FI_, sizeof(c32) * (long)Z * Y * Xh));
-  TOMHIP_CHECK(hipMalloc(&FIn_, sizeof(c32) * (long)Y * Z));
-  TOMHIP_CHECK(hipMalloc(&maxc_, sizeof(float) * N));
-  TOMHIP_CHECK(hipMalloc(&argm_, sizeof(int32_t) * N));
-  TOMHIP_CHECK(hipMalloc(&tmpl_, sizeof(float) * n3));
-  TOMHIP_CHECK(hipMalloc(&t1_, sizeof(float) * n3 * KA_));
-  TOMHIP_CHECK(hipMalloc(&tbox_, sizeof(c32) * n3 * KA_));
-  TOMHIP_CHECK(hipMalloc(&Ty_, sizeof(c32) * (long)KA_ * nz * Y * Xh));
-  TOMHIP_CHECK(hipMalloc(&Tyn_, sizeof(c32) * (long)KA_ * nz * Y));
-  TOMHIP_CHECK(hipMalloc(&Q_, sizeof(c32) * (long)KA_ * Z * Y * Xh));
   if (mode == MODE_FLCF) {
-    TOMHIP_CHECK(hipMalloc(&mu_, sizeof(double) * N));
-    TOMHIP_CHECK(hipMalloc(&sig_, sizeof(double) * N));
-    float* mk_ = nullptr;
-    TOMHIP_CHECK(hipMalloc(&mk_, sizeof(float) * n3 * 2));
-    TOMHIP_CHECK(hipMemcpyAsync(mk_, mask, sizeof(float) * n3, hipMemcpyDeviceToDevice, s_));
-    hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(NT), 0, s_, mk_, mk_ + n3, n3);
-    TOMHIP_CHECK(hipGetLastError());
-    mask_ = mk_;
-    // m = #{M > 0}
+    // m = #{M > 0}, validated before anything is allocated
     std::vector<float> hm(n3);
     TOMHIP_CHECK(hipMemcpyAsync(hm.data(), mask, sizeof(float) * n3, hipMemcpyDeviceToHost, s_));
     TOMHIP_CHECK(hipStreamSynchronize(s_));
@@ -861,10 +842,48 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     m = (double)cnt;
     mask_sum_ = msum;
   }
-  precompute_stats(volume);
+  try {
+    TOMHIP_CHECK(hipMalloc(&FI_, sizeof(c32) * (long)Z * Y * Xh));
+    TOMHIP_CHECK(hipMalloc(&FIn_, sizeof(c32) * (long)Y * Z));
+    TOMHIP_CHECK(hipMalloc(&maxc_, sizeof(float) * N));
+    TOMHIP_CHECK(hipMalloc(&argm_, sizeof(int32_t) * N));
+    TOMHIP_CHECK(hipMalloc(&tmpl_, sizeof(float) * n3));
+    TOMHIP_CHECK(hipMalloc(&t1_, sizeof(float) * n3 * KA_));
+    TOMHIP_CHECK(hipMalloc(&tbox_, sizeof(c32) * n3 * KA_));
+    TOMHIP_CHECK(hipMalloc(&Ty_, sizeof(c32) * (long)KA_ * nz * Y * Xh));
+    TOMHIP_CHECK(hipMalloc(&Tyn_, sizeof(c32) * (long)KA_ * nz * Y));
+    TOMHIP_CHECK(hipMalloc(&Q_, sizeof(c32) * (long)KA_ * Z * Y * Xh));
+    if (mode == MODE_FLCF) {
+      TOMHIP_CHECK(hipMalloc(&mu_, sizeof(double) * N));
+      TOMHIP_CHECK(hipMalloc(&sig_, sizeof(double) * N));
+      float* mk_ = nullptr;
+      TOMHIP_CHECK(hipMalloc(&mk_, sizeof(float) * n3 * 2));
+      mask_ = mk_;
+      TOMHIP_CHECK(hipMemcpyAsync(mk_, mask, sizeof(float) * n3, hipMemcpyDeviceToDevice, s_));
+      hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(NT), 0, s_, mk_, mk_ + n3, n3);
+      TOMHIP_CHECK(hipGetLastError());
+    }
+    precompute_stats(volume);
+  } catch (...) {
+    // a constructor that throws runs no destructor: free what was allocated
+    (void)hipStreamSynchronize(s_);
+    release();
+    throw;
+  }
 }
 
-Search::~Search() {
+void Search::bind_stream(hipStream_t s) {
+  if (s == s_) return;
+  // work already queued for this plan on its old stream comes first
+  if (!swap_ev_) TOMHIP_CHECK(hipEventCreateWithFlags(&swap_ev_, hipEventDisableTiming));
+  TOMHIP_CHECK(hipEventRecord(swap_ev_, s_));
+  TOMHIP_CHECK(hipStreamWaitEvent(s, swap_ev_, 0));
+  s_ = s;
+}
+
+Search::~Search() { release(); }
+
+void Search::release() {
   (void)hipFree(FI_);
   (void)hipFree(FIn_);
   (void)hipFree(maxc_);
@@ -882,12 +901,23 @@ Search::~Search() {
   (void)hipFree(ref_tt_);
   (void)hipFree(ref_work_);
   (void)hipFree(const_cast<float*>(mask_));
+  FI_ = FIn_ = nullptr;
+  maxc_ = nullptr;
+  argm_ = nullptr;
+  tmpl_ = t1_ = wsh_ = rm_ = ref_tt_ = nullptr;
+  tbox_ = Ty_ = Tyn_ = Q_ = ref_work_ = nullptr;
+  mu_ = sig_ = nullptr;
+  mask_ = nullptr;
   drop_graph();
   (void)hipFree(desc_);
+  desc_ = nullptr;
   for (auto& e : evpool_) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
   }
+  evpool_.clear();
+  if (swap_ev_) (void)hipEventDestroy(swap_ev_);
+  swap_ev_ = nullptr;
 }
 
 namespace {

@@ -13,9 +13,13 @@
 //   t' = norm_mask(t, M)                    (volume_fcn.cpp:168-247, non-boolean)
 //   Tf = FFT(t'), DC = 0; Tw = Tf . W_p, DC = 0       (SimpleWedge, cutoff (B-1)/2)
 //   tw = c2r(Tw) (unnormalised); tw' = norm_mask_bool(tw, M) -> var_t; Tw = FFT(tw')
-//   C  = conj(Tw) . P_p, DC = 0;  c = c2r(C) / (sqrt(var_p var_t) B^6)
+//   template wedge W_T of box B, turned with the template (corr.cpp:1086-1111):
+//   Pw = P_p . W_T(a), DC = 0; pw = c2r(Pw); pw' = norm_mask_bool(pw, M) -> var_p;
+//   Pw = FFT(pw')
+//   C  = conj(Tw) . Pw, DC = 0;  c = c2r(C) / (sqrt(var_p var_t) B^6)
 //   peak of c over mask_cc != 0, first in x-fastest order; keep strictly better
-// Without a wedge Tw = Tf and var_t is norm_mask's variance of t'.
+// Without a particle wedge Tw = Tf and var_t is norm_mask's variance of t';
+// without a template wedge Pw = P_p and var_p is the particle's norm_mask variance.
 //
 // Design. One workgroup owns one particle at a time (popped from an atomic
 // queue, largest boxes first) and runs all its orientations; every 3D FFT goes
@@ -148,6 +152,64 @@ __device__ void zpass(c32* H, c32* OUT, c32* buf, const c32* tw, const MID& mid)
   }
 }
 
+// z-line pass over a spectrum that is already z-transformed: mid(kx, ky, kz,
+// v, idx) on SRC, then the inverse z FFT into DST (which may alias SRC).
+template <int B, class MID>
+__device__ void zpass_inv(const c32* SRC, c32* DST, c32* buf, const c32* tw, const MID& mid) {
+  using G = Geo<B>;
+  constexpr int NCOL = (int)G::PL;
+  for (int c0 = 0; c0 < NCOL; c0 += NC) {
+    const int nc = (NCOL - c0 < NC) ? NCOL - c0 : NC;
+    for (int i = threadIdx.x; i < B * NC; i += NTS) {
+      const int kz = i / NC, c = i % NC;
+      if (c < nc) {
+        const int col = c0 + c;
+        const long idx = (long)kz * G::PL + col;
+        buf[i] = mid(col % G::XH1, col / G::XH1, kz, SRC[idx], idx);
+      } else {
+        buf[i] = mk(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    lds_fft<float, B, NC, NTS, 1, Inter<NC>>(buf, tw);
+    for (int i = threadIdx.x; i < B * NC; i += NTS) {
+      const int z = i / NC, c = i % NC;
+      if (c < nc) DST[(long)z * G::PL + c0 + c] = buf[i];
+    }
+    __syncthreads();
+  }
+}
+
+// SimpleWedge turned with the template (src/tom/core/wedge.cpp:149-224,
+// cutoff branch): the reference evaluates its r2c half (z index <= B/2) at
+// grid index (x, y, z) -> (xs, ys, zs) = ((x + B/2) mod B, ...) through the
+// float affine map P (3 x 4, row-major, rotation and -P.centre), in this
+// order of float operations. This layout stores kx <= B/2 for every kz, so an
+// entry with kz > B/2 takes the value of its Hermitian partner, as the
+// reference's half spectrum implies.
+template <int B>
+__device__ __forceinline__ bool wedge_keep_rot(int kx, int ky, int kz, const float* P, float tan_a) {
+  int x = kx, y = ky, z = kz;
+  if (kz > B / 2) {
+    x = (B - kx) % B;
+    y = (B - ky) % B;
+    z = B - kz;
+  }
+  const float xs = (float)((x + B / 2) % B), ys = (float)((y + B / 2) % B), zs = (float)((z + B / 2) % B);
+  float xt = P[2] * zs + P[3];
+  float yt = P[6] * zs + P[7];
+  float zt = P[10] * zs + P[11];
+  xt = P[1] * ys + xt;
+  yt = P[5] * ys + yt;
+  zt = P[9] * ys + zt;
+  const float x_ = P[0] * xs + xt;
+  const float y_ = P[4] * xs + yt;
+  const float z_ = fabsf(P[8] * xs + zt);
+  const float cutoff = (float)((B - 1) / 2);
+  const float r = sqrtf((x_ * x_ + y_ * y_) + z_ * z_);
+  return (r <= cutoff) && ((z_ < 1e-4f) || (tan_a <= fabsf(x_) / z_));
+}
+
 // SimpleWedge (cutoff branch) at unshifted frequency index (kx<=B/2, ky, kz),
 // float arithmetic as src/tom/core/wedge.cpp:188-196.
 template <int B>
@@ -168,6 +230,8 @@ struct Job {
   const long* poff;         // [P] element offsets
   const float* tan_wedge;   // [P] tan(semi-angle); < 0 -> no wedge
   const float* rms;         // [P][A][9]
+  const float* tan_ref;     // [nbox] tan(template wedge semi-angle); < 0 -> none
+  const float* wref_P;      // [P][A][12] template wedge maps (used when tan_ref >= 0)
   int A;
   int P;
   int* counter;
@@ -186,6 +250,7 @@ __device__ __noinline__ void process(const Job& J, int p, c32* buf, c32* tw, dou
   float* R = J.scratch + (long)blockIdx.x * J.sstride;
   c32* H = reinterpret_cast<c32*>(R + G::N3);
   c32* Pp = H + G::NH;
+  c32* Pw = Pp + G::NH;
   const c32* twg = J.tw[B == 32 ? 0 : (B == 48 ? 1 : 2)];
   for (int i = threadIdx.x; i < B; i += NTS) tw[i] = twg[i];
   __syncthreads();
@@ -228,12 +293,50 @@ __device__ __noinline__ void process(const Job& J, int p, c32* buf, c32* tw, dou
   }
   const bool wedge = J.tan_wedge[p] >= 0.f;
   const float tan_a = J.tan_wedge[p];
+  const float tan_r = J.tan_ref[J.pbox[p]];
+  const bool wref = tan_r >= 0.f;
+  const double var_p0 = var_p;
 
   int best = -1;
   float bestv = 0.f;
   int bpos = 0;
   const float cB = (float)((B + 1) / 2);
   for (int a = 0; a < J.A; ++a) {
+    // ---- the template's wedge turned by a, applied to the particle
+    const c32* Pa = Pp;
+    var_p = var_p0;
+    if (wref) {
+      float Pr[12];
+      const float* pr = J.wref_P + ((long)p * J.A + a) * 12;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) Pr[k] = pr[k];
+      zpass_inv<B>(Pp, H, buf, tw, [&](int kx, int ky, int kz, c32 v, long) {
+        if ((kx == 0 && ky == 0 && kz == 0) || !wedge_keep_rot<B>(kx, ky, kz, Pr, tan_r)) return mk(0.f, 0.f);
+        return v;
+      });
+      double t2[2] = {0, 0};
+      auto remask = [&](int x, int y, int z, float v) {
+        const long i = ((long)z * B + y) * B + x;
+        const bool in = M[i] != 0.f;
+        R[i] = in ? v : 0.f;
+        if (in) {
+          t2[0] += (double)v;
+          t2[1] += (double)v * (double)v;
+        }
+      };
+      plane_c2r<B>(H, buf, tw, remask);
+      wg_sum<2>(t2, red);
+      const double mb = t2[0] / n;
+      var_p = (t2[1] - n * mb * mb) / numel;
+      plane_r2c<B>(R, H, buf, tw);
+      const float fmb = (float)mb;
+      zpass<B, false>(H, Pw, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
+        if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
+        const c32 fb = FB[idx];
+        return mk(v.x - fmb * fb.x, v.y - fmb * fb.y);
+      });
+      Pa = Pw;
+    }
     const float* rm = J.rms + ((long)p * J.A + a) * 9;
     float rmr[9];
 #pragma unroll
@@ -287,14 +390,14 @@ __device__ __noinline__ void process(const Job& J, int p, c32* buf, c32* tw, dou
         if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
         const c32 fb = FB[idx];
         const c32 w = mk(v.x - fmb * fb.x, v.y - fmb * fb.y);
-        return cmulc(Pp[idx], w);  // conj(Tw) . P
+        return cmulc(Pa[idx], w);  // conj(Tw) . P
       });
     } else {
       zpass<B, true>(H, nullptr, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
         if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
         const c32 fm = FM[idx], fb = FB[idx];
         const c32 w = mk(v.x - fm1 * fm.x - fm2 * fb.x, v.y - fm1 * fm.y - fm2 * fb.y);
-        return cmulc(Pp[idx], w);
+        return cmulc(Pa[idx], w);
       });
     }
     // ---- correlation back to real space; masked peak (first in x-fastest order)
@@ -428,17 +531,49 @@ __global__ __launch_bounds__(1024) void k_center(const float* T, float* out, lon
 
 int subtomo_workgroups() { return 1024; }
 
+void simple_wedge_map(const float euler_deg[3], int B, float out[12]) {
+  // tom_mex_corr3d converts degrees with a * (PI / 180.) (tom_mex_corr3d.cpp:247-249)
+  const double kPi = 3.141592653589793238512808959406186204433;
+  const double ang[3] = {(double)euler_deg[0] * (kPi / 180.), (double)euler_deg[2] * (kPi / 180.),
+                         (double)euler_deg[1] * (kPi / 180.)};  // {phi, theta, psi}
+  const int axes[3] = {2, 0, 2};
+  double P[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int k = 0; k < 3; ++k) {
+    if (ang[k] - floor(ang[k] / (kPi / 2.)) * (2. * kPi) == 0.) continue;
+    const double sa = sin(ang[k]), ca = cos(ang[k]);
+    const int i0 = axes[k] == 0 ? 1 : 0, i1 = axes[k] == 2 ? 1 : 2;
+    for (int r = 0; r < 3; ++r) {
+      const double p0 = P[r][i0], p1 = P[r][i1];
+      if (axes[k] == 1) {
+        P[r][i0] = p0 * ca + p1 * sa;
+        P[r][i1] = -p0 * sa + p1 * ca;
+      } else {
+        P[r][i0] = p0 * ca - p1 * sa;
+        P[r][i1] = p0 * sa + p1 * ca;
+      }
+    }
+  }
+  const double c = (double)(float)ceil((B - 1) / 2.);
+  for (int r = 0; r < 3; ++r) {
+    out[4 * r + 0] = (float)P[r][0];
+    out[4 * r + 1] = (float)P[r][1];
+    out[4 * r + 2] = (float)P[r][2];
+    out[4 * r + 3] = (float)(-(P[r][0] * c + P[r][1] * c + P[r][2] * c));
+  }
+}
+
 long subtomo_scratch_floats() {
-  // R (B^3 real) + H + P (two half spectra), for the largest box
+  // R (B^3 real) + H + P + Pw (three half spectra), for the largest box
   const long n3 = (long)BMAX * BMAX * BMAX;
   const long nh = (long)BMAX * BMAX * (BMAX / 2 + 1);
-  return n3 + 4 * nh;
+  return n3 + 6 * nh;
 }
 
 void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h, const float* templates_dev,
                  const long* tmpl_off, int64_t P, const int32_t* pbox_dev, const int32_t* order_dev,
                  const float* particles_dev, const long* poff_dev, const float* tan_dev, const float* rms_dev,
-                 int64_t A, int32_t* best_idx, int32_t* best_pos, float* best_val) {
+                 const float* tan_ref_dev, const float* wref_P_dev, int64_t A, int32_t* best_idx, int32_t* best_pos,
+                 float* best_val) {
   const int nbox = (int)boxes_h.size();
   const long sstride = subtomo_scratch_floats();
   const int G = subtomo_workgroups();
@@ -480,6 +615,8 @@ void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h,
     J.poff = poff_dev;
     J.tan_wedge = tan_dev;
     J.rms = rms_dev;
+    J.tan_ref = tan_ref_dev;
+    J.wref_P = wref_P_dev;
     J.A = (int)A;
     J.P = (int)P;
     J.counter = counter;

@@ -376,11 +376,19 @@ class PrunedSearch:
     around the best position so far. Scores are the exact FLCF of the search
     (FP32 correlation, FP64 statistics). An optional `rank(volume, xyz)`
     callable (e.g. the proposal network's particle logit at the candidates)
-    re-orders the local maxima before the K best are kept."""
+    re-orders the local maxima before the K best are kept.
+
+    With `sigmas` set, the candidate threshold comes from the coarse map
+    itself: max(threshold, median + sigmas * 1.4826 MAD) of its best scores
+    (a robust noise level; particles are a small fraction of the voxels), and
+    K is only a cap on memory. Nothing then depends on how many particles the
+    tomogram holds."""
 
     def __init__(self, n_coarse=512, K=400, threshold=0.05, nms_radius=12.0, levels=((10.0, 10.0), (5.0, 5.0)),
-                 bin=0):
+                 bin=0, sigmas=None):
         self.n_coarse, self.K, self.threshold, self.nms_radius = int(n_coarse), int(K), threshold, nms_radius
+        self.sigmas = None if sigmas is None else float(sigmas)
+        self.noise = None
         self.levels = tuple((float(h), float(s)) for h, s in levels)
         self.orientations = orient.super_fibonacci(self.n_coarse).astype(np.float32)
         self.bin = int(bin)
@@ -429,9 +437,32 @@ class PrunedSearch:
         plan.run(self.orientations, 0)
         return plan.finalize()
 
+    def noise_threshold(self, score):
+        """median + sigmas * 1.4826 MAD of the coarse best-score map (at most
+        2^22 voxels sampled with a fixed stride), and the (median, sigma)."""
+        if hasattr(score, "cpu"):
+            import torch
+            flat = score.reshape(-1)
+            step = max(1, flat.numel() >> 22)
+            x = flat[::step].double()
+            med = float(torch.median(x))
+            mad = float(torch.median((x - med).abs()))
+        else:
+            x = np.asarray(score, np.float64).ravel()
+            x = x[::max(1, x.size >> 22)]
+            med = float(np.median(x))
+            mad = float(np.median(np.abs(x - med)))
+        sig = 1.4826 * mad
+        self.noise = (med, sig)
+        return med + self.sigmas * sig
+
     def candidates(self, score, index, rank=None, volume=None):
         """(P, 3) x y z and (P, 3) coarse Euler angles of the kept maxima."""
-        lm = api.local_maxima(score, 2, float(self.threshold))
+        thr = float(self.threshold)
+        if self.sigmas is not None:
+            thr = max(thr, self.noise_threshold(score))
+        self.candidate_threshold = thr
+        lm = api.local_maxima(score, 2, thr)
         if len(lm) == 0:
             return np.zeros((0, 3), np.int64), np.zeros((0, 3), np.float32)
         val = lm[:, 3] if rank is None else np.asarray(rank(volume, lm[:, :3].astype(np.int64)), np.float64)
