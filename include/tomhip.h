@@ -49,6 +49,15 @@ int tomhip_synchronize(tomhip_ctx* ctx);
 int tomhip_rotate3d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, int64_t nz,
                         const float* euler, int neuler, const float* center);
 
+/* The 2D branch of tom_rotatec (rot2d, Sptrans/tom_rotatec.c:49-142, gateway
+ * :466-475): bilinear rotation of an nx x ny image by phi degrees (nphi = 1)
+ * or by a 2x2 matrix rm00 rm10 rm01 rm11 (nphi = 4), pivot center[2]
+ * (0-based; tom_rotate passes ceil(size/2)). Pixels whose source falls
+ * outside the image are 0; the last row takes no y interpolation, as in the
+ * reference. */
+int tomhip_rotate2d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, const float* phi,
+                        int nphi, const float* center);
+
 /* Batch form: count rotations of one volume, dst[r] at dst + r*nx*ny*nz,
  * euler = [count][3] degrees; optional mask multiplies every output. No
  * reference counterpart (the reference loops tom_rotate per angle,
@@ -61,6 +70,23 @@ int tomhip_rotate3d_batch_f32(tomhip_ctx* ctx, const float* src, float* dst, int
  * order, strict '>' from -99999 (Analysis/tom_peakinc.c:23-77, gateway :60).
  * pos is 0-based (the reference MEX adds 1). */
 int tomhip_peak_f32(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val, int32_t pos[3]);
+
+
+/* tom_peak(A) (Analysis/tom_peak.m:80-108): the maximum and EVERY position
+ * holding it, 0-based, in MATLAB's find order (x fastest); pos [capacity][3];
+ * *count = number of ties (-1 error when they exceed capacity). */
+int tomhip_peak_all_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], float* max_val, int32_t* pos,
+                        int64_t capacity, int64_t* count);
+/* tom_peak(A, 'spline'), tom_peak(A, R) and tom_peak(A, R, 'spline')
+ * (Analysis/tom_peak.m:117-123, 203-278, 288-430, tom_intpeak :437-525) for
+ * 3D volumes: the first maximum (x-fastest order); with spline != 0 each axis
+ * is refined by the parabola through the maximum and its two neighbours,
+ * sampled at steps of 0.001 (the first best sample; axes at a border keep the
+ * integer position and the maximum), val = the largest fitted value; with
+ * radius > 0 out receives the map with |d| <= radius around the integer
+ * maximum set to 0 (tom_sphere, clipped to the volume). c is 0-based. */
+int tomhip_peak_radius_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, int spline,
+                           double c[3], double* val, float* out);
 
 /* ---------------------------------------------------------------- masks, weights */
 /* tom_os3_sphereMask(template) (Oscar3/matlabPicker/search/tom_os3_sphereMask.m:4-36):
@@ -132,13 +158,37 @@ int tomhip_flcf_search_f32(tomhip_ctx* ctx, const float* volume, const int64_t N
                            int batch, float* score, int32_t* angle);
 /* tom_os3_corr(img, template, options) FLCF branch
  * (Oscar3/matlabPicker/search/tom_os3_corr.m:77-192): mask NULL uses
- * tom_os3_sphereMask(template). */
+ * tom_os3_sphereMask(template). Includes the low-std branch (:110-125): when
+ * the image's mean local std is below 0.01 the template is renormalised as
+ * (T*1000+1000).*M, 'mean0+1std', and if the image stays featureless (mean
+ * local std below 0.01 of its global std) voxels whose local std is below
+ * 0.05 of it score 0. */
 int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], const float* tmpl,
                         const int64_t n[3], const float* mask, float* out);
 /* tom_oscar(vol, templ, ...) (Filtrans/tom_oscar.m:1-82) over an explicit angle
  * list: ccfmax and the 0-based index of the best angle (-1 never set). */
 int tomhip_oscar_f32(tomhip_ctx* ctx, const float* volume, const int64_t N[3], const float* tmpl,
                      const int64_t n[3], const float* angles, int64_t A, float* ccfmax, int32_t* angle);
+
+/* Box-size FLCF by direct sums: tom_os3_corr(search, tmpl) when the search
+ * volume has the template's size (Oscar3/matlabPicker/search/tom_os3_corr.m:
+ * 77-192, every voxel divided: res = num ./ den), circular; with search =
+ * tmpl it is the template autocorrelation that the second-order correlation
+ * uses (analyze/tom_os3_analyzePeak.m:180-194; os3_soc, Oscar3/c++Picker/src/
+ * tom/os3/os3_correlation.cpp:164-171, whose os3_flcf(pattern, pattern, mask)
+ * makes it). mask NULL: tom_os3_sphereMask(template). Edges <= 128. */
+int tomhip_flcf_box_f32(tomhip_ctx* ctx, const float* search, const float* tmpl, const float* mask,
+                        const int64_t n[3], float* out);
+/* tom_os3_bestPeak (Oscar3/matlabPicker/search/tom_os3_bestPeak.m:13-47;
+ * the three-score rule is os3_results::update, Oscar3/c++Picker/src/tom/os3/
+ * os3_results.cpp:127-150): where every score the rule selects (bit 0 ccc,
+ * bit 1 psr, bit 2 soc / autocorrelation) strictly improves on the result
+ * maps, copy the orientation's maps and write angle. psr / soc and their
+ * result maps may be NULL when the rule does not select them. Result maps
+ * start at -1000 (tom_os3_findTemplate.m:215-224). */
+int tomhip_os3_bestpeak_f32(tomhip_ctx* ctx, const int64_t N[3], int rule, int32_t angle, const float* ccc,
+                            const float* psr, const float* soc, float* res_ccc, float* res_psr, float* res_soc,
+                            int32_t* res_angle);
 
 /* ---------------------------------------------------------------- subtomogram scoring */
 /* Batched, per-particle local angular search with per-particle wedges, in one

@@ -17,6 +17,10 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
   rotate                Sptrans/tom_rotate.m:89-128 + Sptrans/tom_rotatec.c:146-401
                         -> bit-exact vs the reference's rot3dlin (oracle/_ref) and
                            golden vectors; axis invariants tom_rotate.m:59-61
+  rotate2d              Sptrans/tom_rotatec.c:49-142 (rot2d; not buildable here,
+                        its header includes mex.h) -> identity, exact rotation of
+                        a linear ramp (sense and pivot), zero outside, last-row
+                        quirk by hand (test_rotate2d_closed_forms)
   sphere_mask_os3       search/tom_os3_sphereMask.m:4-61 -> closed form
   sphere_mask_gauss     src/tom/core/volume_fcn.cpp:932-953, 1013-1030 -> closed form
   wedge                 Filtrans/tom_wedge.m:62-77 -> grid predicate, kz=0 kept
@@ -62,6 +66,15 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
                         Analysis/tom_peakc.m:39-86 -> separated planted peaks come
                         back in descending order with their erase spheres
                         (test_picklist_planted_peaks)
+  flcf_box              search/tom_os3_corr.m:77-192 at the template's size
+                        -> brute-force direct sums (orc_flcf_direct); the
+                        autocorrelation scores 1 at zero shift
+  bestpeak              search/tom_os3_bestPeak.m:13-47, os3_results.cpp:127-150
+                        -> the ccc rule reproduces find_template's maps; the
+                        three-score rule: recorded maps are their angle's maps
+                        and only strict improvements of every score update
+  os3_scores            c++Picker/src/tom/os3/os3_picker.cpp:55-95 (composition of
+                        the pinned rotate, flcf, psr and flcf_box)
   tom_bin               Sptrans/tom_bin.m:38-51 -> constant and linear-ramp
                         volumes (closed forms), odd-edge cropping, one 2x2x2
                         block summed by hand (test_tom_bin_closed_forms)
@@ -126,6 +139,8 @@ def lib():
         L.orc_rot3d.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, ctypes.c_long, fp,
                                 ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float]
         L.orc_find_max.argtypes = [fp, ip, fp, ip]
+        L.orc_rot2d.argtypes = [fp, fp, ctypes.c_long, ctypes.c_long, fp, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_float]
         L.orc_flcf_direct.argtypes = [fp, lp, dp, dp, lp, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, dp, dp, dp]
         _LIB = L
@@ -182,6 +197,20 @@ def rotate(vol: np.ndarray, euler, center=None) -> np.ndarray:
     out = np.zeros_like(v, order="F")
     lib().orc_rot3d(_fptr(v), _fptr(out), sx, sy, sz, _fptr(e), int(e.size),
                     float(center[0]), float(center[1]), float(center[2]))
+    return out
+
+
+def rotate2d(img: np.ndarray, phi, center=None) -> np.ndarray:
+    """tom_rotate(img, phi, 'linear', center) for a 2D image: rot2d of
+    Sptrans/tom_rotatec.c:49-142 (phi degrees, or a 2x2 matrix given as
+    rm00 rm10 rm01 rm11); default pivot ceil(size/2), 0-based."""
+    v = np.asfortranarray(img, dtype=np.float32)
+    sx, sy = v.shape
+    if center is None:
+        center = [int(np.ceil(s / 2)) for s in v.shape]
+    e = np.ascontiguousarray(np.asarray(phi, dtype=np.float32).ravel())
+    out = np.zeros_like(v, order="F")
+    lib().orc_rot2d(_fptr(v), _fptr(out), sx, sy, _fptr(e), int(e.size), float(center[0]), float(center[1]))
     return out
 
 
@@ -373,6 +402,53 @@ def find_template(V, T, M, angles, W=None, return_all=False):
     if return_all:
         return ccc, ang, np.stack(allmaps)
     return ccc, ang
+
+
+def flcf_at_voxels(V, Tn, M, mu_T, sig_T, angles, voxels, W=None, chunk=256, workers=1):
+    """FLCF scores of the findTemplate loop at chosen voxels only, FP64 direct
+    sums (for volumes too large for full FP64 maps): for voxel y and
+    orientation a, with t_a = remask(apply_weight(rotate(Tn, a) . M, W)) and
+    the template centre floor(n/2) at y (pasteCenter + ifftshift, circular),
+      c  = sum_u t_a(u) V(y + u - floor(n/2)) / m
+      mu = sum_u M(u) V(...) / m,  e2 = sum_u M(u) V(...)^2 / m
+      score = (c - mu mu_T) / (sig_T sqrt(e2 - mu^2))
+    The same quantities as local_stats + flcf (tom_os3_corr.m:77-192,
+    tom_os3_mean.m, tom_os3_std.m), without the global mean shift or the
+    variance floor. Returns (n_voxels, n_angles) float64."""
+    M = np.asarray(M, np.float64)
+    n = M.shape
+    N = V.shape
+    m = float((M > 0).sum())
+    sel = np.nonzero(M > 0)
+    off = [sel[d] - n[d] // 2 for d in range(3)]
+    vox = np.asarray(voxels, np.int64).reshape(-1, 3)
+    idx = [(vox[:, d][:, None] + off[d][None, :]) % N[d] for d in range(3)]
+    patch = np.asarray(V[idx[0], idx[1], idx[2]], np.float64)  # (nv, k)
+    w = M[sel]
+    mu = patch @ w / m
+    e2 = (patch * patch) @ w / m
+    sig = np.sqrt(np.maximum(e2 - mu * mu, 0.0))
+    A = np.asarray(angles, np.float64).reshape(-1, 3)
+    out = np.empty((vox.shape[0], A.shape[0]))
+    Tn32 = np.asarray(Tn, np.float32)
+    for a0 in range(0, A.shape[0], chunk):
+        ts = []
+        for a in A[a0:a0 + chunk]:
+            ts.append(rotate(Tn32, a).astype(np.float64) * M)
+        ts = np.stack(ts)
+        if W is not None:
+            if workers > 1:
+                # ifftshift(fftshift(F) . W) = F . ifftshift(W)
+                import scipy.fft
+                F = scipy.fft.fftn(ts, axes=(1, 2, 3), workers=workers)
+                F *= np.fft.ifftshift(W)[None]
+                ts = np.real(scipy.fft.ifftn(F, axes=(1, 2, 3), workers=workers))
+            else:
+                ts = np.stack([apply_weight(t, W) for t in ts])
+        tk = ts[:, sel[0], sel[1], sel[2]] * w[None]  # re-masked template at the mask's voxels
+        c = tk @ patch.T / m  # (chunk, nv)
+        out[:, a0:a0 + chunk] = ((c - mu[None] * mu_T) / (sig_T * sig[None])).T
+    return out
 
 
 def oscar(V, T, angles):
@@ -659,6 +735,65 @@ def psr(xcf: np.ndarray, n: int) -> np.ndarray:
     nz = sig > 0
     out[nz] = (x[nz] - mu[nz]) / sig[nz]
     return out
+
+
+def flcf_box(search, tmpl, M=None):
+    """tom_os3_corr(search, tmpl) for a search volume of the template's size
+    (Oscar3/matlabPicker/search/tom_os3_corr.m:77-192: image statistics under
+    the mask, template statistics tom_os3_normUnderMask, every voxel divided,
+    circular). M None: tom_os3_sphereMask(template)."""
+    S = np.asarray(search, np.float64)
+    T = np.asarray(tmpl, np.float64)
+    if M is None:
+        M, _ = sphere_mask_os3(T.shape[0])
+    M = np.asarray(M, np.float64)
+    m = float((M > 0).sum())
+    mu_T, sig_T = norm_under_mask(T, M)
+    mu_I, sig_I = local_stats(S, M, m)
+    c = correlation_raw(S, T * M) / m
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return (c - mu_I * mu_T) / (sig_T * sig_I)
+
+
+def bestpeak(res, ccc, psr, soc, angle, rule):
+    """tom_os3_bestPeak (search/tom_os3_bestPeak.m:13-47; the three-score rule
+    is os3_results::update, Oscar3/c++Picker/src/tom/os3/os3_results.cpp:
+    127-150): where every selected score strictly improves, copy the maps and
+    the angle. res: dict of float64 maps (start -1000) and 'angle'."""
+    up = np.ones(np.shape(ccc), bool)
+    if "ccc" in rule:
+        up &= res["ccc"] < ccc
+    if "psr" in rule:
+        up &= res["psr"] < psr
+    if "soc" in rule:
+        up &= res["soc"] < soc
+    res["ccc"][up] = np.asarray(ccc)[up]
+    if psr is not None:
+        res["psr"][up] = np.asarray(psr)[up]
+    if soc is not None:
+        res["soc"][up] = np.asarray(soc)[up]
+    res["angle"][up] = angle
+    return res
+
+
+def os3_scores(V, T, M, angle):
+    """One orientation of the correlation picker with secondary scores
+    (Oscar3/c++Picker/src/tom/os3/os3_picker.cpp:55-95): t = rotate(T).M,
+    ccc = tom_os3_corr(V, t, M) (template statistics of t), psr = the PSR map
+    of ccc, soc = tom_os3_corr(ccc, flcf_box(t, t, M), M). FP64."""
+    M = np.asarray(M, np.float64)
+    m = float((M > 0).sum())
+    t = rotate(np.asarray(T, np.float32), angle).astype(np.float64) * M
+    V = np.asarray(V, np.float64)
+    mu_t, sig_t = norm_under_mask(t, M)
+    mu_I, sig_I = local_stats(V, M, m)
+    ccc = flcf(V, t * M, m, mu_t, sig_t, mu_I, sig_I)
+    pr = psr(ccc, T.shape[0])
+    auto = flcf_box(t, t, M)
+    mu_a, sig_a = norm_under_mask(auto, M)
+    mu_c, sig_c = local_stats(ccc, M, m)
+    soc = flcf(ccc, auto * M, m, mu_a, sig_a, mu_c, sig_c)
+    return ccc, pr, soc
 
 
 def tom_bin(im: np.ndarray, nbin: int = 1) -> np.ndarray:

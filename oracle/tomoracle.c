@@ -10,6 +10,8 @@
  *                   (Sptrans/tom_rotatec.c:146-291)
  *   orc_rot3d       trilinear inverse-mapped 3D rotation, float arithmetic in
  *                   the reference's evaluation order (Sptrans/tom_rotatec.c:293-398)
+ *   orc_rot2d       bilinear 2D rotation of rot2d, float arithmetic in the
+ *                   reference's evaluation order (Sptrans/tom_rotatec.c:49-142)
  *   orc_find_max    global max, first position in x-fastest order, strict '>'
  *                   (Analysis/tom_peakinc.c:23-52)
  *   orc_flcf_direct brute-force direct-space masked FLCF in FP64 for tiny
@@ -20,6 +22,11 @@
  *   orc_rotmat/orc_rot3d: bit-exact against the reference's own rot3dlin and
  *     tom_transf_init_rotmat_3d compiled from its sources (oracle/_ref) and the
  *     committed golden vectors; axis invariants of Sptrans/tom_rotate.m:59-61.
+ *   orc_rot2d: identity at 0 degrees; a linear ramp rotates exactly (bilinear
+ *     interpolation reproduces linear functions) with the reference's sense and
+ *     pivot; out-of-image pixels are 0; the last-row quirk (no y tap) by hand.
+ *     Sptrans/tom_rotatec.c cannot be compiled here (its header defines MATLAB
+ *     and includes mex.h), so no golden vectors exist for it.
  *   orc_find_max: numpy argmax (first occurrence) on x-fastest data; ties.
  *   orc_flcf_direct: the numpy FFT restatement agrees with it; planted copy
  *     scores exactly 1; affine invariance; zero local variance -> 0.
@@ -205,4 +212,51 @@ void orc_flcf_direct(const float *V, const long *N, const double *tm, const doub
                 if (mu_out) mu_out[o] = mu;
                 if (sig_out) sig_out[o] = sig;
             }
+}
+
+/* rot2d (Sptrans/tom_rotatec.c:49-142): phi degrees (nphi 1) or a 2x2 matrix
+ * rm00 rm10 rm01 rm11 (nphi 4); pivot (px, py) 0-based. */
+void orc_rot2d(const float *image, float *rotimg, long sx, long sy, const float *p_phi, int nphi, float px,
+               float py)
+{
+    float rm00, rm01, rm10, rm11;
+    if (nphi == 1) {
+        float phi = p_phi[0];
+        phi = phi * ORC_PI / 180;
+        rm00 = cos(phi);
+        rm01 = -sin(phi);
+        rm10 = sin(phi);
+        rm11 = cos(phi);
+    } else {
+        rm00 = p_phi[0];
+        rm10 = p_phi[1];
+        rm01 = p_phi[2];
+        rm11 = p_phi[3];
+    }
+    const long sx_1 = sx - 1, sy_1 = sy - 1;
+    float pi, pj;
+    long i, j;
+    for (pj = -py, j = 0; j < sy; j++, pj++) {
+        for (pi = -px, i = 0; i < sx; i++, pi++) {
+            const float r_x = px + rm00 * pi + rm10 * pj;
+            if (r_x < 0 || r_x > sx_1) { *rotimg++ = 0; continue; }
+            const float r_y = py + rm01 * pi + rm11 * pj;
+            if (r_y < 0 || r_y > sy_1) { *rotimg++ = 0; continue; }
+            const long floorx = r_x;
+            const float vx2 = r_x - floorx;
+            const long floory = r_y;
+            const float vy2 = r_y - floory;
+            const float *imgp = &image[floorx + sx * floory];
+            float AA, BB;
+            if (r_x < sx - 1) AA = *imgp + (imgp[1] - *imgp) * vx2;
+            else AA = *imgp;
+            if (r_y < sy - 1) {
+                if (r_x < sx - 1) BB = imgp[sx] + (imgp[sx + 1] - imgp[sx]) * vx2;
+                else BB = imgp[sx];
+            } else {
+                BB = 0;
+            }
+            *rotimg++ = AA + (BB - AA) * vy2;
+        }
+    }
 }

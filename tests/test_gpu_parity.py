@@ -55,6 +55,24 @@ def test_rotate_golden_bit_exact(api):
     assert not bad, f"{len(bad)} golden rotations differ, first {bad[:5]}"
 
 
+def test_rotate2d_matches_oracle(api, oracle):
+    """tom_rotate of a 2D image (rot2d) through the C ABI, bit-exact against
+    the oracle's restatement: integer and fractional pivots (the pixel
+    coordinates accumulate by float increments), the matrix form, a 1-pixel
+    wide image."""
+    say("rotate 2d")
+    rng = np.random.default_rng(12)
+    img = rng.standard_normal((45, 38)).astype(np.float32)
+    for phi, c in ((0.0, None), (30.0, None), (-112.7, (22.5, 19.0)), (355.0, (7.3, 30.9)), (90.0, (0, 0))):
+        got = api.tom_rotate(img, phi, "linear", c)
+        np.testing.assert_array_equal(got, oracle.rotate2d(img, phi, c))
+    m = np.array([[0.8, -0.6], [0.6, 0.8]], np.float32)  # rm00 rm10 ; rm01 rm11 column-major
+    np.testing.assert_array_equal(api.tom_rotate(img, m, "linear", (20, 17)),
+                                  oracle.rotate2d(img, m.ravel(order="F"), (20, 17)))
+    line = rng.standard_normal((1, 25)).astype(np.float32)
+    np.testing.assert_array_equal(api.tom_rotate(line, 10.0), oracle.rotate2d(line, 10.0))
+
+
 def test_rotate_batch_matches_single_and_oracle(api, oracle):
     say("rotate batch")
     rng = np.random.default_rng(0)

@@ -425,3 +425,107 @@ def test_cpu_fp32_baseline_matches_oracle(oracle):
     sel = (srt[-1] - srt[-2]) > 1e-3
     assert np.all(cpu.arg.numpy()[sel] == ref_ang[sel])
     assert cpu.arg.numpy()[12, 11, 14] == 2
+
+
+def test_flcf_at_voxels_matches_fft_maps(oracle):
+    """The sampled-voxel direct-sum FLCF equals the FFT restatement's maps
+    (find_template's per-angle scores) at every sampled voxel, including
+    voxels whose template window wraps around the volume edge."""
+    rng = np.random.default_rng(6)
+    n, N = 10, (24, 20, 22)
+    T = rng.standard_normal((n, n, n)).astype(np.float32)
+    V = rng.standard_normal(N).astype(np.float32) + 0.5
+    M = oracle.sphere_mask_gauss(n, 3.5, 1.0)
+    W = oracle.wedge((n, n, n), 30) * oracle.bandpass_weight((n, n, n), 1, 4)
+    angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 45]], float)
+    _, _, allm = oracle.find_template(V, T, M, angles, W=W, return_all=True)
+    Tn, mu_T, sig_T = oracle.prepare_template(T, M.astype(np.float64), W)
+    vox = np.array([[0, 0, 0], [23, 19, 21], [12, 10, 11], [3, 17, 5], [20, 2, 9]])
+    for workers in (1, 2):
+        s = oracle.flcf_at_voxels(V, Tn, M, mu_T, sig_T, angles, vox, W=W, chunk=2, workers=workers)
+        ref = allm[:, vox[:, 0], vox[:, 1], vox[:, 2]].T
+        np.testing.assert_allclose(s, ref, atol=1e-10)
+
+
+def test_rotate2d_closed_forms(oracle):
+    """rot2d (Sptrans/tom_rotatec.c:49-142): 0 degrees is the identity; a
+    linear ramp f = 2x + 3y + 1 rotates exactly under bilinear interpolation,
+    with the content turned by +phi about the pivot (the output at d samples
+    the input at p + R(-phi)(d - p)), where the source lies inside the image;
+    pixels whose source falls outside are 0; the matrix form with the same
+    entries gives the same image."""
+    n = 16
+    rng = np.random.default_rng(2)
+    img = rng.standard_normal((n, n)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.rotate2d(img, 0.0), img)
+    x = np.arange(n, dtype=np.float64)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    ramp = (2 * X + 3 * Y + 1).astype(np.float32)
+    for phi, p in ((30.0, (8, 8)), (-70.0, (6, 9)), (135.0, (8, 7))):
+        out = oracle.rotate2d(ramp, phi, p)
+        a = np.radians(phi)
+        sx = p[0] + np.cos(a) * (X - p[0]) + np.sin(a) * (Y - p[1])
+        sy = p[1] - np.sin(a) * (X - p[0]) + np.cos(a) * (Y - p[1])
+        inside = (sx > 1e-3) & (sx < n - 1 - 1e-3) & (sy > 1e-3) & (sy < n - 1 - 1e-3)
+        outside = (sx < -1e-3) | (sx > n - 1 + 1e-3) | (sy < -1e-3) | (sy > n - 1 + 1e-3)
+        np.testing.assert_allclose(out[inside], (2 * sx + 3 * sy + 1)[inside], atol=1e-3)
+        assert np.all(out[outside] == 0)
+        # the opposite sense would put the ramp elsewhere
+        assert np.abs(out[inside] - (2 * (2 * p[0] - sx) + 3 * (2 * p[1] - sy) + 1)[inside]).max() > 1.0
+    a = np.float32(np.radians(np.float32(30.0)))
+    c, s = np.float32(np.cos(np.float64(a))), np.float32(np.sin(np.float64(a)))
+    np.testing.assert_array_equal(oracle.rotate2d(img, np.array([c, s, -s, c], np.float32), (8, 8)),
+                                  oracle.rotate2d(img, 30.0, (8, 8)))
+
+
+def test_flcf_box_direct_sums_and_autocorrelation(oracle):
+    """Box-size tom_os3_corr equals the brute-force direct-sum FLCF at N = n,
+    and a template's autocorrelation scores 1 at zero shift (the box centre
+    floor(n/2), pasteCenter + ifftshift) under a binary mask."""
+    rng = np.random.default_rng(9)
+    n = 10
+    T = rng.standard_normal((n, n, n))
+    S = rng.standard_normal((n, n, n)) + 0.2
+    M = oracle.sphere_mask_binary(n, 4.2).astype(np.float64)
+    got = oracle.flcf_box(S, T, M)
+    m = float((M > 0).sum())
+    mu_T, sig_T = oracle.norm_under_mask(T, M)
+    sc, _, _ = _direct(oracle, S.astype(np.float32), T * M, M, m, mu_T, sig_T)
+    np.testing.assert_allclose(got, sc, atol=1e-5)  # S rounded to float32 for the C sums
+    auto = oracle.flcf_box(T * M, T * M, M)
+    assert abs(auto[n // 2, n // 2, n // 2] - 1.0) < 1e-12
+    assert np.nanmax(auto) <= 1.0 + 1e-12
+
+
+def test_bestpeak_rules(oracle):
+    """The ccc rule over an angle list reproduces find_template's maps; with
+    all three scores a voxel's recorded maps are those of its recorded angle,
+    and each recorded update improved every score strictly."""
+    rng = np.random.default_rng(10)
+    n, N = 8, (20, 18, 16)
+    T = rng.standard_normal((n, n, n)).astype(np.float32)
+    V = rng.standard_normal(N).astype(np.float32)
+    M = oracle.sphere_mask_binary(n, 3.5)
+    angles = np.array([[0, 0, 0], [30, 60, 90], [90, 30, 45], [10, 20, 30]], float)
+    ref_ccc, ref_ang, allm = oracle.find_template(V, T, M, angles, return_all=True)
+    res = {k: np.full(N, -1000.0) for k in ("ccc", "psr", "soc")}
+    res["angle"] = np.zeros(N, np.int64)
+    for i, s in enumerate(allm):
+        oracle.bestpeak(res, s, None, None, i + 1, "ccc")
+    np.testing.assert_array_equal(res["ccc"], ref_ccc)
+    np.testing.assert_array_equal(res["angle"], ref_ang)
+    maps = [tuple(rng.standard_normal(N) for _ in range(3)) for _ in angles]
+    res = {k: np.full(N, -1000.0) for k in ("ccc", "psr", "soc")}
+    res["angle"] = np.zeros(N, np.int64)
+    for i, (c, p, q) in enumerate(maps):
+        before = {k: res[k].copy() for k in ("ccc", "psr", "soc")}
+        oracle.bestpeak(res, c, p, q, i + 1, "ccc+psr+soc")
+        up = res["angle"] == i + 1
+        assert np.all((before["ccc"] < c)[up] & (before["psr"] < p)[up] & (before["soc"] < q)[up])
+        keep = ~up
+        for k in ("ccc", "psr", "soc"):
+            np.testing.assert_array_equal(res[k][keep], before[k][keep])
+    for i, (c, p, q) in enumerate(maps):
+        sel = res["angle"] == i + 1
+        np.testing.assert_array_equal(res["ccc"][sel], c[sel])
+        np.testing.assert_array_equal(res["soc"][sel], q[sel])

@@ -26,9 +26,14 @@ SIGNATURES = {
     "tomhip_synchronize": (ctypes.c_int, [vp]),
     "tomhip_rotate3d_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, vp,
                                            ctypes.c_int, vp]),
+    "tomhip_rotate2d_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, vp, ctypes.c_int, vp]),
     "tomhip_rotate3d_batch_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, vp,
                                                  ctypes.c_int64, vp, vp]),
     "tomhip_peak_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "tomhip_peak_all_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int64, vp]),
+    "tomhip_peak_radius_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]),
+    "tomhip_flcf_box_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "tomhip_os3_bestpeak_f32": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp]),
     "tomhip_spheremask_os3_f32": (ctypes.c_int, [vp, vp, vp, vp]),
     "tomhip_spheremask_f32": (ctypes.c_int, [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
     "tomhip_weight_f32": (ctypes.c_int, [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]),

@@ -135,11 +135,26 @@ def _prep_ctx(ctx, *bufs) -> Context:
 def tom_rotate(vol, euler, interp: str = "linear", center=None, ctx: Optional[Context] = None):
     """out = tom_rotate(in, [phi psi theta] | 3x3 matrix, 'linear', [center]).
 
-    center is the 0-based pivot handed to tom_rotatec (default ceil(size/2))."""
+    center is the 0-based pivot handed to tom_rotatec (default ceil(size/2)).
+    A 2D image (numpy (nx, ny) or torch (ny, nx)) takes one angle phi in
+    degrees or a 2x2 matrix and goes through rot2d (tom_rotatec.c:49-142)."""
     if interp != "linear":
         raise ValueError("only 'linear' interpolation exists (tom_rotatec.c:424-430)")
     b = _Buf(vol)
     ctx = _prep_ctx(ctx, b)
+    if len(b.dims) == 2:
+        e = np.asarray(euler, dtype=np.float32)
+        if e.size == 4 and e.ndim == 2:
+            e = np.asfortranarray(e).ravel(order="F")
+        e = np.ascontiguousarray(e.ravel())
+        if e.size not in (1, 4):
+            raise ValueError("a 2D rotation takes one angle or a 2x2 matrix")
+        c = np.asarray([np.ceil(d / 2) for d in b.dims] if center is None else np.asarray(center).ravel()[:2],
+                       np.float32)
+        out = _empty_like(b, b.dims)
+        ctx.check(ctx.lib.tomhip_rotate2d_f32(ctx.h, b.ptr, _ptr(out), int(b.dims[0]), int(b.dims[1]), _p(e),
+                                              int(e.size), _p(np.ascontiguousarray(c))))
+        return out
     e = np.asarray(euler, dtype=np.float32)
     neuler = 9 if e.size == 9 else 3
     if e.size == 9 and e.ndim == 2:
@@ -187,11 +202,112 @@ def tom_peakinc(a, ctx: Optional[Context] = None) -> Tuple[float, np.ndarray]:
     return float(val[0]), pos
 
 
-def tom_peak(a, ctx: Optional[Context] = None):
-    """[c val] = tom_peak(A): 1-based position of the (first) maximum."""
-    v, p = tom_peakinc(a, ctx)
-    nd = np.asarray(a).ndim if not _is_torch(a) else a.dim()
-    return (p[:max(nd, 2)] + 1).astype(np.float64), v
+def tom_peak(a, r=None, flag=None, ctx: Optional[Context] = None):
+    """tom_peak in its MATLAB forms (Analysis/tom_peak.m), positions 1-based:
+      [c val]   = tom_peak(A)            every position holding the maximum,
+                                         one row each, in find order (x fastest)
+      [c val]   = tom_peak(A, 'spline')  first maximum refined per axis to 0.001
+      [c val M] = tom_peak(A, R)         first maximum; M = A with |d| <= R zeroed
+      [c val M] = tom_peak(A, R, 'spline')
+    A single maximum is returned as a vector (as MATLAB's one-row c)."""
+    b = _Buf(a)
+    ctx = _prep_ctx(ctx, b)
+    nd = max(len(b.dims), 2)
+    N = _dims3(b.dims)
+    if isinstance(r, str):
+        r, flag = None, r
+    spline = flag == "spline"
+    if flag is not None and not spline:
+        raise ValueError("the only flag is 'spline'")
+    if r is None and not spline:
+        cap = 1 << 16
+        pos = np.zeros((cap, 3), np.int32)
+        val = np.zeros(1, np.float32)
+        cnt = np.zeros(1, np.int64)
+        ctx.check(ctx.lib.tomhip_peak_all_f32(ctx.h, b.ptr, _p(N), _p(val), _p(pos), cap, _p(cnt)))
+        c = (pos[:int(cnt[0]), :nd] + 1).astype(np.float64)
+        return (c[0] if len(c) == 1 else c), float(val[0])
+    c = np.zeros(3, np.float64)
+    v = np.zeros(1, np.float64)
+    out = None
+    if r is not None:
+        out = _empty_like(b, b.dims)
+    ctx.check(ctx.lib.tomhip_peak_radius_f32(ctx.h, b.ptr, _p(N), int(r) if r is not None else 0, int(spline),
+                                             _p(c), _p(v), None if out is None else _ptr(out)))
+    c = c[:nd] + 1
+    if r is None:
+        return c, float(v[0])
+    return c, float(v[0]), out
+
+
+def tom_os3_bestPeak(res, ccc, psr=None, soc=None, angle: int = 0, rule: str = "ccc", ctx: Optional[Context] = None):
+    """In-place tom_os3_bestPeak update of res = dict(ccc=, psr=, soc=, angle=)
+    result maps (torch tensors on the device or numpy arrays; start them at
+    -1000): rule is any '+'-joined subset of 'ccc', 'psr', 'soc' (the
+    reference's options.analysis flags; all three is os3_results::update)."""
+    bits = {"ccc": 1, "psr": 2, "soc": 4}
+    code = 0
+    for k in rule.split("+"):
+        code |= bits[k]
+    cb = _Buf(ccc)
+    ctx = _prep_ctx(ctx, cb)
+    N = _dims3(cb.dims)
+
+    def ptr(x):
+        return None if x is None else _ptr(x)
+    for k in ("ccc", "psr", "soc", "angle"):
+        x = res.get(k)
+        if x is not None and not _is_torch(x) and not (x.flags.f_contiguous or x.ndim < 2):
+            raise ValueError(f"res['{k}'] must be a contiguous device tensor or a Fortran-ordered array")
+    ctx.check(ctx.lib.tomhip_os3_bestpeak_f32(ctx.h, _p(N), code, int(angle), cb.ptr,
+                                              None if psr is None else _Buf(psr).ptr,
+                                              None if soc is None else _Buf(soc).ptr, ptr(res["ccc"]),
+                                              ptr(res.get("psr")) if psr is not None else None,
+                                              ptr(res.get("soc")) if soc is not None else None,
+                                              ptr(res["angle"])))
+    return res
+
+
+def flcf_box(search, template, mask=None, ctx: Optional[Context] = None):
+    """tom_os3_corr(search, template) for a search volume of the template's
+    size (circular; with search = template the autocorrelation of the
+    second-order correlation)."""
+    sb, tb = _Buf(search), _Buf(template)
+    mb = _Buf(mask) if mask is not None else None
+    ctx = _prep_ctx(ctx, sb, tb, mb)
+    n = _dims3(tb.dims)
+    out = _empty_like(tb, tb.dims)
+    ctx.check(ctx.lib.tomhip_flcf_box_f32(ctx.h, sb.ptr, tb.ptr, None if mb is None else mb.ptr, _p(n), _ptr(out)))
+    return out
+
+
+def os3_picker(volume, template, mask, angles, rule: str = "ccc+psr+soc", ctx: Optional[Context] = None):
+    """The per-orientation loop of the reference's correlation picker with
+    its secondary scores (Oscar3/c++Picker/src/tom/os3/os3_picker.cpp:29-121;
+    MATLAB tom_os3_findTemplate.m:255-307 with options.analysis set): for each
+    orientation a, t = tom_rotate(T, a) .* M; ccc = tom_os3_corr(V, t, M);
+    psr = the PSR map of ccc; soc = tom_os3_corr(ccc, flcf_box(t, t, M), M);
+    then tom_os3_bestPeak(rule). volume: a (Z, Y, X) float32 CUDA tensor.
+    Returns dict(ccc, psr, soc, angle (1-based, 0 = never updated)) as tensors."""
+    ctx = ctx or context()
+    dev = volume.device
+    n = tuple(np.asarray(template).shape)
+    Td = torch.from_numpy(np.ascontiguousarray(np.asarray(template, np.float32).transpose(2, 1, 0))).to(dev)
+    Md = torch.from_numpy(np.ascontiguousarray(np.asarray(mask, np.float32).transpose(2, 1, 0))).to(dev)
+    res = {k: torch.full(volume.shape, -1000.0, dtype=torch.float32, device=dev) for k in ("ccc", "psr", "soc")}
+    res["angle"] = torch.zeros(volume.shape, dtype=torch.int32, device=dev)
+    need_psr = "psr" in rule
+    need_soc = "soc" in rule
+    for i, a in enumerate(np.asarray(angles, np.float32).reshape(-1, 3)):
+        t = tom_rotate(Td, a, ctx=ctx) * Md
+        ccc = tom_os3_corr(volume, t, Md, ctx=ctx)
+        psr = tom_os3_psr(ccc, n, ctx=ctx) if need_psr else None
+        soc = None
+        if need_soc:
+            auto = flcf_box(t, t, Md, ctx=ctx)
+            soc = tom_os3_corr(ccc, auto, Md, ctx=ctx)
+        tom_os3_bestPeak(res, ccc, psr, soc, angle=i + 1, rule=rule, ctx=ctx)
+    return res
 
 
 # ---------------------------------------------------------------- masks/weights

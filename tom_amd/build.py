@@ -32,8 +32,9 @@ COMMON = [
 # spell their fused multiply-adds out.
 _NOCONTRACT = ["-ffp-contract=off"]
 EXTRA = {"rotate.hip": _NOCONTRACT, "search.hip": _NOCONTRACT, "subtomo.hip": _NOCONTRACT,
-         "util.hip": _NOCONTRACT, "picks.hip": _NOCONTRACT}
-SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "unet.hip", "api.hip", "em_io.cpp"]
+         "util.hip": _NOCONTRACT, "picks.hip": _NOCONTRACT, "analysis.hip": _NOCONTRACT}
+SOURCES = ["rotate.hip", "search.hip", "subtomo.hip", "picks.hip", "util.hip", "analysis.hip", "unet.hip", "api.hip",
+           "em_io.cpp"]
 
 
 def _compile(src: str) -> str:

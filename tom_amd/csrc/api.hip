@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/tomhip.h"
+#include "analysis.h"
 #include "rotate.h"
 #include "picks.h"
 #include "search.h"
@@ -194,6 +195,34 @@ int tomhip_rotate3d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t n
     out.commit();
     TOMHIP_CHECK(hipStreamSynchronize(s));
     (void)hipFree(drm);
+  });
+}
+
+int tomhip_rotate2d_f32(tomhip_ctx* ctx, const float* src, float* dst, int64_t nx, int64_t ny, const float* phi,
+                        int nphi, const float* center) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && src && dst && phi && center, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(nphi == 1 || nphi == 4, ERR_WRONG_INPUT, "the 2D angle is one value or a 2x2 matrix");
+    TOMHIP_REQUIRE(nx >= 1 && ny >= 1 && nx * ny < (1L << 31), ERR_WRONG_INPUT, "bad dims");
+    float rm[4];
+    rotmat2d_ref(phi, nphi, rm);
+    require_finite(rm, 4);
+    TOMHIP_REQUIRE(std::isfinite(center[0]) && std::isfinite(center[1]), ERR_WRONG_INPUT, "center must be finite");
+    // pixel coordinates relative to the pivot, accumulated by float
+    // increments from -px and -py as rot2d's loops do (tom_rotatec.c:91-93)
+    std::vector<float> tab((size_t)(nx + ny));
+    float c = -center[0];
+    for (int64_t i = 0; i < nx; ++i, c += 1.f) tab[(size_t)i] = c;
+    c = -center[1];
+    for (int64_t j = 0; j < ny; ++j, c += 1.f) tab[(size_t)(nx + j)] = c;
+    const size_t bytes = sizeof(float) * nx * ny;
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, src, bytes, true, false);
+    DevBuf out(s, dst, bytes, false, true);
+    DevBuf dt(s, tab.data(), sizeof(float) * tab.size(), true, false);
+    launch_rotate2d(in.get<float>(), out.get<float>(), nx, ny, dt.get<float>(), rm, center[0], center[1], s);
+    out.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }
 
@@ -590,9 +619,11 @@ int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], c
     DevBuf mk_(s, mask, sizeof(float) * n3, true, false);
     float* own_mask = nullptr;
     int32_t* ang = nullptr;
+    float* tren = nullptr;
     auto cleanup = [&] {
       (void)hipFree(own_mask);
       (void)hipFree(ang);
+      (void)hipFree(tren);
     };
     try {
       const float* M = mask ? mk_.get<float>() : nullptr;
@@ -601,17 +632,34 @@ int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], c
         launch_spheremask_os3((int)n[0], (int)n[1], (int)n[2], own_mask, s);
         M = own_mask;
       }
-      double mu_t = 0, sig_t = 1;
-      template_stats(s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], M, &mu_t, &sig_t);
       Search S(*ctx->tw, s, v.get<float>(), (int)N[0], (int)N[1], (int)N[2], M, (int)n[0], (int)n[1], (int)n[2],
                MODE_FLCF, 1);
-      S.set_template(t.get<float>(), nullptr, mu_t, sig_t, false);
+      // tom_os3_corr.m:110-125: an image whose mean local std is below 0.01
+      // is renormalised with its template ((T*1000+1000).*M, 'mean0+1std');
+      // if the renormalised image's mean local std is still below 0.01, its
+      // voxels of local std below 0.05 score 0. The image's renormalisation
+      // does not change the score (the local statistics here are FP64), so
+      // only its global std is needed to express both thresholds.
+      const float* Tuse = t.get<float>();
+      bool hot = false;
+      double s_img = 1.0;
+      if (mean_f64(S.local_std(), NN, s) < 0.01) {
+        s_img = std_f32(v.get<float>(), NN, s);
+        TOMHIP_CHECK(hipMalloc(&tren, sizeof(float) * n3));
+        renorm_template_lowstd(t.get<float>(), M, n3, tren, s);
+        Tuse = tren;
+        hot = !(s_img > 0) || mean_f64(S.local_std(), NN, s) / s_img < 0.01;
+      }
+      double mu_t = 0, sig_t = 1;
+      template_stats(s, Tuse, (int)n[0], (int)n[1], (int)n[2], M, &mu_t, &sig_t);
+      S.set_template(Tuse, nullptr, mu_t, sig_t, false);
       const float zero[3] = {0, 0, 0};
       std::vector<float> rm = rotmats(zero, 1);  // exact identity
       S.run(rm.data(), 1, 0);
       TOMHIP_CHECK(hipMalloc(&ang, sizeof(int32_t) * NN));
       DevBuf o(s, out, sizeof(float) * NN, false, true);
       S.finalize(o.get<float>(), ang);
+      if (hot) zero_where_below(o.get<float>(), S.local_std(), NN, 0.05 * s_img, s);
       o.commit();
       TOMHIP_CHECK(hipStreamSynchronize(s));
     } catch (...) {
@@ -619,6 +667,178 @@ int tomhip_os3_corr_f32(tomhip_ctx* ctx, const float* img, const int64_t N[3], c
       throw;
     }
     cleanup();
+  });
+}
+
+int tomhip_flcf_box_f32(tomhip_ctx* ctx, const float* search, const float* tmpl, const float* mask,
+                        const int64_t n[3], float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && search && tmpl && out, ERR_WRONG_INPUT, "null pointer");
+    check_dims3(n, "box");
+    TOMHIP_REQUIRE(n[0] <= 128 && n[1] <= 128 && n[2] <= 128, ERR_UNSUPPORTED, "box edges must be <= 128");
+    const long n3 = n[0] * n[1] * n[2];
+    hipStream_t s = ctx->stream;
+    DevBuf sv(s, search, sizeof(float) * n3, true, false);
+    DevBuf t(s, tmpl, sizeof(float) * n3, true, false);
+    DevBuf mk_(s, mask, sizeof(float) * n3, true, false);
+    DevBuf o(s, out, sizeof(float) * n3, false, true);
+    float* own_mask = nullptr;
+    try {
+      const float* M = mask ? mk_.get<float>() : nullptr;
+      if (!M) {  // tom_os3_corr's default: tom_os3_sphereMask(template)
+        TOMHIP_CHECK(hipMalloc(&own_mask, sizeof(float) * n3));
+        launch_spheremask_os3((int)n[0], (int)n[1], (int)n[2], own_mask, s);
+        M = own_mask;
+      }
+      double mu_t = 0, sig_t = 1;
+      template_stats(s, t.get<float>(), (int)n[0], (int)n[1], (int)n[2], M, &mu_t, &sig_t);
+      flcf_box(sv.get<float>(), (int)n[0], (int)n[1], (int)n[2], t.get<float>(), M, mu_t, sig_t, o.get<float>(), s);
+      o.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(own_mask);
+      throw;
+    }
+    (void)hipFree(own_mask);
+  });
+}
+
+int tomhip_os3_bestpeak_f32(tomhip_ctx* ctx, const int64_t N[3], int rule, int32_t angle, const float* ccc,
+                            const float* psr, const float* soc, float* res_ccc, float* res_psr, float* res_soc,
+                            int32_t* res_angle) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && ccc && res_ccc && res_angle, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(rule >= 1 && rule <= 7, ERR_WRONG_INPUT, "rule must select ccc (1), psr (2) and/or soc (4)");
+    TOMHIP_REQUIRE(!(rule & 2) || (psr && res_psr), ERR_WRONG_INPUT, "the rule needs the psr maps");
+    TOMHIP_REQUIRE(!(rule & 4) || (soc && res_soc), ERR_WRONG_INPUT, "the rule needs the soc maps");
+    TOMHIP_REQUIRE((psr == nullptr) == (res_psr == nullptr) && (soc == nullptr) == (res_soc == nullptr),
+                   ERR_WRONG_INPUT, "psr / soc maps come with their result maps");
+    check_dims3(N, "map");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf c(s, ccc, sizeof(float) * NN, true, false);
+    DevBuf p(s, psr, sizeof(float) * NN, true, false);
+    DevBuf q(s, soc, sizeof(float) * NN, true, false);
+    DevBuf rc(s, res_ccc, sizeof(float) * NN, true, true);
+    DevBuf rp(s, res_psr, sizeof(float) * NN, true, true);
+    DevBuf rq(s, res_soc, sizeof(float) * NN, true, true);
+    DevBuf ra(s, res_angle, sizeof(int32_t) * NN, true, true);
+    bestpeak_update(c.get<float>(), p.get<float>(), q.get<float>(), NN, rule, angle, rc.get<float>(),
+                    rp.get<float>(), rq.get<float>(), ra.get<int32_t>(), s);
+    rc.commit();
+    rp.commit();
+    rq.commit();
+    ra.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tomhip_peak_all_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], float* max_val, int32_t* pos,
+                        int64_t capacity, int64_t* count) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && max_val && pos && count, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(capacity >= 1 && capacity <= (1L << 30), ERR_WRONG_INPUT, "bad capacity");
+    check_dims3(N, "map");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, map, sizeof(float) * NN, true, false);
+    float v;
+    long i0;
+    global_max(in.get<float>(), NN, &v, &i0, s);
+    float* dv = nullptr;
+    long* idx = nullptr;
+    long k = 0;
+    std::vector<long> h;
+    try {
+      TOMHIP_CHECK(hipMalloc(&dv, sizeof(float)));
+      TOMHIP_CHECK(hipMalloc(&idx, sizeof(long) * capacity));
+      TOMHIP_CHECK(hipMemcpyAsync(dv, &v, sizeof(float), hipMemcpyHostToDevice, s));
+      k = collect_equal(in.get<float>(), NN, dv, capacity, idx, s);
+      h.resize((size_t)std::min<long>(k, capacity));
+      TOMHIP_CHECK(hipMemcpyAsync(h.data(), idx, sizeof(long) * h.size(), hipMemcpyDeviceToHost, s));
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(dv);
+      (void)hipFree(idx);
+      throw;
+    }
+    (void)hipFree(dv);
+    (void)hipFree(idx);
+    TOMHIP_REQUIRE(k <= capacity, ERR_WRONG_INPUT,
+                   std::to_string(k) + " tied maxima exceed the capacity " + std::to_string(capacity));
+    std::sort(h.begin(), h.end());  // MATLAB's find order: x fastest
+    *max_val = v;
+    *count = (int64_t)h.size();
+    for (size_t j = 0; j < h.size(); ++j) {
+      pos[3 * j + 0] = (int32_t)(h[j] % N[0]);
+      pos[3 * j + 1] = (int32_t)((h[j] / N[0]) % N[1]);
+      pos[3 * j + 2] = (int32_t)(h[j] / (N[0] * N[1]));
+    }
+  });
+}
+
+namespace {
+// tom_intpeak's per-axis fit (Analysis/tom_peak.m:437-525): the spline through
+// three points is their parabola; its maximum over x = 1:0.001:3 (first on
+// ties) gives the offset and the value.
+void parabola_max(const double p[3], double* val, int* k_best) {
+  const double a = (p[0] - 2 * p[1] + p[2]) / 2.0, b = (p[2] - p[0]) / 2.0;
+  double best = -INFINITY;
+  int kb = 0;
+  for (int k = 0; k <= 2000; ++k) {
+    const double x = 1.0 + k * 0.001 - 2.0;
+    const double y = p[1] + b * x + a * x * x;
+    if (y > best) {
+      best = y;
+      kb = k;
+    }
+  }
+  *val = best;
+  *k_best = kb;
+}
+}  // namespace
+
+int tomhip_peak_radius_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], int radius, int spline,
+                           double c[3], double* val, float* out) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && c && val, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(radius <= 0 || out, ERR_WRONG_INPUT, "the radius form writes the erased map to out");
+    TOMHIP_REQUIRE(radius <= 4096, ERR_WRONG_INPUT, "radius too large");
+    check_dims3(N, "map");
+    TOMHIP_REQUIRE(radius <= 0 || N[2] > 1, ERR_UNSUPPORTED, "the radius form is built for 3D volumes");
+    const long NN = N[0] * N[1] * N[2];
+    hipStream_t s = ctx->stream;
+    DevBuf in(s, map, sizeof(float) * NN, true, false);
+    float v;
+    long i0;
+    global_max(in.get<float>(), NN, &v, &i0, s);
+    TOMHIP_REQUIRE(i0 >= 0, ERR_WRONG_INPUT, "empty map");
+    const int p[3] = {(int)(i0 % N[0]), (int)((i0 / N[0]) % N[1]), (int)(i0 / (N[0] * N[1]))};
+    double vals[3] = {v, v, v};
+    for (int d = 0; d < 3; ++d) c[d] = p[d];
+    if (spline) {
+      const long st[3] = {1, N[0], N[0] * N[1]};
+      for (int d = 0; d < 3; ++d) {
+        if (p[d] == 0 || p[d] == N[d] - 1) continue;  // border: position and value as found
+        float q[3];
+        for (int k = 0; k < 3; ++k)
+          TOMHIP_CHECK(hipMemcpyAsync(&q[k], in.get<float>() + i0 + (k - 1) * st[d], sizeof(float),
+                                      hipMemcpyDeviceToHost, s));
+        TOMHIP_CHECK(hipStreamSynchronize(s));
+        const double pd[3] = {q[0], q[1], q[2]};
+        int kb = 0;
+        parabola_max(pd, &vals[d], &kb);
+        c[d] = (double)p[d] - 1.0 + kb * 0.001;
+      }
+    }
+    *val = spline ? std::max(vals[0], std::max(vals[1], vals[2])) : (double)v;
+    if (radius > 0) {
+      DevBuf o(s, out, sizeof(float) * NN, false, true);
+      TOMHIP_CHECK(hipMemcpyAsync(o.get<float>(), in.get<float>(), sizeof(float) * NN, hipMemcpyDeviceToDevice, s));
+      erase_sphere(o.get<float>(), (int)N[0], (int)N[1], (int)N[2], p[0], p[1], p[2], radius, s);
+      o.commit();
+      TOMHIP_CHECK(hipStreamSynchronize(s));
+    }
   });
 }
 

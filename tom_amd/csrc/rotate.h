@@ -11,4 +11,11 @@ void rotmat_ref(const float* euler, int neuler, float* rm);
 void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, const float* mask, int sx,
                    int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream,
                    const long* desc = nullptr, long rm_step = 0);
+// rot2d (Sptrans/tom_rotatec.c:49-142): phi in degrees (n == 1) or a 2x2
+// matrix given as rm00 rm10 rm01 rm11 (n == 4) -> rm00 rm10 rm01 rm11.
+void rotmat2d_ref(const float* phi_or_R, int n, float* rm);
+// 2D rotation of an sx x sy image; tab_dev: device [sx + sy] pixel
+// coordinates relative to the pivot (pi then pj, as rot2d accumulates them).
+void launch_rotate2d(const float* src, float* dst, long sx, long sy, const float* tab_dev, const float rm[4],
+                     float px, float py, hipStream_t stream);
 }  // namespace tomhip

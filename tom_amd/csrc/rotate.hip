@@ -143,6 +143,75 @@ void rotmat_ref(const float* euler, int neuler, float* rm) {
   rm[8] = cthe;
 }
 
+namespace {
+
+// rot2d of Sptrans/tom_rotatec.c:49-142 (the gateway's 2D branch): bilinear,
+// inverse-mapped, in the reference's float order. The pixel coordinates
+// pi, pj come from host tables because the reference accumulates them by
+// repeated float increments from -px, -py.
+__global__ __launch_bounds__(NT_ROT) void k_rotate2d(const float* __restrict__ img, float* __restrict__ out,
+                                                     long sx, long sy, const float* __restrict__ tab,
+                                                     float rm00, float rm01, float rm10, float rm11, float px,
+                                                     float py) {
+  const long n = sx * sy;
+  for (long idx = (long)blockIdx.x * NT_ROT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT_ROT) {
+    const long i = idx % sx, j = idx / sx;
+    const float pi = tab[i], pj = tab[sx + j];
+    const long sx_1 = sx - 1, sy_1 = sy - 1;
+    const float r_x = px + rm00 * pi + rm10 * pj;
+    if (r_x < 0 || r_x > (float)sx_1) {
+      out[idx] = 0.f;
+      continue;
+    }
+    const float r_y = py + rm01 * pi + rm11 * pj;
+    if (r_y < 0 || r_y > (float)sy_1) {
+      out[idx] = 0.f;
+      continue;
+    }
+    const long floorx = (long)r_x;
+    const float vx2 = r_x - (float)floorx;
+    const long floory = (long)r_y;
+    const float vy2 = r_y - (float)floory;
+    const float* p = img + floorx + sx * floory;
+    const bool inx = r_x < (float)(sx - 1);
+    const float a0 = p[0];
+    const float AA = inx ? a0 + (p[1] - a0) * vx2 : a0;
+    float BB = 0.f;
+    if (r_y < (float)(sy - 1)) {
+      const float b0 = p[sx];
+      BB = inx ? b0 + (p[sx + 1] - b0) * vx2 : b0;
+    }
+    out[idx] = AA + (BB - AA) * vy2;
+  }
+}
+
+}  // namespace
+
+void rotmat2d_ref(const float* phi_or_R, int n, float* rm) {
+  if (n == 4) {  // rm00 rm10 rm01 rm11, as the gateway reads a 2x2 matrix
+    rm[0] = phi_or_R[0];
+    rm[1] = phi_or_R[1];
+    rm[2] = phi_or_R[2];
+    rm[3] = phi_or_R[3];
+    return;
+  }
+  const double kPi = 3.14159265358979323846264338327950288419716939937510;
+  const float phi = (float)((double)phi_or_R[0] * kPi / 180);
+  rm[0] = (float)cos((double)phi);   // rm00
+  rm[1] = (float)sin((double)phi);   // rm10
+  rm[2] = (float)-sin((double)phi);  // rm01
+  rm[3] = (float)cos((double)phi);   // rm11
+}
+
+void launch_rotate2d(const float* src, float* dst, long sx, long sy, const float* tab_dev, const float rm[4],
+                     float px, float py, hipStream_t stream) {
+  const long n = sx * sy;
+  const long bx = std::min((n + NT_ROT - 1) / NT_ROT, 65535L * 16);
+  hipLaunchKernelGGL(k_rotate2d, dim3((unsigned)bx), dim3(NT_ROT), 0, stream, src, dst, sx, sy, tab_dev, rm[0],
+                     rm[2], rm[1], rm[3], px, py);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
 void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, const float* mask, int sx,
                    int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream,
                    const long* desc, long rm_step) {

@@ -75,6 +75,8 @@ class Search {
   void bind_stream(hipStream_t s);
 
   void set_profile(bool on) { profile_ = on; }
+  // FP64 local standard deviation of the volume under the mask (device, N).
+  const double* local_std() const { return sig_; }
   // Replay full angle batches through one captured hipGraph (default on;
   // profiling runs eager so every launch can be timed).
   void set_graph(bool on) { use_graph_ = on; }

@@ -8,12 +8,12 @@
 
 static tomhip_ctx* tomhip_mex_ctx = 0;
 
-static void tomhip_mex_cleanup(void) {
+static inline void tomhip_mex_cleanup(void) {
   if (tomhip_mex_ctx) tomhip_destroy(tomhip_mex_ctx);
   tomhip_mex_ctx = 0;
 }
 
-static tomhip_ctx* tomhip_mex_context(void) {
+static inline tomhip_ctx* tomhip_mex_context(void) {
   if (!tomhip_mex_ctx) {
     if (tomhip_init(0, &tomhip_mex_ctx) != 0) mexErrMsgTxt(tomhip_last_error(0));
     mexAtExit(tomhip_mex_cleanup);
@@ -22,11 +22,11 @@ static tomhip_ctx* tomhip_mex_context(void) {
 }
 
 /* nonzero return code -> MATLAB error, like mexErrMsgTxt in the reference */
-static void tomhip_mex_check(int rc) {
+static inline void tomhip_mex_check(int rc) {
   if (rc != 0) mexErrMsgTxt(tomhip_last_error(tomhip_mex_ctx));
 }
 
-static void tomhip_mex_dims3(const mxArray* a, int64_t d[3]) {
+static inline void tomhip_mex_dims3(const mxArray* a, int64_t d[3]) {
   const mwSize nd = mxGetNumberOfDimensions(a);
   const mwSize* dm = mxGetDimensions(a);
   d[0] = (int64_t)dm[0];
