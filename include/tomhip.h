@@ -71,6 +71,16 @@ int tomhip_rotate3d_batch_f32(tomhip_ctx* ctx, const float* src, float* dst, int
  * pos is 0-based (the reference MEX adds 1). */
 int tomhip_peak_f32(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val, int32_t pos[3]);
 
+/* Allocation-free form of tomhip_peak_f32. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops such as tom_peakc's per-pick calls, harnesses
+ * that forbid hipMalloc). Same result as tomhip_peak_f32; the map must be
+ * device memory (anything else returns -1), outputs are caller-owned host
+ * scalars. The block partials live in a 16 KiB static device array per GPU
+ * (held until the process ends, registered by tomhip_init); calls that
+ * overlap on one GPU share it and race. No hipMalloc / hipFree is made. */
+int tomhip_peak_f32_noalloc(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val,
+                            int32_t pos[3]);
 
 /* tom_peak(A) (Analysis/tom_peak.m:80-108): the maximum and EVERY position
  * holding it, 0-based, in MATLAB's find order (x fastest); pos [capacity][3];

@@ -30,6 +30,7 @@ SIGNATURES = {
     "tomhip_rotate3d_batch_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, vp,
                                                  ctypes.c_int64, vp, vp]),
     "tomhip_peak_f32": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "tomhip_peak_f32_noalloc": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "tomhip_peak_all_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int64, vp]),
     "tomhip_peak_radius_f32": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]),
     "tomhip_flcf_box_f32": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),

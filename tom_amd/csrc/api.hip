@@ -140,6 +140,8 @@ int tomhip_init(int device, tomhip_ctx** out) {
     // stream), so caller buffers written there are complete before we read them
     TOMHIP_CHECK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
     c->stream = c->own;
+    // device address of the allocation-free peak's static scratch, once per device
+    peak_scratch_register(device);
     c->tw = std::make_unique<Twiddles>();
     *out = c.release();
   });
@@ -265,6 +267,28 @@ int tomhip_peak_f32(tomhip_ctx* ctx, const float* map, const int32_t dims[3], fl
     if (!(v > -99999.0f)) {
       *max_val = -99999.0f;
       return;  // positions are left as passed in, like the reference
+    }
+    *max_val = v;
+    pos[0] = (int32_t)(i % dims[0]);
+    pos[1] = (int32_t)((i / dims[0]) % dims[1]);
+    pos[2] = (int32_t)(i / ((long)dims[0] * dims[1]));
+  });
+}
+
+int tomhip_peak_f32_noalloc(tomhip_ctx* ctx, const float* map, const int32_t dims[3], float* max_val,
+                             int32_t pos[3]) {
+  return guard(ctx, [&] {
+    TOMHIP_REQUIRE(ctx && map && dims && max_val && pos, ERR_WRONG_INPUT, "null pointer");
+    TOMHIP_REQUIRE(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, ERR_WRONG_INPUT, "bad dims");
+    TOMHIP_REQUIRE(is_device_ptr(map), ERR_WRONG_INPUT,
+                   "tomhip_peak_f32_noalloc needs the map in device memory (it copies and allocates nothing)");
+    const long n = (long)dims[0] * dims[1] * dims[2];
+    float v;
+    long i;
+    global_max_noalloc(map, n, &v, &i, ctx->stream, ctx->device);
+    if (!(v > -99999.0f)) {
+      *max_val = -99999.0f;
+      return;
     }
     *max_val = v;
     pos[0] = (int32_t)(i % dims[0]);

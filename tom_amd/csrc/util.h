@@ -18,6 +18,12 @@ void launch_spheremask(int nx, int ny, int nz, float radius, float sigma, float 
 void launch_weight(int nx, int ny, int nz, int use_wedge, double semi_angle_deg, int use_bp, double lo,
                    double hi, int shifted, float* out, hipStream_t s);
 void global_max(const float* a, long n, float* val, long* idx, hipStream_t s);
+// The same reduction with no allocation: block partials in a 16 KiB static
+// device array (one per GPU, registered once per device by
+// peak_scratch_register with that device current). Calls that overlap on one
+// GPU share the array and race.
+void peak_scratch_register(int device);
+void global_max_noalloc(const float* a, long n, float* val, long* idx, hipStream_t s, int device);
 // os3_psr's ring mask (sphere nx/2 minus sphere max(nx/4, 5)) of a box
 void launch_psr_ring(int nx, int ny, int nz, float* out, hipStream_t s);
 // out = (x - mu) / sig where sig > 0, else 0

@@ -2,6 +2,8 @@
 // masks, wedge/bandpass weights, global maximum.
 #include <cfloat>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "fft_kernels.h"
 #include "util.h"
@@ -278,6 +280,92 @@ void launch_weight(int nx, int ny, int nz, int use_wedge, double semi_angle_deg,
   hipLaunchKernelGGL(k_weight, dim3(gridn((long)nx * ny * nz)), dim3(NT), 0, s, nx, ny, nz, use_wedge, ta,
                      use_bp, lo, hi, shifted, out);
   TOMHIP_CHECK(hipGetLastError());
+}
+
+namespace {
+
+// Allocation-free global max: the block partials and the reduced result live
+// in one static device array per GPU (each device holds its own copy until
+// the process ends). Host code never passes `scratch` itself to a kernel; it
+// uses the per-device address taken with hipGetSymbolAddress at set-up.
+constexpr int kPeakBlocks = 1024;
+constexpr size_t kPvOff = 0;                                   // float[kPeakBlocks]
+constexpr size_t kPiOff = kPvOff + sizeof(float) * kPeakBlocks;  // long[kPeakBlocks]
+constexpr size_t kResOff = kPiOff + sizeof(long) * kPeakBlocks;  // {float v; long i}
+constexpr size_t kScratchBytes = 16384;
+static_assert(kResOff + 16 <= kScratchBytes, "peak scratch too small");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+__global__ void k_max_final(const float* pv, const long* pi, int nb, float* rv, long* ri) {
+  __shared__ float sv[kPeakBlocks];
+  __shared__ long si[kPeakBlocks];
+  const int t = threadIdx.x;
+  sv[t] = t < nb ? pv[t] : -FLT_MAX;
+  si[t] = t < nb ? pi[t] : -1;
+  __syncthreads();
+  for (int k = kPeakBlocks / 2; k > 0; k >>= 1) {
+    if (t < k) {
+      const float v2 = sv[t + k];
+      const long i2 = si[t + k];
+      const float v1 = sv[t];
+      const long i1 = si[t];
+      if (i2 >= 0 && (i1 < 0 || v2 > v1 || (v2 == v1 && i2 < i1))) {
+        sv[t] = v2;
+        si[t] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    *rv = sv[0];
+    *ri = si[0];
+  }
+}
+
+}  // namespace
+
+void peak_scratch_register(int device) {
+  TOMHIP_REQUIRE(device >= 0 && device < kMaxDevices, ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  TOMHIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  TOMHIP_REQUIRE(p != nullptr, ERR_HIP, "no device address for the peak scratch");
+  g_scratch[device] = p;
+}
+
+void global_max_noalloc(const float* a, long n, float* val, long* idx, hipStream_t s, int device) {
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+  }
+  TOMHIP_REQUIRE(base != nullptr, ERR_INTERNAL, "peak scratch not registered for this device");
+  unsigned char* b = static_cast<unsigned char*>(base);
+  float* pv = reinterpret_cast<float*>(b + kPvOff);
+  long* pi = reinterpret_cast<long*>(b + kPiOff);
+  float* rv = reinterpret_cast<float*>(b + kResOff);
+  long* ri = reinterpret_cast<long*>(b + kResOff + 8);
+  const unsigned nb = gridn(n) > (unsigned)kPeakBlocks ? (unsigned)kPeakBlocks : gridn(n);
+  hipLaunchKernelGGL(k_max_partial, dim3(nb), dim3(NT), 0, s, a, n, pv, pi);
+  TOMHIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kPeakBlocks), 0, s, (const float*)pv, (const long*)pi, (int)nb, rv,
+                     ri);
+  TOMHIP_CHECK(hipGetLastError());
+  struct {
+    float v;
+    int pad;
+    long i;
+  } res;
+  static_assert(sizeof(res) == 16, "result layout");
+  TOMHIP_CHECK(hipMemcpyAsync(&res, rv, sizeof(res), hipMemcpyDeviceToHost, s));
+  TOMHIP_CHECK(hipStreamSynchronize(s));
+  *val = res.v;
+  *idx = res.i;
 }
 
 void global_max(const float* a, long n, float* val, long* idx, hipStream_t s) {
