@@ -382,9 +382,24 @@ def main():
         exhaustive_s = pre_ms / 1e3 + time.perf_counter() - t0
         log(f"[bench] exhaustive judge: {exhaustive_s:.3f} s")
     found = None
+    picks = None
     if exhaustive_s is not None:
         sc = score.cpu().numpy().transpose(2, 1, 0)
         found = float(np.mean([sc[tuple(p)] > 0.2 for p in pos[cls == 0]]))
+        # ---- the picklist of the score map (tom_os3_returnPicklist, 1000
+        # picks), wall time per call on resident maps
+        api.tom_os3_returnPicklist(score, (cfg["n"],) * 3, 1000, angles=ang)
+        torch.cuda.synchronize()
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            rows = api.tom_os3_returnPicklist(score, (cfg["n"],) * 3, 1000, angles=ang)
+            reps.append(time.perf_counter() - t0)
+        picks = {"rows": int(len(rows)), "requested": 1000, "ms": round(min(reps) * 1e3, 2),
+                 "us_per_pick": round(min(reps) * 1e6 / max(1, len(rows)), 2),
+                 "what": "wall time of one call on the resident score and angle maps: map copy, border erase, "
+                         "brick table, every pick (best of 3)"}
+        log(f"[bench] picklist: {picks['rows']} rows in {picks['ms']} ms")
 
     # ---- pruned search (and the proposal network) on the same tomogram,
     # checked against the exhaustive run
@@ -502,7 +517,8 @@ def main():
                    "precompute_ms": round(pre_ms, 1), "finalize_ms": round(fin_ms, 2),
                    "planted_found_frac": found, "max_abs_err_vs_fp64": err, "per_kernel": per_kernel,
                    "timed_path": "hipGraph replay" if graph else "eager launches with per-kernel HIP events",
-                   "graph_replay": graph, "eager_with_events": eager, "autotune": tuned, "pruned_search": sur},
+                   "graph_replay": graph, "eager_with_events": eager, "autotune": tuned, "pruned_search": sur,
+                   "picklist": picks},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -19,8 +19,8 @@ Function -> reference it restates -> how it is pinned (tests/test_oracle.py):
                            golden vectors; axis invariants tom_rotate.m:59-61
   rotate2d              Sptrans/tom_rotatec.c:49-142 (rot2d; not buildable here,
                         its header includes mex.h) -> identity, exact rotation of
-                        a linear ramp (sense and pivot), zero outside, last-row
-                        quirk by hand (test_rotate2d_closed_forms)
+                        a linear ramp (sense and pivot), zero outside, the last
+                        row (no y tap, vy2 = 0) by hand (test_rotate2d_closed_forms)
   sphere_mask_os3       search/tom_os3_sphereMask.m:4-61 -> closed form
   sphere_mask_gauss     src/tom/core/volume_fcn.cpp:932-953, 1013-1030 -> closed form
   wedge                 Filtrans/tom_wedge.m:62-77 -> grid predicate, kz=0 kept
@@ -818,35 +818,85 @@ def tom_bin(im: np.ndarray, nbin: int = 1) -> np.ndarray:
 # picklist (tom_os3_returnPicklist -> tom_peakc)
 # ----------------------------------------------------------------------------
 def erase_borders(volume, tsize):
-    """tom_os3_eraseBorders: keep the centred box of size N - (n+1)."""
-    inner = np.ones([s - (t + 1) for s, t in zip(volume.shape, tsize)])
-    return np.asarray(volume, np.float64) * paste_center(volume.shape, inner)
+    """tom_os3_eraseBorders (tools/tom_os3_eraseBorders.m): volume .* a mask
+    that keeps the centred box of size N - (n+1); a product, so NaN stays NaN
+    and negative border values become -0."""
+    shape = np.shape(volume)
+    inner = [max(s - (t + 1), 0) for s, t in zip(shape, tsize)]
+    if len(shape) == 3 and shape[2] == 1:
+        inner[2] = 1  # an image (one z plane): borders in x and y only
+    keep = paste_center(shape, np.ones(inner, np.float32)).astype(np.float32)
+    return np.asarray(volume, np.float32) * keep
 
 
-def peakc(a, radius):
-    """tom_peakc(in, radius) (Analysis/tom_peakc.m:39-86) for interior peaks:
-    first maximum (x-fastest), then zero the voxels of the cut-out box
-    [c-r, c+r) closer than r to its centre (size/2+1, 1-based). Returns
-    (1-based position, value, modified copy)."""
-    v, p = find_max(np.asarray(a, np.float32))
-    a = np.array(a, np.float64, copy=True)
-    lo = [max(int(p[d]) + 1 - radius, 1) - 1 for d in range(3)]
-    hi = [min(lo[d] + 2 * radius, a.shape[d]) for d in range(3)]
-    box = a[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
-    cen = [s / 2.0 + 1 for s in box.shape]
-    I, J, K = np.meshgrid(*[np.arange(s) for s in box.shape], indexing="ij")
+def _peakc_region(c, radius, N):
+    """The cut-out and paste of tom_peakc along one axis (1-based peak c):
+    tom_cut_out (Sptrans/tom_cut_out.m) clamps the start c - r to 1 and keeps
+    the size 2r, clipped at the far end; tom_paste (Sptrans/tom_paste.m) puts
+    the masked box back at the unclamped start c - r, clipped to the volume.
+    Returns (target first, target last, source shift, box size), 1-based."""
+    coord = c - radius
+    pos = max(coord, 1)
+    bound = pos + 2 * radius - 1 if pos + 2 * radius <= N else N
+    dims = bound - pos + 1
+    t0, t1 = max(coord, 1), min(coord + dims - 1, N)
+    return t0, t1, pos - coord, dims, coord
+
+
+def peakc(a, radius, inplace=False):
+    """tom_peakc(in, radius) (Analysis/tom_peakc.m:39-86) on a 3D map (a 2D
+    map is (nx, ny, 1)): the first maximum c (1-based, x-fastest, tom_peakinc),
+    then in = tom_paste(in, out .* (sphere == 0), c - r) with out =
+    tom_cut_out(in, c - r, [2r 2r 2r], 'nofill') and the sphere of
+    tom_spheremask (Sptrans/tom_spheremask.m: 1 where the distance to the
+    box centre size/2 + 1 is below r; for a 2D map the z centre is 1.5).
+    Near the low border the cut-out starts at 1 while the paste starts at
+    c - r, so the kept box voxels land shifted by 1 - (c - r). A map with a
+    single z plane keeps z fixed (the reference's 2D branch). Returns
+    (1-based position, value, map); the map is float32, Fortran-ordered."""
+    a = np.asarray(a, np.float32)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    if not inplace or not a.flags.f_contiguous:
+        a = np.array(a, np.float32, order="F", copy=True)
+    v, p = find_max(a)
+    c = [int(q) + 1 for q in p]
+    N = a.shape
+    reg = [_peakc_region(c[d], radius, N[d]) for d in range(3)]
+    if N[2] == 1:
+        reg[2] = (1, 1, 0, 1, 1)
+    dims = [r[3] for r in reg]
+    cen = [d / 2.0 + 1.0 for d in dims]
+    I, J, K = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
     dist = np.sqrt((I + 1 - cen[0]) ** 2 + (J + 1 - cen[1]) ** 2 + (K + 1 - cen[2]) ** 2)
-    box[dist < radius] = 0
-    return p + 1, v, a
+    keep = (dist >= radius).astype(np.float32)
+    tgt = tuple(slice(r[0] - 1, r[1]) for r in reg)
+    src = tuple(slice(r[0] - 1 + r[2], r[1] + r[2]) for r in reg)
+    box = tuple(slice(r[0] - r[4], r[1] - r[4] + 1) for r in reg)
+    if all(r[1] >= r[0] for r in reg):
+        a[tgt] = a[src] * keep[box]
+    return np.asarray(c), v, a
 
 
-def return_picklist(peaks, angles, tsize, n):
-    """tom_os3_returnPicklist rows [x y z value angle] (no binning)."""
+def return_picklist(peaks, angles, tsize, n, erase=None):
+    """tom_os3_returnPicklist rows [x y z value angle] (no binning). The loop
+    runs while std(peaks(:)) > 0, i.e. while the map holds only finite
+    values (a NaN or Inf makes the std NaN) that are not all equal. erase:
+    None = the reference's rule (borders erased unless the map has the
+    template's size), False = never."""
     r = int(np.ceil(tsize[0] / 2))
-    pk = erase_borders(peaks, tsize) if tuple(peaks.shape) != tuple(tsize) else np.asarray(peaks, np.float64)
+    pk = np.asarray(peaks, np.float32)
+    if pk.ndim == 2:
+        pk = pk[:, :, None]
+    if erase is not False and tuple(pk.shape) != tuple(tsize):
+        pk = erase_borders(pk, tsize)
+    pk = np.array(pk, np.float32, order="F", copy=True)
+    angs = None if angles is None else np.asarray(angles)
+    if angs is not None and angs.ndim == 2:
+        angs = angs[:, :, None]
     rows = []
-    while len(rows) < n and np.std(pk) > 0:
-        c, v, pk = peakc(pk.astype(np.float32), r)
-        ang = 0 if angles is None else angles[c[0] - 1, c[1] - 1, c[2] - 1]
+    while len(rows) < n and np.isfinite(pk).all() and pk.max() > pk.min():
+        c, v, pk = peakc(pk, r, inplace=True)
+        ang = 0 if angs is None else angs[c[0] - 1, c[1] - 1, c[2] - 1]
         rows.append([c[0], c[1], c[2], v, ang])
     return np.asarray(rows, np.float64).reshape(-1, 5)

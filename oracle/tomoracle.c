@@ -24,7 +24,8 @@
  *     committed golden vectors; axis invariants of Sptrans/tom_rotate.m:59-61.
  *   orc_rot2d: identity at 0 degrees; a linear ramp rotates exactly (bilinear
  *     interpolation reproduces linear functions) with the reference's sense and
- *     pivot; out-of-image pixels are 0; the last-row quirk (no y tap) by hand.
+ *     pivot; out-of-image pixels are 0. On the last row (r_y == sy-1) rot2d
+ *     takes no y tap (BB = 0), which leaves the value unchanged since vy2 = 0.
  *     Sptrans/tom_rotatec.c cannot be compiled here (its header defines MATLAB
  *     and includes mex.h), so no golden vectors exist for it.
  *   orc_find_max: numpy argmax (first occurrence) on x-fastest data; ties.

@@ -113,7 +113,10 @@ def test_os3_picker_scores_match_oracle(api, oracle):
         err = np.abs(got - ref)
         say(f"  {k}: max |gpu - fp64| {err.max():.2e}")
         assert np.all(err <= tol)
-    assert res["ccc"].cpu().numpy().transpose(2, 1, 0)[28, 26, 38] > 0.8  # the planted copy
+    # the planted copy (template centre ceil(n/2) at V[28, 26, 38]) is the
+    # best-scoring voxel; its height is set by the noise under the mask
+    c = res["ccc"].cpu().numpy().transpose(2, 1, 0)
+    assert np.unravel_index(np.argmax(c), c.shape) == (28, 26, 38) and c[28, 26, 38] > 0.3
     res2 = api.os3_picker(Vd, T, M, angles, rule="ccc+psr+soc")
     ang = res2["angle"].cpu().numpy()
     assert set(np.unique(ang)) <= {0, 1, 2}
@@ -124,7 +127,7 @@ def test_os3_corr_low_std_branch(api, oracle):
     """tom_os3_corr.m:110-125. (1) An image of mean local std below 0.01 is
     scored with the renormalised template (T*1000+1000).*M, 'mean0+1std',
     which changes the score under a soft mask. (2) When even the renormalised
-    image is featureless (a lone spike in a near-zero volume: mean local std
+    image is featureless (a lone spike in a near-zero volume, mean local std
     below 0.01 of the global std), voxels of local std below 0.05 of the
     global std score 0 while the spike's neighbourhood keeps its FLCF."""
     say("os3_corr low-std branch")
@@ -146,6 +149,7 @@ def test_os3_corr_low_std_branch(api, oracle):
     V = (1e-3 * rng.standard_normal((64, 32, 32))).astype(np.float32)
     got = api.tom_os3_corr(V, T, Ms)
     ref, sig_I, s_img = expected(V, T, Ms)
+    assert sig_I.mean() < 0.01 and sig_I.mean() / s_img >= 0.01  # first test only
     assert not np.any(sig_I < 0.05 * s_img)
     err = np.abs(got - ref).max()
     plain = oracle.flcf_angle(V.astype(np.float64), T.astype(np.float64), Ms.astype(np.float64),
@@ -157,10 +161,13 @@ def test_os3_corr_low_std_branch(api, oracle):
     assert err < 1e-4 and np.abs(plain - ref).max() > 1e-3
     Mb = oracle.sphere_mask_binary(n, 5)
     N = (256, 256, 128)
-    V = (1e-6 * rng.standard_normal(N)).astype(np.float32)
+    V = (1e-7 * rng.standard_normal(N)).astype(np.float32)
     V[100, 120, 60] = 1.0
     got = api.tom_os3_corr(V, T, Mb)
     ref, sig_I, s_img = expected(V, T, Mb)
+    # both tests of tom_os3_corr.m:110-125 fire: mean local std below 0.01,
+    # and below 0.01 again after the image is brought to unit std
+    assert sig_I.mean() / s_img < 0.01
     hot = sig_I < 0.05 * s_img
     assert hot.mean() > 0.99 and (~hot).sum() > 100
     say(f"  hot voxels {int(hot.sum())}, kept {int((~hot).sum())}")

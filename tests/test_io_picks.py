@@ -76,3 +76,61 @@ def test_picklist_matches_oracle(oracle):
     np.testing.assert_array_equal(got[:, :3], ref[:, :3])
     np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=0, atol=1e-7)
     np.testing.assert_array_equal(got[:, 4], ref[:, 4])
+
+
+@pytest.mark.gpu
+def test_picklist_cases_match_oracle(oracle):
+    """The brick-table pick kernel row for row against the oracle: no border
+    erase (peaks at the low borders take tom_paste's shifted write-back),
+    all-negative maps (erased zeros become the maximum), integer ties (first
+    in x-fastest order), an image (one z plane), a map that runs flat before
+    n picks, and maps that give no rows (constant; a NaN)."""
+    from tom_amd import api
+    rng = np.random.default_rng(17)
+    N = (37, 35, 20)
+    ang = rng.integers(0, 5000, N).astype(np.int32)
+    cases = [("uniform, no border erase", rng.random(N).astype(np.float32), (8, 8, 8), 150, False),
+             ("all negative", -rng.random(N).astype(np.float32) - 0.5, (8, 8, 8), 40, False),
+             ("integer ties", rng.integers(0, 4, N).astype(np.float32), (6, 6, 6), 200, False),
+             ("normal, borders erased", rng.standard_normal(N).astype(np.float32), (10, 10, 10), 300, True),
+             ("spikes run flat", np.zeros(N, np.float32), (8, 8, 8), 50, True)]
+    cases[-1][1][[10, 20, 15], [12, 20, 9], [8, 10, 11]] = [3.0, 1.0, 2.0]
+    for name, a, t, n, erase in cases:
+        print(f"  picklist: {name}, map {N}, template {t}, n {n}", flush=True)
+        got = api.tom_os3_returnPicklist(a, t, n, angles=ang, erase_borders=erase)
+        ref = oracle.return_picklist(a, ang, t, n, erase=None if erase else False)
+        assert got.shape == ref.shape, name
+        np.testing.assert_array_equal(got[:, :3], ref[:, :3], err_msg=name)
+        np.testing.assert_array_equal(got[:, 3], ref[:, 3].astype(np.float32), err_msg=name)
+        np.testing.assert_array_equal(got[:, 4], ref[:, 4], err_msg=name)
+    assert len(api.tom_os3_returnPicklist(cases[-1][1], (8, 8, 8), 50)) == 3
+    img = rng.random((40, 33)).astype(np.float32)
+    print("  picklist: image 40x33", flush=True)
+    got = api.tom_os3_returnPicklist(img, (8, 8, 1), 80, erase_borders=False)
+    ref = oracle.return_picklist(img, None, (8, 8, 1), 80, erase=False)
+    assert got.shape == ref.shape and np.array_equal(got[:, :4], ref[:, :4].astype(np.float32))
+    flat = np.full(N, 0.25, np.float32)
+    assert api.tom_os3_returnPicklist(flat, N, 10).shape == (0, 5)
+    nan = rng.random(N).astype(np.float32)
+    nan[18, 17, 10] = np.nan
+    assert api.tom_os3_returnPicklist(nan, (8, 8, 8), 10).shape == (0, 5)
+
+
+@pytest.mark.gpu
+def test_picklist_large_map_matches_oracle(oracle):
+    """A 160x150x140 map with 300 picks (bricks of many rewrites) against
+    the oracle."""
+    from tom_amd import api
+    rng = np.random.default_rng(23)
+    N = (160, 150, 140)
+    a = (0.1 * rng.standard_normal(N)).astype(np.float32)
+    x, y, z = [np.arange(s) for s in N]
+    for _ in range(200):
+        c = [rng.integers(0, s) for s in N]
+        a[max(c[0] - 3, 0):c[0] + 4, max(c[1] - 3, 0):c[1] + 4, max(c[2] - 3, 0):c[2] + 4] += rng.uniform(0.2, 1.0)
+    print(f"  picklist: map {N}, 300 picks", flush=True)
+    got = api.tom_os3_returnPicklist(a, (24, 24, 24), 300)
+    ref = oracle.return_picklist(a, None, (24, 24, 24), 300)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, :3], ref[:, :3])
+    np.testing.assert_array_equal(got[:, 3], ref[:, 3].astype(np.float32))

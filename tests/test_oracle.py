@@ -377,6 +377,55 @@ def test_picklist_planted_peaks(oracle):
     assert after[28, 12, 22] == 0 and after[28, 12, 20] == 0 and after[28, 12, 24] != 0
 
 
+def test_peakc_border_paste_by_hand(oracle):
+    """tom_peakc near the borders, values worked out by hand from
+    Sptrans/tom_cut_out.m (start clamped to 1, size 2r kept) and
+    Sptrans/tom_paste.m (pasted at the unclamped start c - r).
+    Peak at MATLAB (1,1,1), r = 3: the cut-out is in(1:6,1:6,1:6) with sphere
+    centre 4; it is pasted at -2, so a(1:3,1:3,1:3) = box(4:6,4:6,4:6). Of
+    those 27 voxels only the four at box distance >= 3 from the centre keep a
+    value, a(3,3,3) = in(6,6,6), a(3,3,2) = in(6,6,5), a(3,2,3) = in(6,5,6),
+    a(2,3,3) = in(5,6,6); the other 23 become 0; nothing else changes.
+    Peak at the last x voxel: the clipped box is 4 wide, its centre 3 sits one
+    voxel inside, so a voxel 3 away along -x is zeroed while one 3 away along
+    +y (distance sqrt(10) to the centre) is kept."""
+    N = (12, 11, 10)
+    a = (np.arange(np.prod(N), dtype=np.float32).reshape(N, order="F") + 1) / 1000
+    a[0, 0, 0] = 50.0
+    c, v, b = oracle.peakc(a, 3)
+    assert list(c) == [1, 1, 1] and v == 50.0
+    exp = a.copy()
+    exp[0:3, 0:3, 0:3] = 0
+    exp[2, 2, 2], exp[2, 2, 1], exp[2, 1, 2], exp[1, 2, 2] = a[5, 5, 5], a[5, 5, 4], a[5, 4, 5], a[4, 5, 5]
+    np.testing.assert_array_equal(b, exp)
+    a2 = a.copy()
+    a2[0, 0, 0] = 0.5
+    a2[11, 5, 5] = 60.0
+    c, v, b = oracle.peakc(a2, 3)
+    assert list(c) == [12, 6, 6] and v == 60.0
+    assert b[11, 5, 5] == 0 and b[8, 5, 5] == 0 and b[11, 8, 5] == a2[11, 8, 5] and b[7, 5, 5] == a2[7, 5, 5]
+    # the erased voxels are the box voxels closer than 3 to the centre (10.5 .. ): count them
+    changed = b != a2
+    assert changed.sum() == np.count_nonzero(
+        [(i - 2) ** 2 + (j - 3) ** 2 + (k - 3) ** 2 < 9 for i in range(4) for j in range(6) for k in range(6)])
+
+
+def test_picklist_stops_when_flat_or_not_finite(oracle):
+    """The loop runs while std(peaks(:)) > 0: a constant map and a map with
+    a NaN give no rows; a map of four separated spikes on zeros gives exactly
+    four rows (after them every voxel is 0)."""
+    N, n = (30, 30, 30), 4
+    # (a template of the map's size: no border erase)
+    assert oracle.return_picklist(np.full(N, 2.0, np.float32), None, N, 10).shape == (0, 5)
+    a = np.zeros(N, np.float32)
+    for i, p in enumerate([(8, 8, 8), (20, 9, 12), (10, 21, 19), (19, 20, 8)]):
+        a[p] = 1 + i
+    rows = oracle.return_picklist(a, None, (n, n, n), 10)
+    assert rows.shape == (4, 5) and list(rows[:, 3]) == [4, 3, 2, 1]
+    a[15, 15, 15] = np.nan
+    assert oracle.return_picklist(a, None, (n, n, n), 10).shape == (0, 5)
+
+
 def test_tom_bin_closed_forms(oracle):
     """tom_bin restatement: a constant stays constant, a linear ramp
     a + bx + cy + dz bins to its value at the block centres, odd trailing

@@ -151,9 +151,10 @@ def tom_rotate(vol, euler, interp: str = "linear", center=None, ctx: Optional[Co
             raise ValueError("a 2D rotation takes one angle or a 2x2 matrix")
         c = np.asarray([np.ceil(d / 2) for d in b.dims] if center is None else np.asarray(center).ravel()[:2],
                        np.float32)
+        c = np.ascontiguousarray(c)
         out = _empty_like(b, b.dims)
         ctx.check(ctx.lib.tomhip_rotate2d_f32(ctx.h, b.ptr, _ptr(out), int(b.dims[0]), int(b.dims[1]), _p(e),
-                                              int(e.size), _p(np.ascontiguousarray(c))))
+                                              int(e.size), _p(c)))
         return out
     e = np.asarray(euler, dtype=np.float32)
     neuler = 9 if e.size == 9 else 3
@@ -259,9 +260,12 @@ def tom_os3_bestPeak(res, ccc, psr=None, soc=None, angle: int = 0, rule: str = "
         x = res.get(k)
         if x is not None and not _is_torch(x) and not (x.flags.f_contiguous or x.ndim < 2):
             raise ValueError(f"res['{k}'] must be a contiguous device tensor or a Fortran-ordered array")
+    # keep the staged psr / soc arrays alive until the call returns
+    pb = None if psr is None else _Buf(psr)
+    qb = None if soc is None else _Buf(soc)
     ctx.check(ctx.lib.tomhip_os3_bestpeak_f32(ctx.h, _p(N), code, int(angle), cb.ptr,
-                                              None if psr is None else _Buf(psr).ptr,
-                                              None if soc is None else _Buf(soc).ptr, ptr(res["ccc"]),
+                                              None if pb is None else pb.ptr,
+                                              None if qb is None else qb.ptr, ptr(res["ccc"]),
                                               ptr(res.get("psr")) if psr is not None else None,
                                               ptr(res.get("soc")) if soc is not None else None,
                                               ptr(res["angle"])))
@@ -650,10 +654,11 @@ def tom_oscar(vol, template, phi_start, phi_end, phi_incr, psi_start, psi_end, p
     ang = np.asarray([(p, s, t) for p in _colon(phi_start, phi_incr, phi_end)
                       for s in _colon(psi_start, psi_incr, psi_end)
                       for t in _colon(the_start, the_incr, the_end)], np.float32).reshape(-1, 3)
+    ang = np.ascontiguousarray(ang)
     N, n = _dims3(vb.dims), _dims3(tb.dims)
     sc = _empty_like(vb, vb.dims)
     an = _empty_like(vb, vb.dims, np.int32)
-    ctx.check(ctx.lib.tomhip_oscar_f32(ctx.h, vb.ptr, _p(N), tb.ptr, _p(n), _p(np.ascontiguousarray(ang)),
+    ctx.check(ctx.lib.tomhip_oscar_f32(ctx.h, vb.ptr, _p(N), tb.ptr, _p(n), _p(ang),
                                        ang.shape[0], _ptr(sc), _ptr(an)))
     idx = an.cpu().numpy() if _is_torch(an) else an
     ccf = sc.cpu().numpy() if _is_torch(sc) else sc

@@ -905,18 +905,10 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
     DevBuf m(s, map, sizeof(float) * NN, true, false);
     DevBuf a(s, angle_map, sizeof(int32_t) * NN, true, false);
     DevBuf o(s, out, sizeof(float) * 5 * (n > 0 ? n : 1), false, true);
-    float* work = nullptr;
-    TOMHIP_CHECK(hipMalloc(&work, sizeof(float) * NN));
-    try {
-      *count = picklist(m.get<float>(), (int)N[0], (int)N[1], (int)N[2], angle_map ? a.get<int32_t>() : nullptr,
-                        (int)tsize[0], (int)tsize[1], (int)tsize[2], n, erase_borders, work, o.get<float>(), s);
-      o.commit();
-      TOMHIP_CHECK(hipStreamSynchronize(s));
-    } catch (...) {
-      (void)hipFree(work);
-      throw;
-    }
-    (void)hipFree(work);
+    *count = picklist(m.get<float>(), (int)N[0], (int)N[1], (int)N[2], angle_map ? a.get<int32_t>() : nullptr,
+                      (int)tsize[0], (int)tsize[1], (int)tsize[2], n, erase_borders, o.get<float>(), s);
+    o.commit();
+    TOMHIP_CHECK(hipStreamSynchronize(s));
   });
 }
 
