@@ -109,10 +109,16 @@ def cpu_baseline(cfg, V_host, T, M, W, angles, k, A):
 
 # coarse-to-fine pruned search per workload: coarse orientations, candidates
 # kept, refinement levels (+-half, step in degrees)
+# Pruned-search settings per workload. Candidates are the coarse map's local
+# maxima above median + sigmas * robust sigma of that map, thinned to one per
+# nms radius (the picklist's own erase radius, ceil(n/2)); K only caps memory.
+# K_cmp is the equal candidate budget at which the proposal network and the
+# coarse maxima are compared.
 PRUNED = {
-    "c1": dict(n_coarse=256, K=64, levels=((10.0, 10.0), (5.0, 5.0)), bin=0),
-    "c2": dict(n_coarse=2048, K=400, levels=((15.0, 15.0), (5.0, 5.0)), bin=1),
-    "c5": dict(n_coarse=2048, K=2400, levels=((15.0, 15.0), (7.5, 7.5), (3.75, 3.75)), bin=1),
+    "c1": dict(n_coarse=256, sigmas=8.0, K=5000, K_cmp=64, levels=((10.0, 10.0), (5.0, 5.0)), bin=0),
+    "c2": dict(n_coarse=2048, sigmas=8.0, K=20000, K_cmp=400, levels=((15.0, 15.0), (5.0, 5.0)), bin=1),
+    "c5": dict(n_coarse=2048, sigmas=2.0, K=20000, K_cmp=2400, levels=((15.0, 15.0), (7.5, 7.5), (3.75, 3.75)),
+               bin=1),
 }
 
 
@@ -143,11 +149,15 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
                 "timings_s": {k: round(v, 4) for k, v in ps.timings.items()}}
 
     par = PRUNED.get(args.config, PRUNED["c2"])
-    ps = S.PrunedSearch(n_coarse=args.coarse or par["n_coarse"], K=par["K"], levels=par["levels"], bin=par["bin"])
+    ps = S.PrunedSearch(n_coarse=args.coarse or par["n_coarse"], K=par["K"], levels=par["levels"], bin=par["bin"],
+                        sigmas=par["sigmas"], nms_radius=float(math.ceil(cfg["n"] / 2)))
     if ps.bin:
         ps.set_binned_template(T, M, W)
     out = timed(ps)
     out["bin"] = ps.bin
+    out.update(candidate_rule=f"coarse local maxima above median + {par['sigmas']:g} robust sigma "
+                              f"({ps.candidate_threshold:.4f}), one per {ps.nms_radius:g} voxels; K is a memory cap",
+               coarse_noise=[round(v, 5) for v in ps.noise])
     out.update(exhaustive_wall_s=round(exhaustive_s, 4), exhaustive_peaks_gt_0_3=int(len(ref)),
                precompute_s=round(pre_s, 4))
     log(f"[bench] pruned search: recall {out['recall_gt_0.3']:.4f} of {len(ref)} exhaustive peaks > 0.3, "
@@ -157,7 +167,7 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
         meta = json.loads(str(w["meta"]))
         if meta.get("aux") == "coarse":
             nc = int(meta.get("coarse", 256))
-            K = par["K"]
+            K = par["K_cmp"]
             lv = ((10.0, 10.0), (5.0, 5.0))
             net = S.Surrogate(w["params"], w["bins"])
             base = timed(S.PrunedSearch(n_coarse=nc, K=K, levels=lv))

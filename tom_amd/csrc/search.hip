@@ -445,10 +445,20 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
   constexpr int TT = NL * NTASK;
   constexpr int PERT = (TT + NT_C - 1) / NT_C;
   static_assert(NJ0 % 2 == 0 && R0 % 2 == 0, "pass C pairing needs even NJ0 and R0");
-  using Lay = Contig<XH>;
+  // LDS: the FFT lines are element-major (lanes of a wave run across the
+  // block's lines, as in the y pass), raw keeps the global x-fastest rows at
+  // a pitch of XH + 2. After the batch both areas hold the running max/argmax
+  // in x-fastest order for a coalesced update of the resident maps.
+  constexpr int LP = NL + 1;
+  constexpr int RP = XH + 2;
+  constexpr int FP = XH + 1;
+  using Lay = Inter<LP>;
   using IN = StageInfo<XH, S - 1, NL, NT_C, Lay>;
-  __shared__ c32 buf[NL * XH];
-  __shared__ __align__(16) c32 raw[NL * XH];
+  constexpr int SM = (XH * LP + NL * RP > 2 * NL * FP) ? XH * LP + NL * RP : 2 * NL * FP;
+  __shared__ __align__(16) c32 smem[SM];
+  c32* buf = smem;
+  c32* raw = smem + XH * LP + ((XH * LP) & 1);  // 16-byte aligned
+  static_assert(XH * LP + ((XH * LP) & 1) + NL * RP <= SM, "pass C LDS carve");
   __shared__ c32 twH[XH];
   __shared__ c32 twR[XH];
   load_table<float, XH, NT_C>(twH, twHg);
@@ -479,8 +489,8 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
   for (int a = 0; a < ka; ++a) {
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-      const int i = threadIdx.x + k * NT_C;
-      if (i < NV) reinterpret_cast<float4*>(raw)[i] = pf[k];
+      const int i = threadIdx.x + k * NT_C;  // float4 i: row i / (XH/2), elements 2 (i % (XH/2)) + 0, 1
+      if (i < NV) reinterpret_cast<float4*>(raw + (i / (XH / 2)) * RP)[i % (XH / 2)] = pf[k];
     }
     if (a + 1 < ka) fetch(a + 1);
     __syncthreads();
@@ -488,10 +498,10 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
     for (int p = 0; p < PERT; ++p) {
       const int t = threadIdx.x + p * NT_C;
       if ((TT % NT_C == 0) || t < TT) {
-        const int line = t / NTASK, j = t % NTASK;
+        const int line = t % NL, j = t / NL;
         const bool ok = line < nlv;
-        const c32* g = raw + line * XH;
-        c32* row = buf + line * XH;
+        const c32* g = raw + line * RP;
+        auto row = [&](int e) -> c32& { return buf[Lay::addr(line, e)]; };
         c32 va[R0], vb[R0], z1[R0], z2[R0];
 #pragma unroll
         for (int r = 0; r < R0; ++r) va[r] = ok ? g[j + r * NJ0] : mk(0.f, 0.f);
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
           z1[R0 / 2] = mk(2.f * va[R0 / 2].x, -2.f * va[R0 / 2].y);
           Bfly<float, R0, 1>::run(z1);
 #pragma unroll
-          for (int r = 0; r < R0; ++r) row[r] = z1[r];
+          for (int r = 0; r < R0; ++r) row(r) = z1[r];
         } else if (j == NJ0 / 2) {
 #pragma unroll
           for (int r = 0; r < R0 / 2; ++r) {
@@ -520,7 +530,7 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
           }
           Bfly<float, R0, 1>::run(z1);
 #pragma unroll
-          for (int r = 0; r < R0; ++r) row[j * R0 + r] = z1[r];
+          for (int r = 0; r < R0; ++r) row(j * R0 + r) = z1[r];
         } else {
           const int j2 = NJ0 - j;
 #pragma unroll
@@ -537,8 +547,8 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
           Bfly<float, R0, 1>::run(z2);
 #pragma unroll
           for (int r = 0; r < R0; ++r) {
-            row[j * R0 + r] = z1[r];
-            row[j2 * R0 + r] = z2[r];
+            row(j * R0 + r) = z1[r];
+            row(j2 * R0 + r) = z2[r];
           }
         }
       }
@@ -562,26 +572,38 @@ __global__ __launch_bounds__(NT_C) __attribute__((amdgpu_waves_per_eu(4))) void 
     });
     __syncthreads();
   }
+  // stage the batch's max / argmax in x-fastest order (row pitch FP), then
+  // compare with the resident maps in coalesced float2 / int2 rows
+  float2* sval = reinterpret_cast<float2*>(smem);
+  int2* sidx = reinterpret_cast<int2*>(smem + NL * FP);
 #pragma unroll
   for (int p = 0; p < IN::PER; ++p) {
     if (IN::valid(p)) {
       int line, j;
       IN::idx(p, line, j);
-      if (line < nlv) {
 #pragma unroll
-        for (int r = 0; r < IN::R; ++r) {
-          const long v = ((l0 + line) * XH + j + r * IN::NS) * 2;
-          const float2 cur = *reinterpret_cast<const float2*>(maxc + v);
-          if (bx[p][r] > cur.x) {
-            maxc[v] = bx[p][r];
-            argm[v] = first_index + (int)((ix[p] >> (4 * r)) & 15u);
-          }
-          if (by[p][r] > cur.y) {
-            maxc[v + 1] = by[p][r];
-            argm[v + 1] = first_index + (int)((iy[p] >> (4 * r)) & 15u);
-          }
-        }
+      for (int r = 0; r < IN::R; ++r) {
+        const int e = line * FP + j + r * IN::NS;
+        sval[e] = make_float2(bx[p][r], by[p][r]);
+        sidx[e] = make_int2(first_index + (int)((ix[p] >> (4 * r)) & 15u),
+                            first_index + (int)((iy[p] >> (4 * r)) & 15u));
       }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nlv * XH; i += NT_C) {
+    const int line = i / XH, e = i % XH;
+    const long v = ((l0 + line) * XH + e) * 2;
+    const float2 cur = *reinterpret_cast<const float2*>(maxc + v);
+    const float2 nv = sval[line * FP + e];
+    const int2 ni = sidx[line * FP + e];
+    if (nv.x > cur.x) {
+      maxc[v] = nv.x;
+      argm[v] = ni.x;
+    }
+    if (nv.y > cur.y) {
+      maxc[v + 1] = nv.y;
+      argm[v + 1] = ni.y;
     }
   }
 }
