@@ -227,54 +227,100 @@ __global__ __launch_bounds__(NT) void k_box_xy_inv(const c32* in, float* t, cons
 }
 
 // ---------------------------------------------------------------- template fwd
-// grid (Xh/TW, nz, ka). Output Ty[a][uz][ky][kx] for kx < Xh, Tyn[a][uz][ky]
-// for kx = Xh (tile 0 only).
+// x: direct DFT of the n non-zero samples of every template row for each
+// kx in [0, X/2], as register tiles of 4 rows x 4 frequencies (a row's
+// samples and a frequency's twiddles are each read once per tile and used 4
+// times). grid (nz, ka). Output Tx[a][uz][uy][kx], row length Xh + 1.
+// Summation order per output is ux ascending with fmaf, the twiddle index
+// kx (ux - hx) mod X advanced by kx per sample.
+constexpr int TXU = 4, TXK = 4;
+__global__ __launch_bounds__(NT) void k_tmpl_xdft(const float* t2, c32* Tx, const c32* twXg, int X, int nx, int ny,
+                                                  int nz) {
+  __shared__ float plane[64 * 65];  // rows at pitch nx + 1
+  __shared__ c32 twX[1024];
+  const int Xh = X / 2, XP = Xh + 1, PX = nx + 1;
+  const int uz = blockIdx.x, a = blockIdx.y;
+  for (int i = threadIdx.x; i < X; i += NT) twX[i] = twXg[i];
+  const float* src = t2 + ((long)a * nz + uz) * ny * nx;
+  for (int i = threadIdx.x; i < nx * ny; i += NT) plane[(i / nx) * PX + i % nx] = src[i];
+  __syncthreads();
+  const int hx = nx / 2;
+  const int nUT = (ny + TXU - 1) / TXU, nKT = (XP + TXK - 1) / TXK;
+  c32* dst = Tx + ((long)a * nz + uz) * ny * XP;
+  // lanes run over row tiles first: the twiddles of a wave come from a few
+  // frequency tiles (few LDS addresses), the padded rows from distinct banks
+  for (int q = threadIdx.x; q < nUT * nKT; q += NT) {
+    const int ut = q % nUT, kt = q / nUT;
+    int kx[TXK], m[TXK];
+#pragma unroll
+    for (int i = 0; i < TXK; ++i) {
+      kx[i] = kt * TXK + i;
+      const int kk = kx[i] <= Xh ? kx[i] : 0;
+      int mm = (int)(((long)kk * hx) % X);
+      m[i] = (mm == 0) ? 0 : X - mm;
+      kx[i] = kk;
+    }
+    float re[TXU][TXK], im[TXU][TXK];
+#pragma unroll
+    for (int r = 0; r < TXU; ++r)
+#pragma unroll
+      for (int i = 0; i < TXK; ++i) re[r][i] = im[r][i] = 0.f;
+    const float* rows = plane + ut * TXU * PX;
+    for (int ux = 0; ux < nx; ++ux) {
+      float v[TXU];
+#pragma unroll
+      for (int r = 0; r < TXU; ++r) v[r] = (ut * TXU + r < ny) ? rows[r * PX + ux] : 0.f;
+#pragma unroll
+      for (int i = 0; i < TXK; ++i) {
+        const c32 w = twX[m[i]];
+#pragma unroll
+        for (int r = 0; r < TXU; ++r) {
+          re[r][i] = fmaf(v[r], w.x, re[r][i]);
+          im[r][i] = fmaf(-v[r], w.y, im[r][i]);
+        }
+        m[i] += kx[i];
+        if (m[i] >= X) m[i] -= X;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TXU; ++r) {
+      const int uy = ut * TXU + r;
+      if (uy < ny) {
+#pragma unroll
+        for (int i = 0; i < TXK; ++i)
+          if (kt * TXK + i <= Xh) dst[(long)uy * XP + kt * TXK + i] = mk(re[r][i], im[r][i]);
+      }
+    }
+  }
+}
+
+// y: Stockham FFT of the ny template rows placed with the template centre at
+// the origin. grid (Xh/TW, nz, ka). Output Ty[a][uz][ky][kx] for kx < Xh,
+// Tyn[a][uz][ky] for kx = Xh (tile 0 only).
 template <int Y, int TW>
-__global__ __launch_bounds__(NT) void k_tmpl_fwd_xy(const float* t2, c32* Ty, c32* Tyn, const c32* twXg,
-                                                    const c32* twYg, int X, int nx, int ny, int nz) {
+__global__ __launch_bounds__(NT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* Tyn, const c32* twYg, int X, int ny,
+                                                   int nz) {
   constexpr int P = TW + 1;
   __shared__ c32 buf[Y * P];
   __shared__ c32 twY[Y];
-  __shared__ c32 twX[1024];
-  __shared__ float plane[64 * 64];
-  __shared__ c32 xd[64 * P];  // x-DFT of the ny template rows, [uy][c]
-  const int Xh = X / 2;
+  const int Xh = X / 2, XP = Xh + 1;
   const int tile = blockIdx.x, uz = blockIdx.y, a = blockIdx.z;
   const bool nyq = (tile == 0);
-  const int NLa = nyq ? P : TW;
   load_table<float, Y, NT>(twY, twYg);
-  for (int i = threadIdx.x; i < X; i += NT) twX[i] = twXg[i];
-  const float* src = t2 + ((long)a * nz + uz) * ny * nx;
-  for (int i = threadIdx.x; i < nx * ny; i += NT) plane[i] = src[i];
   __syncthreads();
-  const int hx = nx / 2, hy = ny / 2;
+  const c32* src = Tx + ((long)a * nz + uz) * ny * XP + tile * TW;
+  const c32* srn = Tx + ((long)a * nz + uz) * ny * XP + Xh;
+  const int hy = ny / 2;
   const int sY = 2 * (Y / 2) - Y;
-  for (int i = threadIdx.x; i < ny * NLa; i += NT) {
-    const int uy = i / NLa, c = i % NLa;
-    const int kx = (c < TW) ? tile * TW + c : Xh;
-    // phase index (kx * (ux - hx)) mod X, advanced by kx per sample
-    int m = (int)(((long)kx * hx) % X);
-    m = (m == 0) ? 0 : X - m;
-    float re = 0.f, im = 0.f;
-    const float* row = plane + uy * nx;
-    for (int ux = 0; ux < nx; ++ux) {
-      const c32 w = twX[m];
-      const float v = row[ux];
-      re = fmaf(v, w.x, re);
-      im = fmaf(-v, w.y, im);
-      m += kx;
-      if (m >= X) m -= X;
-    }
-    xd[uy * P + c] = mk(re, im);
-  }
-  __syncthreads();
-  // y FFT: row ry of the wrapped placement holds template row
+  // row ry of the wrapped placement holds template row
   // uy = (ry + hy - sY) mod Y when that is < ny, zero otherwise; stage 0 reads
-  // those rows from xd, the last stage writes Ty (and Tyn) directly
+  // those rows of the x-DFT straight from Tx, the last stage writes Ty (and
+  // Tyn) directly
   auto ld = [&](int c, int ry) -> c32 {
     int uy = ry + hy - sY;
     if (uy >= Y) uy -= Y;
-    return (uy < ny) ? xd[uy * P + c] : mk(0.f, 0.f);
+    if (uy >= ny) return mk(0.f, 0.f);
+    return (c < TW) ? src[(long)uy * XP + c] : srn[(long)uy * XP];
   };
   c32* dst = Ty + ((long)a * nz + uz) * Y * Xh + tile * TW;
   c32* dn = Tyn + ((long)a * nz + uz) * Y;
@@ -399,6 +445,146 @@ __global__ __launch_bounds__(NT_A) void k_passA(const c32* Ty, const c32* Tyn, c
     stageN_l2g<float, Z, TW, NT_A, 1, Lay>(buf, twZ, [&](int, int, int c, int z, c32 v) {
       q[(long)z * Y * Xh + c] = v;
     });
+    __syncthreads();
+  }
+}
+
+// Pass A for Z = 256 = 16 x 16 as a four-step transform: every thread keeps
+// 16 elements of one line in registers and runs 16-point DFTs there; the two
+// transposes of each direction's middle step are the only LDS round trips
+// (four barriers per orientation instead of ten), and the F_I factors of a
+// thread's outputs stay in its registers for the whole batch.
+//   forward, z = n1 + 16 n2, kz = k2 + 16 k1:
+//     X[kz] = sum_n1 W16^(n1 k1) W256^(n1 k2) sum_n2 W16^(n2 k2) x[z]
+//   inverse, z = m1 + 16 m2:
+//     q[z] = sum_k2 W16^-(k2 m2) W256^-(k2 m1) sum_k1 W16^-(k1 m1) P[kz]
+// grid (Xh/TW - 1, Y) for the kx tiles after tile 0 (which carries the
+// Nyquist column and runs k_passA), TW = 16 lines, 256 threads.
+template <int DIR>
+__device__ __forceinline__ c32 w16(int m) {
+  // exp(DIR 2 pi i m / 16) for m in [0, 16)
+  constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+  const float cs[5] = {1.f, c1, h, s1, 0.f};  // cos(2 pi m / 16), m = 0..4
+  const int q = m & 3, o = m >> 2;            // m = q + 4 o: rotate by o quarter turns
+  c32 w = mk(cs[q], cs[4 - q]);
+  for (int k = 0; k < o; ++k) w = mk(-w.y, w.x);
+  if (DIR < 0) w.y = -w.y;
+  return w;
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft16(c32* v) {
+  // n = a + 4 b, k = kb + 4 ka: 4-point DFTs over b (in place: (a, kb) at
+  // a + 4 kb), twiddles, 4-point DFTs over a (in place: (kb, ka) at a + 4 kb
+  // with a -> ka), then the register renaming to natural order
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    c32 q[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+    Bfly<float, 4, DIR>::run(q);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) v[a + 4 * kb] = (a == 0 || kb == 0) ? q[kb] : cmul(q[kb], w16<DIR>(a * kb));
+  }
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    c32 q[4] = {v[4 * kb], v[4 * kb + 1], v[4 * kb + 2], v[4 * kb + 3]};
+    Bfly<float, 4, DIR>::run(q);
+#pragma unroll
+    for (int ka = 0; ka < 4; ++ka) v[4 * kb + ka] = q[ka];
+  }
+  // X[kb + 4 ka] sits at v[4 kb + ka]: transpose the 4 x 4 register block
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int ka = kb + 1; ka < 4; ++ka) {
+      const c32 t = v[4 * kb + ka];
+      v[4 * kb + ka] = v[4 * ka + kb];
+      v[4 * ka + kb] = t;
+    }
+}
+
+// NYQ: the launch for kx tile 0 (grid (1, Y, ka), one orientation per
+// block), whose blocks carry a 17th
+// line, the kx = X/2 column, packed into column 0 as P(0) + i P(X/2) before
+// the inverse; else grid (Xh/TW - 1, Y) for tiles 1.., 16 lines.
+template <bool NYQ>
+__global__ __launch_bounds__(NYQ ? 17 * 16 : 256) void k_passA4(const c32* Ty, const c32* Tyn, const c32* FI,
+                                                                const c32* FIn, c32* Q, const c32* twg, int Y,
+                                                                int Xh, int nz, int ka) {
+  constexpr int Z = 256, TW = 16;
+  constexpr int NL = NYQ ? TW + 1 : TW, NTH = NL * 16;
+  // per line 16 rows of 17 at a line pitch of 292 (= 4 mod 32): row-wise
+  // writes and column-wise reads of a wave spread over all banks
+  constexpr int RS = 17, LP = 292;
+  __shared__ c32 T[NL * LP];
+  __shared__ c32 tw[Z];
+  __shared__ c32 pn[NYQ ? Z : 1];  // the product of the kx = X/2 column, [k2][k1]
+  const int tile = NYQ ? 0 : blockIdx.x + 1, ky = blockIdx.y;
+  const int line = threadIdx.x % NL, i = threadIdx.x / NL;
+  const bool nyq_line = NYQ && line == TW;
+  for (int k = threadIdx.x; k < Z; k += NTH) tw[k] = twg[k];
+  c32 fi[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) {
+    const int kz = i + 16 * k1;
+    fi[k1] = nyq_line ? FIn[(long)ky * Z + kz] : FI[((long)kz * Y + ky) * Xh + tile * TW + line];
+  }
+  __syncthreads();
+  const int hz = nz / 2;
+  c32* Tl = T + line * LP;
+  // this thread's template column: Ty[a][uz][ky][kx], or Tyn[a][uz][ky]
+  const long su = nyq_line ? (long)Y : (long)Y * Xh;
+  const c32* src = nyq_line ? Tyn + ky : Ty + (long)ky * Xh + tile * TW + line;
+  // tile 0 (few blocks) spreads its orientations over blocks: grid z
+  const int a0 = NYQ ? blockIdx.z : 0, a1 = NYQ ? blockIdx.z + 1 : ka;
+  src += a0 * nz * su;
+  for (int a = a0; a < a1; ++a, src += nz * su) {
+    c32 v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      int uz = i + 16 * n2 + hz;  // wrapped placement: plane uz of the template sits at z
+      if (uz >= Z) uz -= Z;
+      v[n2] = (uz < nz) ? src[uz * su] : mk(0.f, 0.f);
+    }
+    dft16<-1>(v);
+#pragma unroll
+    for (int k2 = 1; k2 < 16; ++k2) v[k2] = cmul(v[k2], conjg(tw[i * k2]));
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) Tl[k2 * RS + i] = v[k2];
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = Tl[i * RS + n1];  // this thread now holds k2 = i
+    dft16<-1>(v);                                             // X[i + 16 k1]
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = cmulc(fi[k1], v[k1]);
+    if constexpr (NYQ) {  // Nyquist packing
+      if (nyq_line) {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) pn[i * 16 + k1] = v[k1];
+      }
+      __syncthreads();
+      if (line == 0) {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+          const c32 p = pn[i * 16 + k1];
+          v[k1] = mk(v[k1].x - p.y, v[k1].y + p.x);
+        }
+      }
+    }
+    dft16<1>(v);
+#pragma unroll
+    for (int m1 = 1; m1 < 16; ++m1) v[m1] = cmul(v[m1], tw[i * m1]);
+    __syncthreads();
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) Tl[m1 * RS + i] = v[m1];
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) v[k2] = Tl[i * RS + k2];  // this thread now holds m1 = i
+    dft16<1>(v);                                             // q[i + 16 m2]
+    if (!nyq_line) {
+      c32* q = Q + (long)a * Z * Y * Xh + (long)ky * Xh + tile * TW + line;
+#pragma unroll
+      for (int m2 = 0; m2 < 16; ++m2) q[(long)(i + 16 * m2) * Y * Xh] = v[m2];
+    }
     __syncthreads();
   }
 }
@@ -875,6 +1061,10 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     TOMHIP_CHECK(hipMalloc(&Ty_, sizeof(c32) * (long)KA_ * nz * Y * Xh));
     TOMHIP_CHECK(hipMalloc(&Tyn_, sizeof(c32) * (long)KA_ * nz * Y));
     TOMHIP_CHECK(hipMalloc(&Q_, sizeof(c32) * (long)KA_ * Z * Y * Xh));
+    // the template's x-DFT rows live in Q between the template forward and
+    // pass A, unless the template is as large as the volume in y and z
+    const long txn = (long)KA_ * nz * ny * (Xh + 1);
+    if (txn > (long)KA_ * Z * Y * Xh) TOMHIP_CHECK(hipMalloc(&Tx_, sizeof(c32) * txn));
     if (mode == MODE_FLCF) {
       TOMHIP_CHECK(hipMalloc(&mu_, sizeof(double) * N));
       TOMHIP_CHECK(hipMalloc(&sig_, sizeof(double) * N));
@@ -915,6 +1105,7 @@ void Search::release() {
   (void)hipFree(tbox_);
   (void)hipFree(Ty_);
   (void)hipFree(Tyn_);
+  (void)hipFree(Tx_);
   (void)hipFree(Q_);
   (void)hipFree(mu_);
   (void)hipFree(sig_);
@@ -927,7 +1118,7 @@ void Search::release() {
   maxc_ = nullptr;
   argm_ = nullptr;
   tmpl_ = t1_ = wsh_ = rm_ = ref_tt_ = nullptr;
-  tbox_ = Ty_ = Tyn_ = Q_ = ref_work_ = nullptr;
+  tbox_ = Ty_ = Tyn_ = Tx_ = Q_ = ref_work_ = nullptr;
   mu_ = sig_ = nullptr;
   mask_ = nullptr;
   drop_graph();
@@ -1271,7 +1462,13 @@ void Search::launch_passA(int ka) {
     constexpr int TW = decltype(TWc)::value;
     dispatch_len(Z, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
-      if constexpr ((long)L * (TW + 1) * 8 + (long)L * 8 <= kLdsMax) {
+      if constexpr (L == 256 && TW == 16) {
+        hipLaunchKernelGGL(k_passA4<true>, dim3(1, Y, ka), dim3(17 * 16), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
+                           (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+        if (Xh > TW)
+          hipLaunchKernelGGL(k_passA4<false>, dim3(Xh / TW - 1, Y), dim3(256), 0, s_, (const c32*)Ty_,
+                             (const c32*)Tyn_, (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
+      } else if constexpr ((long)L * (TW + 1) * 8 + (long)L * 8 <= kLdsMax) {
         hipLaunchKernelGGL((k_passA<L, TW, true>), dim3(1, Y), dim3(NT_A), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
                            (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
         if (Xh > TW)
@@ -1305,15 +1502,27 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index, const long
   if (wsh_) timed(K_BOX, [&] { box_filter(ka, t1_, tbox_); });
 
   timed(K_TFWD, [&] {
+    // the x-DFT rows go to the front of Q, which is free until pass A
+    c32* Tx = Tx_ ? Tx_ : Q_;
+    hipLaunchKernelGGL(k_tmpl_xdft, dim3(nz, ka), dim3(NT), 0, s_, (const float*)t1_, Tx, tw_.f32(X), X, nx, ny,
+                       nz);
     dispatch_len(Y, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
-      // the narrower tile keeps the block's LDS under 64 KB (two blocks per CU)
-      if constexpr (L >= 512) {
-        hipLaunchKernelGGL((k_tmpl_fwd_xy<L, 8>), dim3(Xh / 8, nz, ka), dim3(NT), 0, s_, (const float*)t1_, Ty_,
-                           Tyn_, tw_.f32(X), tw_.f32(L), X, nx, ny, nz);
+      // the widest tile (longest contiguous rows of Ty) whose block LDS
+      // still lets two blocks share a CU
+      if constexpr (L >= 1024) {
+        hipLaunchKernelGGL((k_tmpl_fwd_y<L, 8>), dim3(Xh / 8, nz, ka), dim3(NT), 0, s_, (const c32*)Tx, Ty_, Tyn_,
+                           tw_.f32(L), X, ny, nz);
+      } else if constexpr (L >= 512) {
+        hipLaunchKernelGGL((k_tmpl_fwd_y<L, 16>), dim3(Xh / 16, nz, ka), dim3(NT), 0, s_, (const c32*)Tx, Ty_,
+                           Tyn_, tw_.f32(L), X, ny, nz);
       } else {
-        hipLaunchKernelGGL((k_tmpl_fwd_xy<L, 16>), dim3(Xh / 16, nz, ka), dim3(NT), 0, s_, (const float*)t1_,
-                           Ty_, Tyn_, tw_.f32(X), tw_.f32(L), X, nx, ny, nz);
+        if (Xh % 32 == 0)
+          hipLaunchKernelGGL((k_tmpl_fwd_y<L, 32>), dim3(Xh / 32, nz, ka), dim3(NT), 0, s_, (const c32*)Tx, Ty_,
+                             Tyn_, tw_.f32(L), X, ny, nz);
+        else
+          hipLaunchKernelGGL((k_tmpl_fwd_y<L, 16>), dim3(Xh / 16, nz, ka), dim3(NT), 0, s_, (const c32*)Tx, Ty_,
+                             Tyn_, tw_.f32(L), X, ny, nz);
       }
     });
     TOMHIP_CHECK(hipGetLastError());
