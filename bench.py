@@ -38,8 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch group measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (fetch scaled x2.000, write x1.000 by a 1 GiB device copy in the
 # same runs); profiles/flcf_c2_hbm_traffic.md has the commands and counts.
-MEASURED_TRAFFIC = {("c2", 4): {"passA": 1.5514e9, "passB": 2.1476e9, "passC": 1.6414e9},
-                    ("c2", 16): {"passA": 5.3959e9, "passB": 8.5902e9, "passC": 5.0465e9}}
+MEASURED_TRAFFIC = {("c2", 16): {"passA": 5.6560e9, "passB": 8.5902e9, "passC": 4.9832e9}}
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)

@@ -114,6 +114,10 @@ def test_picklist_cases_match_oracle(oracle):
     nan = rng.random(N).astype(np.float32)
     nan[18, 17, 10] = np.nan
     assert api.tom_os3_returnPicklist(nan, (8, 8, 8), 10).shape == (0, 5)
+    # tom_peakc's 1D branch (size(in,1) == 1) is refused, not guessed
+    from tom_amd._lib import TomHipError
+    with pytest.raises(TomHipError):
+        api.tom_os3_returnPicklist(rng.random((1, 30, 1)).astype(np.float32), (8, 8, 1), 5, erase_borders=False)
 
 
 @pytest.mark.gpu

@@ -900,6 +900,8 @@ int tomhip_picklist_f32(tomhip_ctx* ctx, const float* map, const int64_t N[3], c
     check_dims3(N, "map");
     check_dims3(tsize, "template");
     TOMHIP_REQUIRE(n >= 0 && n <= 10000000, ERR_WRONG_INPUT, "bad n");
+    // tom_peakc's 1D branch (size(in,1) == 1) is not this path
+    TOMHIP_REQUIRE(!(N[0] == 1 && N[1] * N[2] > 1), ERR_UNSUPPORTED, "row-vector maps are not supported");
     const long NN = N[0] * N[1] * N[2];
     hipStream_t s = ctx->stream;
     DevBuf m(s, map, sizeof(float) * NN, true, false);

@@ -140,7 +140,7 @@ class Search {
   long ref_cap_ = 0;         // templates of n3 voxels each buffer holds
   c32* Ty_ = nullptr;        // [KA][nz][Y][Xh]
   c32* Tyn_ = nullptr;       // [KA][nz][Y]
-  c32* Tx_ = nullptr;        // [KA][nz][ny][Xh+1] when it does not fit in Q
+  c32* Tx_ = nullptr;        // [KA][nz][ny][pitch] template x-DFT rows, when they do not fit in Q
   c32* Q_ = nullptr;         // [KA][Z][Y][Xh]
   bool started_ = false;
   bool remask_ = true;

@@ -230,15 +230,17 @@ __global__ __launch_bounds__(NT) void k_box_xy_inv(const c32* in, float* t, cons
 // x: direct DFT of the n non-zero samples of every template row for each
 // kx in [0, X/2], as register tiles of 4 rows x 4 frequencies (a row's
 // samples and a frequency's twiddles are each read once per tile and used 4
-// times). grid (nz, ka). Output Tx[a][uz][uy][kx], row length Xh + 1.
+// times). grid (nz, ka). Output Tx[a][uz][uy][kx], row pitch tx_pitch(Xh).
 // Summation order per output is ux ascending with fmaf, the twiddle index
 // kx (ux - hx) mod X advanced by kx per sample.
 constexpr int TXU = 4, TXK = 4;
+// row pitch of Tx: Xh + 1 frequencies rounded up to 16 (128-byte rows)
+__host__ __device__ constexpr int tx_pitch(int Xh) { return (Xh + 1 + 15) / 16 * 16; }
 __global__ __launch_bounds__(NT) void k_tmpl_xdft(const float* t2, c32* Tx, const c32* twXg, int X, int nx, int ny,
                                                   int nz) {
   __shared__ float plane[64 * 65];  // rows at pitch nx + 1
   __shared__ c32 twX[1024];
-  const int Xh = X / 2, XP = Xh + 1, PX = nx + 1;
+  const int Xh = X / 2, XP = tx_pitch(Xh), PX = nx + 1;
   const int uz = blockIdx.x, a = blockIdx.y;
   for (int i = threadIdx.x; i < X; i += NT) twX[i] = twXg[i];
   const float* src = t2 + ((long)a * nz + uz) * ny * nx;
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* 
   constexpr int P = TW + 1;
   __shared__ c32 buf[Y * P];
   __shared__ c32 twY[Y];
-  const int Xh = X / 2, XP = Xh + 1;
+  const int Xh = X / 2, XP = tx_pitch(Xh);
   const int tile = blockIdx.x, uz = blockIdx.y, a = blockIdx.z;
   const bool nyq = (tile == 0);
   load_table<float, Y, NT>(twY, twYg);
@@ -1063,7 +1065,7 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     TOMHIP_CHECK(hipMalloc(&Q_, sizeof(c32) * (long)KA_ * Z * Y * Xh));
     // the template's x-DFT rows live in Q between the template forward and
     // pass A, unless the template is as large as the volume in y and z
-    const long txn = (long)KA_ * nz * ny * (Xh + 1);
+    const long txn = (long)KA_ * nz * ny * tx_pitch(Xh);
     if (txn > (long)KA_ * Z * Y * Xh) TOMHIP_CHECK(hipMalloc(&Tx_, sizeof(c32) * txn));
     if (mode == MODE_FLCF) {
       TOMHIP_CHECK(hipMalloc(&mu_, sizeof(double) * N));
