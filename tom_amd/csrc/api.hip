@@ -1105,11 +1105,7 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
       b.T = nullptr;
       b.FM = b.FB = nullptr;
     }
-    // processing order: largest boxes first; wedge tangent; rotation matrices
-    std::vector<int32_t> order(P);
-    for (int64_t i = 0; i < P; ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return edges[pb[a]] > edges[pb[b]]; });
+    // wedge tangents; rotation matrices
     // SimpleWedge(angle * PI / 180., cutoff) of a scalar wedge argument
     // (tom_mex_corr3d.cpp:334,410); a negative or NaN angle means no wedge
     // ([] in the gateway), 0 keeps only the cutoff sphere as the reference does
@@ -1146,15 +1142,12 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
       return d;
     };
     try {
-      const int32_t* d_pb = (const int32_t*)up(pb.data(), sizeof(int32_t) * P);
-      const int32_t* d_order = (const int32_t*)up(order.data(), sizeof(int32_t) * P);
       const long* d_poff = (const long*)up(poffl.data(), sizeof(long) * P);
       const float* d_tan = (const float*)up(tanw.data(), sizeof(float) * P);
       const float* d_rm = (const float*)up(rms.data(), sizeof(float) * rms.size());
-      const float* d_tanr = (const float*)up(tanr.data(), sizeof(float) * nbox);
       const float* d_wmap = any_ref ? (const float*)up(wmap.data(), sizeof(float) * wmap.size()) : nullptr;
-      subtomo_run(*ctx->tw, s, boxes, dT.get<float>(), toff.data(), P, d_pb, d_order, dP.get<float>(), d_poff, d_tan,
-                  d_rm, d_tanr, d_wmap, A, oA.get<int32_t>(), oP.get<int32_t>(), oC.get<float>());
+      subtomo_run(*ctx->tw, s, boxes, dT.get<float>(), toff.data(), P, pb, dP.get<float>(), d_poff, d_tan, d_rm, tanr,
+                  d_wmap, A, oA.get<int32_t>(), oP.get<int32_t>(), oC.get<float>());
       oA.commit();
       oP.commit();
       oC.commit();

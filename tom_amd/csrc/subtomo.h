@@ -20,18 +20,17 @@ struct SubtomoBox {
   const c32* FB;     // device: half spectrum of (M != 0)
 };
 
-int subtomo_workgroups();
 long subtomo_scratch_floats();
 // boxes_h[k].M / Mcc / B / n / SM / SM2 filled by the caller; T, FM, FB are
-// produced here. rms_dev: [P][A][9] rotation matrices. tan_ref_dev: [nbox]
-// tangent of each box template's wedge semi-angle (< 0: no template wedge);
-// wref_P_dev: [P][A][12] float maps of that wedge turned by each orientation
-// (read only for boxes with a template wedge; may be null when none has one).
+// produced here. pbox_h: [P] box of each particle (host). rms_dev: [P][A][9]
+// rotation matrices. tanr_h: [nbox] tangent of each box template's wedge
+// semi-angle (< 0: no template wedge); wref_P_dev: [P][A][12] float maps of
+// that wedge turned by each orientation (read only for boxes with a template
+// wedge; may be null when none has one).
 void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h, const float* templates_dev,
-                 const long* tmpl_off, int64_t P, const int32_t* pbox_dev, const int32_t* order_dev,
-                 const float* particles_dev, const long* poff_dev, const float* tan_dev, const float* rms_dev,
-                 const float* tan_ref_dev, const float* wref_P_dev, int64_t A, int32_t* best_idx, int32_t* best_pos,
-                 float* best_val);
+                 const long* tmpl_off, int64_t P, const std::vector<int32_t>& pbox_h, const float* particles_dev,
+                 const long* poff_dev, const float* tan_dev, const float* rms_dev, const std::vector<float>& tanr_h,
+                 const float* wref_P_dev, int64_t A, int32_t* best_idx, int32_t* best_pos, float* best_val);
 
 // The float affine map of tom::SimpleWedge turned by [phi psi theta] degrees
 // for a box of edge B (src/tom/core/wedge.cpp:171-184 with

@@ -21,14 +21,16 @@
 // Without a particle wedge Tw = Tf and var_t is norm_mask's variance of t';
 // without a template wedge Pw = P_p and var_p is the particle's norm_mask variance.
 //
-// Design. One workgroup owns one particle at a time (popped from an atomic
-// queue, largest boxes first) and runs all its orientations; every 3D FFT goes
-// through the workgroup's own global scratch (L2/MALL resident): a z-plane
-// pass (x then y Stockham FFTs of one B x B plane in LDS) and a z-line pass
-// (B-point FFTs of 64 columns in LDS) with the spectral product fused between
-// the forward and the inverse z transforms. The mask's mean subtraction of
-// norm_mask is linear, so it is applied in Fourier space with precomputed
-// FFT(M) and FFT(M != 0) instead of an extra real-space pass.
+// Design. Items are (particle, orientation) pairs, grouped by box and cut
+// into chunks of whole particles. Each step is one launch over the chunk with
+// one z plane (x/y FFTs in LDS) or one batch of 64 half-spectrum columns (z
+// FFT in LDS) of one item per block, so every launch has thousands of blocks
+// and the items' half spectra stream through HBM; the spectral products,
+// wedges and mean removals are fused between the forward and inverse z
+// transforms, and the re-mask's c2r and r2c of a plane share one kernel. The
+// mask's mean subtraction of norm_mask is linear, so it is applied in Fourier
+// space with precomputed FFT(M) and FFT(M != 0) instead of an extra
+// real-space pass.
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -54,26 +56,6 @@ struct Geo {
   static constexpr long NH = PL * B;
 };
 
-// Workgroup-wide FP64 sums of K values (all threads get the result).
-template <int K>
-__device__ void wg_sum(double* v, double* red) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double x = v[k];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) red[k * (NTS / 64) + w] = x;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0;
-    for (int i = 0; i < NTS / 64; ++i) s += red[k * (NTS / 64) + i];
-    v[k] = s;
-  }
-  __syncthreads();
-}
-
 // real volume R (B^3, x fastest) -> half spectrum H [kz? no: z][ky][kx<=B/2]:
 // forward FFT along x and y of every z plane.
 template <int B>
@@ -90,31 +72,6 @@ __device__ void plane_r2c(const float* R, c32* H, c32* buf, const c32* tw) {
       const int ky = i / G::XH1, kx = i % G::XH1;
       dst[i] = buf[ky * B + kx];
     }
-    __syncthreads();
-  }
-}
-
-// half spectrum H (after the z inverse) -> real values, plane by plane:
-// inverse FFT along y, Hermitian completion in x, inverse FFT along x;
-// epi(x, y, z, value) receives every real output (unnormalised).
-template <int B, class EPI>
-__device__ void plane_c2r(const c32* H, c32* buf, const c32* tw, EPI& epi) {
-  using G = Geo<B>;
-  for (int z = 0; z < B; ++z) {
-    const c32* src = H + (long)z * G::PL;
-    for (int i = threadIdx.x; i < G::PL; i += NTS) {
-      const int ky = i / G::XH1, kx = i % G::XH1;
-      buf[ky * B + kx] = src[i];
-    }
-    __syncthreads();
-    lds_fft<float, B, G::XH1, NTS, 1, Inter<B>>(buf, tw);
-    for (int i = threadIdx.x; i < B * (B - G::XH1); i += NTS) {
-      const int y = i / (B - G::XH1), kx = G::XH1 + i % (B - G::XH1);
-      buf[y * B + kx] = conjg(buf[y * B + (B - kx)]);
-    }
-    __syncthreads();
-    lds_fft<float, B, B, NTS, 1, Contig<B>>(buf, tw);
-    for (int i = threadIdx.x; i < B * B; i += NTS) epi(i % B, i / B, z, buf[i].x);
     __syncthreads();
   }
 }
@@ -147,34 +104,6 @@ __device__ void zpass(c32* H, c32* OUT, c32* buf, const c32* tw, const MID& mid)
     for (int i = threadIdx.x; i < B * NC; i += NTS) {
       const int z = i / NC, c = i % NC;
       if (c < nc) dst[(long)z * G::PL + c0 + c] = buf[i];
-    }
-    __syncthreads();
-  }
-}
-
-// z-line pass over a spectrum that is already z-transformed: mid(kx, ky, kz,
-// v, idx) on SRC, then the inverse z FFT into DST (which may alias SRC).
-template <int B, class MID>
-__device__ void zpass_inv(const c32* SRC, c32* DST, c32* buf, const c32* tw, const MID& mid) {
-  using G = Geo<B>;
-  constexpr int NCOL = (int)G::PL;
-  for (int c0 = 0; c0 < NCOL; c0 += NC) {
-    const int nc = (NCOL - c0 < NC) ? NCOL - c0 : NC;
-    for (int i = threadIdx.x; i < B * NC; i += NTS) {
-      const int kz = i / NC, c = i % NC;
-      if (c < nc) {
-        const int col = c0 + c;
-        const long idx = (long)kz * G::PL + col;
-        buf[i] = mid(col % G::XH1, col / G::XH1, kz, SRC[idx], idx);
-      } else {
-        buf[i] = mk(0.f, 0.f);
-      }
-    }
-    __syncthreads();
-    lds_fft<float, B, NC, NTS, 1, Inter<NC>>(buf, tw);
-    for (int i = threadIdx.x; i < B * NC; i += NTS) {
-      const int z = i / NC, c = i % NC;
-      if (c < nc) DST[(long)z * G::PL + c0 + c] = buf[i];
     }
     __syncthreads();
   }
@@ -222,224 +151,400 @@ __device__ __forceinline__ bool wedge_keep(int kx, int ky, int kz, float tan_a) 
   return (r <= cutoff) && ((fz < 1e-4f) || (tan_a <= fabsf(fx) / fz));
 }
 
-struct Job {
-  const SubtomoBox* boxes;
-  const int32_t* order;     // [P] processing order
-  const int32_t* pbox;      // [P] index into boxes
+// ---------------------------------------------------------------- batched pipeline
+// Items are (particle, orientation) pairs of one box; a chunk holds whole
+// particles. Every kernel runs one z plane (plane kernels) or one batch of
+// NC half-spectrum columns (z kernels) of one item per block, so a launch has
+// thousands of blocks and the half spectra stream through HBM at full width:
+//   particles: K_load(V.M, sums) + r2c -> z fwd (mean removal) -> Pp
+//   template wedge (per item): z inv of Pp . W_T(a) -> c2r + re-mask + sums
+//       + r2c (one kernel, plane in LDS) -> z fwd (mean removal) -> Pw
+//   per item: K_load(rot(T).M, sums) + r2c -> [particle wedge: z fwd . W_p +
+//       z inv -> c2r + re-mask + sums + r2c] -> z fwd, mean removal,
+//       conj(Tw) . P, z inv -> c2r + masked peak per plane -> best per particle
+// Per-plane partial sums (FP64) and peaks are reduced in plane order, so the
+// results do not depend on scheduling.
+constexpr int NTB = 256;
+
+struct Chunk {
+  SubtomoBox bx;
+  int np, A;
+  const int32_t* pid;       // [np] particle ids
   const float* particles;   // packed
-  const long* poff;         // [P] element offsets
-  const float* tan_wedge;   // [P] tan(semi-angle); < 0 -> no wedge
+  const long* poff;         // [P]
+  const float* tanw;        // [P] tan of the particle wedge (< 0: none)
+  float tan_r;              // tan of the box template's wedge (< 0: none)
   const float* rms;         // [P][A][9]
-  const float* tan_ref;     // [nbox] tan(template wedge semi-angle); < 0 -> none
-  const float* wref_P;      // [P][A][12] template wedge maps (used when tan_ref >= 0)
-  int A;
-  int P;
-  int* counter;
-  float* scratch;
-  long sstride;             // floats per workgroup
+  const float* wmap;        // [P][A][12]
+  c32* H;                   // [max(np A, np)][NH]
+  c32* Pp;                  // [np][NH]
+  c32* Pw;                  // [np A][NH] (template wedge only)
+  double* S4p;              // [np][B][4]
+  double* S4t;              // [np A][B][4]
+  double* S2t;              // [np A][B][2]
+  double* S2p;              // [np A][B][2]
+  float* pkv;               // [np A][B]
+  int* pki;                 // [np A][B]
+  const c32* tw;
   int32_t* best_idx;
-  int32_t* best_pos;        // [P][3]
+  int32_t* best_pos;
   float* best_val;
-  const c32* tw[3];         // tables for 32, 48, 64
 };
 
-template <int B>
-__device__ __noinline__ void process(const Job& J, int p, c32* buf, c32* tw, double* red) {
-  using G = Geo<B>;
-  const SubtomoBox& bx = J.boxes[J.pbox[p]];
-  float* R = J.scratch + (long)blockIdx.x * J.sstride;
-  c32* H = reinterpret_cast<c32*>(R + G::N3);
-  c32* Pp = H + G::NH;
-  c32* Pw = Pp + G::NH;
-  const c32* twg = J.tw[B == 32 ? 0 : (B == 48 ? 1 : 2)];
-  for (int i = threadIdx.x; i < B; i += NTS) tw[i] = twg[i];
-  __syncthreads();
-  const float* M = bx.M;
-  const c32* FM = bx.FM;
-  const c32* FB = bx.FB;
-  const double n = (double)bx.n, SM = bx.SM;
-  const double numel = (double)G::N3;
-
-  // ---- particle: norm_mask (non-boolean) folded into Fourier space
-  double var_p;
-  {
-    const float* V = J.particles + J.poff[p];
-    double s[4] = {0, 0, 0, 0};  // sum_B v, sum v M, sum v^2 M^2, sum v M^2
-    for (long i = threadIdx.x; i < G::N3; i += NTS) {
-      const float m = M[i];
-      const float v = V[i];
-      R[i] = v * m;
-      if (m != 0.f) {
-        const double dv = v, dm = m;
-        s[0] += dv;
-        s[1] += dv * dm;
-        s[2] += dv * dv * dm * dm;
-        s[3] += dv * dm * dm;
-      }
-    }
-    wg_sum<4>(s, red);
-    const double mu1 = s[0] / n;
-    const double mu2 = (s[1] - mu1 * SM) / n;
-    // sum over B of ((v - mu1) M)^2
-    const double q = s[2] - 2.0 * mu1 * s[3] + mu1 * mu1 * bx.SM2;
-    var_p = (q - n * mu2 * mu2) / numel;
-    plane_r2c<B>(R, H, buf, tw);
-    const float fm1 = (float)mu1, fm2 = (float)mu2;
-    zpass<B, false>(H, Pp, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
-      if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
-      const c32 fm = FM[idx], fb = FB[idx];
-      return mk(v.x - fm1 * fm.x - fm2 * fb.x, v.y - fm1 * fm.y - fm2 * fb.y);
-    });
-  }
-  const bool wedge = J.tan_wedge[p] >= 0.f;
-  const float tan_a = J.tan_wedge[p];
-  const float tan_r = J.tan_ref[J.pbox[p]];
-  const bool wref = tan_r >= 0.f;
-  const double var_p0 = var_p;
-
-  int best = -1;
-  float bestv = 0.f;
-  int bpos = 0;
-  const float cB = (float)((B + 1) / 2);
-  for (int a = 0; a < J.A; ++a) {
-    // ---- the template's wedge turned by a, applied to the particle
-    const c32* Pa = Pp;
-    var_p = var_p0;
-    if (wref) {
-      float Pr[12];
-      const float* pr = J.wref_P + ((long)p * J.A + a) * 12;
+// Block sums of K doubles over NTB threads; the result is valid in thread 0.
+template <int K>
+__device__ void blk_sum(double* v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) Pr[k] = pr[k];
-      zpass_inv<B>(Pp, H, buf, tw, [&](int kx, int ky, int kz, c32 v, long) {
-        if ((kx == 0 && ky == 0 && kz == 0) || !wedge_keep_rot<B>(kx, ky, kz, Pr, tan_r)) return mk(0.f, 0.f);
-        return v;
-      });
-      double t2[2] = {0, 0};
-      auto remask = [&](int x, int y, int z, float v) {
-        const long i = ((long)z * B + y) * B + x;
-        const bool in = M[i] != 0.f;
-        R[i] = in ? v : 0.f;
-        if (in) {
-          t2[0] += (double)v;
-          t2[1] += (double)v * (double)v;
-        }
-      };
-      plane_c2r<B>(H, buf, tw, remask);
-      wg_sum<2>(t2, red);
-      const double mb = t2[0] / n;
-      var_p = (t2[1] - n * mb * mb) / numel;
-      plane_r2c<B>(R, H, buf, tw);
-      const float fmb = (float)mb;
-      zpass<B, false>(H, Pw, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
-        if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
-        const c32 fb = FB[idx];
-        return mk(v.x - fmb * fb.x, v.y - fmb * fb.y);
-      });
-      Pa = Pw;
+  for (int k = 0; k < K; ++k) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[k * (NTB / 64) + w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double t = 0;
+      for (int i = 0; i < NTB / 64; ++i) t += red[k * (NTB / 64) + i];
+      v[k] = t;
     }
-    const float* rm = J.rms + ((long)p * J.A + a) * 9;
-    float rmr[9];
+  }
+  __syncthreads();
+}
+
+// norm_mask statistics (volume_fcn.cpp:168-247) from per-plane sums
+// {sum v, sum v M, sum v^2 M^2, sum v M^2} over M != 0: mean mu1, mean of the
+// masked volume mu2, variance.
+struct Norm4 {
+  double mu1, mu2, var;
+};
+__device__ Norm4 norm4(const double* S, int B, const SubtomoBox& bx) {
+  double s[4] = {0, 0, 0, 0};
+  for (int z = 0; z < B; ++z)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += S[4 * z + k];
+  const double n = (double)bx.n, numel = (double)B * B * B;
+  Norm4 r;
+  r.mu1 = s[0] / n;
+  r.mu2 = (s[1] - r.mu1 * bx.SM) / n;
+  const double q = s[2] - 2.0 * r.mu1 * s[3] + r.mu1 * r.mu1 * bx.SM2;
+  r.var = (q - n * r.mu2 * r.mu2) / numel;
+  return r;
+}
+// boolean re-mask statistics {sum v, sum v^2} over M != 0: mean, variance
+struct Norm2 {
+  double mb, var;
+};
+__device__ Norm2 norm2(const double* S, int B, const SubtomoBox& bx) {
+  double t0 = 0, t1 = 0;
+  for (int z = 0; z < B; ++z) {
+    t0 += S[2 * z];
+    t1 += S[2 * z + 1];
+  }
+  const double n = (double)bx.n, numel = (double)B * B * B;
+  Norm2 r;
+  r.mb = t0 / n;
+  r.var = (t1 - n * r.mb * r.mb) / numel;
+  return r;
+}
+
+// plane z of a real volume, in buf as complex rows (row y = x line) -> its
+// x and y forward FFT -> half-spectrum plane H[ky][kx <= B/2]
+template <int B>
+__device__ void plane_fwd_lds(c32* buf, const c32* tw, c32* Hz) {
+  constexpr int XH1 = B / 2 + 1;
+  lds_fft<float, B, B, NTB, -1, Contig<B>>(buf, tw);
+  lds_fft<float, B, XH1, NTB, -1, Inter<B>>(buf, tw);
+  for (int i = threadIdx.x; i < B * XH1; i += NTB) Hz[i] = buf[(i / XH1) * B + i % XH1];
+  __syncthreads();
+}
+// half-spectrum plane -> real values in buf[y*B + x].x (unnormalised)
+template <int B>
+__device__ void plane_inv_lds(const c32* Hz, c32* buf, const c32* tw) {
+  constexpr int XH1 = B / 2 + 1;
+  for (int i = threadIdx.x; i < B * XH1; i += NTB) buf[(i / XH1) * B + i % XH1] = Hz[i];
+  __syncthreads();
+  lds_fft<float, B, XH1, NTB, 1, Inter<B>>(buf, tw);
+  for (int i = threadIdx.x; i < B * (B - XH1); i += NTB) {
+    const int y = i / (B - XH1), kx = XH1 + i % (B - XH1);
+    buf[y * B + kx] = conjg(buf[y * B + (B - kx)]);
+  }
+  __syncthreads();
+  lds_fft<float, B, B, NTB, 1, Contig<B>>(buf, tw);
+}
+
+// grid (B, count). TMPL = false: plane z of particle i of the chunk, v = V M;
+// TMPL = true: plane z of item it = i A + a, v = rot(T, a) M. Writes the
+// plane's four norm_mask sums and its half spectrum H[item].
+template <int B, bool TMPL>
+__global__ __launch_bounds__(NTB) void k_b_load(Chunk C) {
+  using G = Geo<B>;
+  __shared__ c32 buf[B * B];
+  __shared__ c32 tw[B];
+  __shared__ double red[4 * (NTB / 64)];
+  const int z = blockIdx.x, item = blockIdx.y;
+  for (int i = threadIdx.x; i < B; i += NTB) tw[i] = C.tw[i];
+  const float* M = C.bx.M + (long)z * B * B;
+  double s[4] = {0, 0, 0, 0};
+  const float* V = nullptr;
+  float rmr[9];
+  if (!TMPL) {
+    V = C.particles + C.poff[C.pid[item]] + (long)z * B * B;
+  } else {
+    const int p = C.pid[item / C.A], a = item % C.A;
+    const float* rm = C.rms + ((long)p * C.A + a) * 9;
 #pragma unroll
     for (int k = 0; k < 9; ++k) rmr[k] = rm[k];
-    // ---- rotated template times the mask, with norm_mask's sums
-    double s[4] = {0, 0, 0, 0};
-    for (long i = threadIdx.x; i < G::N3; i += NTS) {
-      const int x = (int)(i % B), y = (int)((i / B) % B), z = (int)(i / (B * B));
-      const float t = rot3d_voxel(bx.T, B, B, B, rmr, cB, cB, cB, x, y, z);
-      const float m = M[i];
-      R[i] = t * m;
-      if (m != 0.f) {
-        const double dv = t, dm = m;
-        s[0] += dv;
-        s[1] += dv * dm;
-        s[2] += dv * dv * dm * dm;
-        s[3] += dv * dm * dm;
+  }
+  const float cB = (float)((B + 1) / 2);
+  for (int i = threadIdx.x; i < B * B; i += NTB) {
+    const float v = TMPL ? rot3d_voxel(C.bx.T, B, B, B, rmr, cB, cB, cB, i % B, i / B, z) : V[i];
+    const float m = M[i];
+    buf[i] = mk(v * m, 0.f);
+    if (m != 0.f) {
+      const double dv = v, dm = m;
+      s[0] += dv;
+      s[1] += dv * dm;
+      s[2] += dv * dv * dm * dm;
+      s[3] += dv * dm * dm;
+    }
+  }
+  blk_sum<4>(s, red);
+  double* S = (TMPL ? C.S4t : C.S4p) + ((long)item * B + z) * 4;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S[k] = s[k];
+  }
+  plane_fwd_lds<B>(buf, tw, C.H + (long)item * G::NH + (long)z * G::PL);
+}
+
+enum ZMode { Z_PART, Z_WEDGE, Z_FINAL, Z_WREF };
+
+// grid (ceil(PL/NC), count): NC half-spectrum columns of one item, z forward
+// FFT, the mode's product, then (Z_WEDGE, Z_FINAL) the z inverse back to H or
+// (Z_PART, Z_WREF) the forward result to Pp / Pw.
+template <int B, int MODE>
+__global__ __launch_bounds__(NTB) void k_b_z(Chunk C) {
+  using G = Geo<B>;
+  constexpr int NCOL = (int)G::PL, XH1 = B / 2 + 1;
+  __shared__ c32 buf[B * NC];
+  __shared__ c32 tw[B];
+  __shared__ float sc[3];  // fm1, fm2 (or fmb, 0), tan of the particle wedge
+  const int c0 = blockIdx.x * NC, item = blockIdx.y;
+  const int nc = (NCOL - c0 < NC) ? NCOL - c0 : NC;
+  const int ci = (MODE == Z_PART) ? item : item / C.A;  // chunk particle index
+  const bool wedge = C.tanw[C.pid[ci]] >= 0.f;
+  if (MODE == Z_WEDGE && !wedge) return;  // items of particles without a wedge skip this step
+  for (int i = threadIdx.x; i < B; i += NTB) tw[i] = C.tw[i];
+  if (threadIdx.x == 0) {
+    if (MODE == Z_PART || (MODE != Z_WREF && !(MODE == Z_FINAL && wedge))) {
+      const Norm4 r = norm4((MODE == Z_PART ? C.S4p : C.S4t) + (long)item * B * 4, B, C.bx);
+      sc[0] = (float)r.mu1;
+      sc[1] = (float)r.mu2;
+    } else {
+      const Norm2 r = norm2((MODE == Z_WREF ? C.S2p : C.S2t) + (long)item * B * 2, B, C.bx);
+      sc[0] = (float)r.mb;
+      sc[1] = 0.f;
+    }
+    sc[2] = C.tanw[C.pid[ci]];
+  }
+  c32* Hi = C.H + (long)item * G::NH;
+  for (int i = threadIdx.x; i < B * NC; i += NTB) {
+    const int zz = i / NC, c = i % NC;
+    buf[i] = (c < nc) ? Hi[(long)zz * G::PL + c0 + c] : mk(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<float, B, NC, NTB, -1, Inter<NC>>(buf, tw);
+  const float f1 = sc[0], f2 = sc[1], tan_a = sc[2];
+  const c32* Pa = (MODE == Z_FINAL) ? ((C.tan_r >= 0.f) ? C.Pw + (long)item * G::NH : C.Pp + (long)ci * G::NH)
+                                    : nullptr;
+  for (int i = threadIdx.x; i < B * NC; i += NTB) {
+    const int kz = i / NC, c = i % NC;
+    if (c < nc) {
+      const int col = c0 + c, kx = col % XH1, ky = col / XH1;
+      const long idx = (long)kz * G::PL + col;
+      const c32 v = buf[i];
+      c32 r;
+      if (kx == 0 && ky == 0 && kz == 0) {
+        r = mk(0.f, 0.f);
+      } else if (MODE == Z_PART || MODE == Z_WEDGE || (MODE == Z_FINAL && !wedge)) {
+        if (MODE == Z_WEDGE && !wedge_keep<B>(kx, ky, kz, tan_a)) {
+          r = mk(0.f, 0.f);
+        } else {
+          const c32 fm = C.bx.FM[idx], fb = C.bx.FB[idx];
+          r = mk(v.x - f1 * fm.x - f2 * fb.x, v.y - f1 * fm.y - f2 * fb.y);
+          if (MODE == Z_FINAL) r = cmulc(Pa[idx], r);  // conj(Tw) . P
+        }
+      } else {  // Z_WREF, or Z_FINAL after the particle wedge: boolean-mask mean
+        const c32 fb = C.bx.FB[idx];
+        r = mk(v.x - f1 * fb.x, v.y - f1 * fb.y);
+        if (MODE == Z_FINAL) r = cmulc(Pa[idx], r);
+      }
+      buf[i] = r;
+    }
+  }
+  __syncthreads();
+  c32* dst;
+  if (MODE == Z_WEDGE || MODE == Z_FINAL) {
+    lds_fft<float, B, NC, NTB, 1, Inter<NC>>(buf, tw);
+    dst = Hi;
+  } else {
+    dst = (MODE == Z_PART ? C.Pp : C.Pw) + (long)item * G::NH;
+  }
+  for (int i = threadIdx.x; i < B * NC; i += NTB) {
+    const int zz = i / NC, c = i % NC;
+    if (c < nc) dst[(long)zz * G::PL + c0 + c] = buf[i];
+  }
+}
+
+// grid (ceil(PL/NC), np A): the particle spectrum times the template's wedge
+// turned by the item's orientation (DC 0), z inverse -> H[item]
+template <int B>
+__global__ __launch_bounds__(NTB) void k_b_wref(Chunk C) {
+  using G = Geo<B>;
+  constexpr int NCOL = (int)G::PL, XH1 = B / 2 + 1;
+  __shared__ c32 buf[B * NC];
+  __shared__ c32 tw[B];
+  __shared__ float Pr[12];
+  const int c0 = blockIdx.x * NC, item = blockIdx.y;
+  const int nc = (NCOL - c0 < NC) ? NCOL - c0 : NC;
+  const int ci = item / C.A, a = item % C.A, p = C.pid[ci];
+  for (int i = threadIdx.x; i < B; i += NTB) tw[i] = C.tw[i];
+  if (threadIdx.x < 12) Pr[threadIdx.x] = C.wmap[((long)p * C.A + a) * 12 + threadIdx.x];
+  __syncthreads();
+  const c32* src = C.Pp + (long)ci * G::NH;
+  for (int i = threadIdx.x; i < B * NC; i += NTB) {
+    const int kz = i / NC, c = i % NC;
+    c32 v = mk(0.f, 0.f);
+    if (c < nc) {
+      const int col = c0 + c, kx = col % XH1, ky = col / XH1;
+      const long idx = (long)kz * G::PL + col;
+      if (!((kx == 0 && ky == 0 && kz == 0) || !wedge_keep_rot<B>(kx, ky, kz, Pr, C.tan_r))) v = src[idx];
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  lds_fft<float, B, NC, NTB, 1, Inter<NC>>(buf, tw);
+  c32* dst = C.H + (long)item * G::NH;
+  for (int i = threadIdx.x; i < B * NC; i += NTB) {
+    const int zz = i / NC, c = i % NC;
+    if (c < nc) dst[(long)zz * G::PL + c0 + c] = buf[i];
+  }
+}
+
+// grid (B, np A): plane z of H[item] back to real space, boolean re-mask
+// (M != 0) with its two sums (into S2t, or S2p with PART), and forward again
+// into the same plane of H[item]
+template <int B, bool PART>
+__global__ __launch_bounds__(NTB) void k_b_remask(Chunk C) {
+  using G = Geo<B>;
+  __shared__ c32 buf[B * B];
+  __shared__ c32 tw[B];
+  __shared__ double red[2 * (NTB / 64)];
+  const int z = blockIdx.x, item = blockIdx.y;
+  if (!PART && C.tanw[C.pid[item / C.A]] < 0.f) return;  // no particle wedge: nothing to re-mask
+  for (int i = threadIdx.x; i < B; i += NTB) tw[i] = C.tw[i];
+  c32* Hz = C.H + (long)item * G::NH + (long)z * G::PL;
+  plane_inv_lds<B>(Hz, buf, tw);
+  const float* M = C.bx.M + (long)z * B * B;
+  double t[2] = {0, 0};
+  for (int i = threadIdx.x; i < B * B; i += NTB) {
+    const float v = buf[i].x;
+    const bool in = M[i] != 0.f;
+    buf[i] = mk(in ? v : 0.f, 0.f);
+    if (in) {
+      t[0] += (double)v;
+      t[1] += (double)v * (double)v;
+    }
+  }
+  blk_sum<2>(t, red);
+  double* S = (PART ? C.S2p : C.S2t) + ((long)item * B + z) * 2;
+  if (threadIdx.x == 0) {
+    S[0] = t[0];
+    S[1] = t[1];
+  }
+  plane_fwd_lds<B>(buf, tw, Hz);
+}
+
+// grid (B, np A): plane z of the correlation back to real space; its maximum
+// over mask_cc != 0 (first in x-fastest order) -> pkv / pki[item][z]
+template <int B>
+__global__ __launch_bounds__(NTB) void k_b_peak(Chunk C) {
+  using G = Geo<B>;
+  __shared__ c32 buf[B * B];
+  __shared__ c32 tw[B];
+  __shared__ float rv[NTB / 64];
+  __shared__ int ri[NTB / 64];
+  const int z = blockIdx.x, item = blockIdx.y;
+  for (int i = threadIdx.x; i < B; i += NTB) tw[i] = C.tw[i];
+  plane_inv_lds<B>(C.H + (long)item * G::NH + (long)z * G::PL, buf, tw);
+  const float* Mcc = C.bx.Mcc ? C.bx.Mcc + (long)z * B * B : nullptr;
+  float pv = -FLT_MAX;
+  int pi = 0x7fffffff;
+  for (int i = threadIdx.x; i < B * B; i += NTB) {
+    if (Mcc == nullptr || Mcc[i] != 0.f) {
+      const float v = buf[i].x;
+      const int gi = z * B * B + i;
+      if (v > pv || (v == pv && gi < pi)) {
+        pv = v;
+        pi = gi;
       }
     }
-    wg_sum<4>(s, red);
-    const double mu1 = s[0] / n;
-    const double mu2 = (s[1] - mu1 * SM) / n;
-    const float fm1 = (float)mu1, fm2 = (float)mu2;
-    double var_t = (s[2] - 2.0 * mu1 * s[3] + mu1 * mu1 * bx.SM2 - n * mu2 * mu2) / numel;
-    plane_r2c<B>(R, H, buf, tw);
-    if (wedge) {
-      // Tf - DC, times the particle's wedge, DC = 0; back to real space
-      zpass<B, true>(H, nullptr, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
-        if ((kx == 0 && ky == 0 && kz == 0) || !wedge_keep<B>(kx, ky, kz, tan_a)) return mk(0.f, 0.f);
-        const c32 fm = FM[idx], fb = FB[idx];
-        return mk(v.x - fm1 * fm.x - fm2 * fb.x, v.y - fm1 * fm.y - fm2 * fb.y);
-      });
-      // boolean re-mask with the variance of the re-masked volume
-      double t2[2] = {0, 0};
-      auto remask = [&](int x, int y, int z, float v) {
-        const long i = ((long)z * B + y) * B + x;
-        const bool in = M[i] != 0.f;
-        R[i] = in ? v : 0.f;
-        if (in) {
-          t2[0] += (double)v;
-          t2[1] += (double)v * (double)v;
-        }
-      };
-      plane_c2r<B>(H, buf, tw, remask);
-      wg_sum<2>(t2, red);
-      const double mb = t2[0] / n;
-      var_t = (t2[1] - n * mb * mb) / numel;
-      plane_r2c<B>(R, H, buf, tw);
-      const float fmb = (float)mb;
-      zpass<B, true>(H, nullptr, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
-        if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
-        const c32 fb = FB[idx];
-        const c32 w = mk(v.x - fmb * fb.x, v.y - fmb * fb.y);
-        return cmulc(Pa[idx], w);  // conj(Tw) . P
-      });
-    } else {
-      zpass<B, true>(H, nullptr, buf, tw, [&](int kx, int ky, int kz, c32 v, long idx) {
-        if (kx == 0 && ky == 0 && kz == 0) return mk(0.f, 0.f);
-        const c32 fm = FM[idx], fb = FB[idx];
-        const c32 w = mk(v.x - fm1 * fm.x - fm2 * fb.x, v.y - fm1 * fm.y - fm2 * fb.y);
-        return cmulc(Pa[idx], w);
-      });
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_xor(pv, o, 64);
+    const int oi = __shfl_xor(pi, o, 64);
+    if (ov > pv || (ov == pv && oi < pi)) {
+      pv = ov;
+      pi = oi;
     }
-    // ---- correlation back to real space; masked peak (first in x-fastest order)
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rv[threadIdx.x >> 6] = pv;
+    ri[threadIdx.x >> 6] = pi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pv = rv[0];
+    pi = ri[0];
+    for (int k = 1; k < NTB / 64; ++k) {
+      if (rv[k] > pv || (rv[k] == pv && ri[k] < pi)) {
+        pv = rv[k];
+        pi = ri[k];
+      }
+    }
+    C.pkv[(long)item * B + z] = pv;
+    C.pki[(long)item * B + z] = pi;
+  }
+}
+
+// one thread per particle of the chunk: the score of every orientation
+// (peak / (sqrt(var_p var_t) B^6)) and the first best one
+template <int B>
+__global__ void k_b_best(Chunk C) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= C.np) return;
+  const int p = C.pid[ci];
+  const bool wedge = C.tanw[p] >= 0.f, wref = C.tan_r >= 0.f;
+  const double numel = (double)B * B * B;
+  const double var_p0 = norm4(C.S4p + (long)ci * B * 4, B, C.bx).var;
+  int best = -1, bpos = 0;
+  float bestv = 0.f;
+  for (int a = 0; a < C.A; ++a) {
+    const long item = (long)ci * C.A + a;
     float pv = -FLT_MAX;
     int pi = 0x7fffffff;
-    const float* Mcc = bx.Mcc;
-    auto peak = [&](int x, int y, int z, float v) {
-      const int i = (z * B + y) * B + x;
-      if (Mcc == nullptr || Mcc[i] != 0.f) {
-        if (v > pv || (v == pv && i < pi)) {
-          pv = v;
-          pi = i;
-        }
-      }
-    };
-    plane_c2r<B>(H, buf, tw, peak);
-    // workgroup argmax
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) {
-      const float ov = __shfl_xor(pv, o, 64);
-      const int oi = __shfl_xor(pi, o, 64);
-      if (ov > pv || (ov == pv && oi < pi)) {
-        pv = ov;
-        pi = oi;
+    for (int z = 0; z < B; ++z) {  // planes in index order: strict '>' keeps the first
+      const float v = C.pkv[item * B + z];
+      const int i = C.pki[item * B + z];
+      if (v > pv || (v == pv && i < pi)) {
+        pv = v;
+        pi = i;
       }
     }
-    float* redf = reinterpret_cast<float*>(red);
-    int* redi = reinterpret_cast<int*>(red + NTS / 64);
-    if (lane == 0) {
-      redf[w] = pv;
-      redi[w] = pi;
-    }
-    __syncthreads();
-    pv = redf[0];
-    pi = redi[0];
-    for (int k = 1; k < NTS / 64; ++k) {
-      if (redf[k] > pv || (redf[k] == pv && redi[k] < pi)) {
-        pv = redf[k];
-        pi = redi[k];
-      }
-    }
-    __syncthreads();
+    const double var_p = wref ? norm2(C.S2p + item * B * 2, B, C.bx).var : var_p0;
+    const double var_t = wedge ? norm2(C.S2t + item * B * 2, B, C.bx).var : norm4(C.S4t + item * B * 4, B, C.bx).var;
     const double den = sqrt(var_p * var_t) * numel * numel;
     const float val = (den > 0 && pi != 0x7fffffff) ? (float)((double)pv / den) : -FLT_MAX;
     if (pi != 0x7fffffff && den > 0 && (best < 0 || val > bestv)) {
@@ -448,35 +553,11 @@ __device__ __noinline__ void process(const Job& J, int p, c32* buf, c32* tw, dou
       bpos = pi;
     }
   }
-  if (threadIdx.x == 0) {
-    J.best_idx[p] = best;
-    J.best_val[p] = best >= 0 ? bestv : NAN;
-    J.best_pos[3 * p + 0] = best >= 0 ? bpos % B : 0;
-    J.best_pos[3 * p + 1] = best >= 0 ? (bpos / B) % B : 0;
-    J.best_pos[3 * p + 2] = best >= 0 ? bpos / (B * B) : 0;
-  }
-}
-
-__global__ __launch_bounds__(NTS) void k_subtomo(Job J) {
-  __shared__ c32 buf[BMAX * BMAX];
-  __shared__ c32 tw[BMAX];
-  __shared__ double red[4 * (NTS / 64)];
-  __shared__ int sh_q;
-  while (true) {
-    if (threadIdx.x == 0) sh_q = atomicAdd(J.counter, 1);
-    __syncthreads();
-    const int q = sh_q;
-    __syncthreads();
-    if (q >= J.P) break;  // every wave of the workgroup leaves together
-    const int p = J.order[q];
-    const int B = J.boxes[J.pbox[p]].B;
-    if (B == 32)
-      process<32>(J, p, buf, tw, red);
-    else if (B == 48)
-      process<48>(J, p, buf, tw, red);
-    else
-      process<64>(J, p, buf, tw, red);
-  }
+  C.best_idx[p] = best;
+  C.best_val[p] = best >= 0 ? bestv : NAN;
+  C.best_pos[3 * p + 0] = best >= 0 ? bpos % B : 0;
+  C.best_pos[3 * p + 1] = best >= 0 ? (bpos / B) % B : 0;
+  C.best_pos[3 * p + 2] = best >= 0 ? bpos / (B * B) : 0;
 }
 
 // Per box: FFT(M) and FFT(M != 0) half spectra. One workgroup per box.
@@ -529,8 +610,6 @@ __global__ __launch_bounds__(1024) void k_center(const float* T, float* out, lon
 
 }  // namespace
 
-int subtomo_workgroups() { return 1024; }
-
 void simple_wedge_map(const float euler_deg[3], int B, float out[12]) {
   // tom_mex_corr3d converts degrees with a * (PI / 180.) (tom_mex_corr3d.cpp:247-249)
   const double kPi = 3.141592653589793238512808959406186204433;
@@ -569,17 +648,45 @@ long subtomo_scratch_floats() {
   return n3 + 6 * nh;
 }
 
+template <int B>
+static void run_box_chunks(hipStream_t s, Chunk C, const int32_t* d_ids, long nids, bool wref, long budget) {
+  using G = Geo<B>;
+  const int A = C.A;
+  // chunk of whole particles: its items' spectra and sums within the budget,
+  // and at most 65,535 items (grid y)
+  const long per_item = G::NH * 8L * (wref ? 2 : 1) + (long)B * (4 + 2 + 2) * 8 + (long)B * 8;
+  const long per_part = G::NH * 8L + (long)B * 4 * 8;
+  long npc = budget / (A * per_item + per_part);
+  npc = std::max(1L, std::min(npc, 65535L / A));
+  const unsigned ncolb = (unsigned)((G::PL + NC - 1) / NC);
+  for (long c0 = 0; c0 < nids; c0 += npc) {
+    C.np = (int)std::min(npc, nids - c0);
+    C.pid = d_ids + c0;
+    const unsigned np = (unsigned)C.np, ni = (unsigned)(C.np * A);
+    hipLaunchKernelGGL((k_b_load<B, false>), dim3(B, np), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL((k_b_z<B, Z_PART>), dim3(ncolb, np), dim3(NTB), 0, s, C);
+    if (wref) {
+      hipLaunchKernelGGL(k_b_wref<B>, dim3(ncolb, ni), dim3(NTB), 0, s, C);
+      hipLaunchKernelGGL((k_b_remask<B, true>), dim3(B, ni), dim3(NTB), 0, s, C);
+      hipLaunchKernelGGL((k_b_z<B, Z_WREF>), dim3(ncolb, ni), dim3(NTB), 0, s, C);
+    }
+    hipLaunchKernelGGL((k_b_load<B, true>), dim3(B, ni), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL((k_b_z<B, Z_WEDGE>), dim3(ncolb, ni), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL((k_b_remask<B, false>), dim3(B, ni), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL((k_b_z<B, Z_FINAL>), dim3(ncolb, ni), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL(k_b_peak<B>, dim3(B, ni), dim3(NTB), 0, s, C);
+    hipLaunchKernelGGL(k_b_best<B>, dim3((np + 63) / 64), dim3(64), 0, s, C);
+    TOMHIP_CHECK(hipGetLastError());
+  }
+}
+
 void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h, const float* templates_dev,
-                 const long* tmpl_off, int64_t P, const int32_t* pbox_dev, const int32_t* order_dev,
-                 const float* particles_dev, const long* poff_dev, const float* tan_dev, const float* rms_dev,
-                 const float* tan_ref_dev, const float* wref_P_dev, int64_t A, int32_t* best_idx, int32_t* best_pos,
-                 float* best_val) {
+                 const long* tmpl_off, int64_t P, const std::vector<int32_t>& pbox_h, const float* particles_dev,
+                 const long* poff_dev, const float* tan_dev, const float* rms_dev, const std::vector<float>& tanr_h,
+                 const float* wref_P_dev, int64_t A, int32_t* best_idx, int32_t* best_pos, float* best_val) {
   const int nbox = (int)boxes_h.size();
   const long sstride = subtomo_scratch_floats();
-  const int G = subtomo_workgroups();
-  const long n3max = (long)BMAX * BMAX * BMAX;
   const long nhmax = (long)BMAX * BMAX * (BMAX / 2 + 1);
-  // per box: centred template, FFT(M), FFT(M != 0)
   std::vector<void*> owned;
   auto alloc = [&](size_t bytes) {
     void* p = nullptr;
@@ -588,6 +695,7 @@ void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h,
     return p;
   };
   try {
+    // per box: centred template, FFT(M), FFT(M != 0)
     for (int k = 0; k < nbox; ++k) {
       SubtomoBox& b = boxes_h[k];
       const long n3 = (long)b.B * b.B * b.B;
@@ -600,37 +708,83 @@ void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h,
     }
     SubtomoBox* boxes_d = (SubtomoBox*)alloc(sizeof(SubtomoBox) * nbox);
     TOMHIP_CHECK(hipMemcpyAsync(boxes_d, boxes_h.data(), sizeof(SubtomoBox) * nbox, hipMemcpyHostToDevice, s));
-    float* scratch = (float*)alloc(sizeof(float) * sstride * (size_t)(G > nbox ? G : nbox));
-    (void)n3max;
+    float* scratch = (float*)alloc(sizeof(float) * sstride * (size_t)nbox);
     hipLaunchKernelGGL(k_box_prep, dim3(nbox), dim3(NTS), 0, s, boxes_d, scratch, sstride, tws.f32(32),
                        tws.f32(48), tws.f32(64));
     TOMHIP_CHECK(hipGetLastError());
-    int* counter = (int*)alloc(sizeof(int));
-    TOMHIP_CHECK(hipMemsetAsync(counter, 0, sizeof(int), s));
-    Job J;
-    J.boxes = boxes_d;
-    J.order = order_dev;
-    J.pbox = pbox_dev;
-    J.particles = particles_dev;
-    J.poff = poff_dev;
-    J.tan_wedge = tan_dev;
-    J.rms = rms_dev;
-    J.tan_ref = tan_ref_dev;
-    J.wref_P = wref_P_dev;
-    J.A = (int)A;
-    J.P = (int)P;
-    J.counter = counter;
-    J.scratch = scratch;
-    J.sstride = sstride;
-    J.best_idx = best_idx;
-    J.best_pos = best_pos;
-    J.best_val = best_val;
-    J.tw[0] = tws.f32(32);
-    J.tw[1] = tws.f32(48);
-    J.tw[2] = tws.f32(64);
-    const int grid = (int)((P < G) ? P : G);
-    hipLaunchKernelGGL(k_subtomo, dim3(grid), dim3(NTS), 0, s, J);
-    TOMHIP_CHECK(hipGetLastError());
+    // particles of each box, in input order
+    std::vector<std::vector<int32_t>> ids(nbox);
+    for (int64_t i = 0; i < P; ++i) ids[pbox_h[i]].push_back((int32_t)i);
+    std::vector<int32_t> all;
+    std::vector<long> first(nbox + 1, 0);
+    for (int k = 0; k < nbox; ++k) {
+      first[k] = (long)all.size();
+      all.insert(all.end(), ids[k].begin(), ids[k].end());
+    }
+    first[nbox] = (long)all.size();
+    int32_t* d_ids = (int32_t*)alloc(sizeof(int32_t) * all.size());
+    TOMHIP_CHECK(hipMemcpyAsync(d_ids, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, s));
+    // chunk buffers sized once for the largest box of this call
+    const long budget = 2L << 30;
+    long maxH = 0, maxPp = 0, maxPw = 0, maxS = 0, maxIt = 0, maxNp = 0;
+    for (int k = 0; k < nbox; ++k) {
+      if (ids[k].empty()) continue;
+      const int B = boxes_h[k].B;
+      const long nh = (long)B * B * (B / 2 + 1);
+      const bool wref = tanr_h[k] >= 0.f;
+      const long per_item = nh * 8L * (wref ? 2 : 1) + (long)B * (4 + 2 + 2) * 8 + (long)B * 8;
+      const long per_part = nh * 8L + (long)B * 4 * 8;
+      long npc = budget / (A * per_item + per_part);
+      npc = std::max(1L, std::min(npc, 65535L / A));
+      npc = std::min(npc, (long)ids[k].size());
+      maxH = std::max(maxH, npc * A * nh);
+      maxPp = std::max(maxPp, npc * nh);
+      if (wref) maxPw = std::max(maxPw, npc * A * nh);
+      maxS = std::max(maxS, npc * A * (long)B);
+      maxIt = std::max(maxIt, npc * A);
+      maxNp = std::max(maxNp, npc * (long)B);
+    }
+    Chunk C{};
+    C.A = (int)A;
+    C.particles = particles_dev;
+    C.poff = poff_dev;
+    C.tanw = tan_dev;
+    C.rms = rms_dev;
+    C.wmap = wref_P_dev;
+    C.H = (c32*)alloc(sizeof(c32) * maxH);
+    C.Pp = (c32*)alloc(sizeof(c32) * maxPp);
+    C.Pw = maxPw ? (c32*)alloc(sizeof(c32) * maxPw) : nullptr;
+    C.S4p = (double*)alloc(sizeof(double) * 4 * maxNp);
+    C.S4t = (double*)alloc(sizeof(double) * 4 * maxS);
+    C.S2t = (double*)alloc(sizeof(double) * 2 * maxS);
+    C.S2p = (double*)alloc(sizeof(double) * 2 * maxS);
+    C.pkv = (float*)alloc(sizeof(float) * maxS);
+    C.pki = (int*)alloc(sizeof(int) * maxS);
+    C.best_idx = best_idx;
+    C.best_pos = best_pos;
+    C.best_val = best_val;
+    (void)maxIt;
+    // largest boxes first
+    std::vector<int> korder(nbox);
+    for (int k = 0; k < nbox; ++k) korder[k] = k;
+    std::stable_sort(korder.begin(), korder.end(), [&](int a, int b) { return boxes_h[a].B > boxes_h[b].B; });
+    for (int k : korder) {
+      if (ids[k].empty()) continue;
+      C.bx = boxes_h[k];
+      C.tan_r = tanr_h[k];
+      const bool wref = tanr_h[k] >= 0.f;
+      const long nids = first[k + 1] - first[k];
+      if (C.bx.B == 32) {
+        C.tw = tws.f32(32);
+        run_box_chunks<32>(s, C, d_ids + first[k], nids, wref, budget);
+      } else if (C.bx.B == 48) {
+        C.tw = tws.f32(48);
+        run_box_chunks<48>(s, C, d_ids + first[k], nids, wref, budget);
+      } else {
+        C.tw = tws.f32(64);
+        run_box_chunks<64>(s, C, d_ids + first[k], nids, wref, budget);
+      }
+    }
     TOMHIP_CHECK(hipStreamSynchronize(s));
   } catch (...) {
     for (void* p : owned) (void)hipFree(p);
