@@ -289,6 +289,11 @@ int tomhip_unet_num_params(tomhip_unet* h, int64_t* n);
  * 1a 1b, head), each layer W[taps][Cout][Cin] then bias[Cout] */
 int tomhip_unet_get_params(tomhip_unet* h, float* out);
 int tomhip_unet_set_params(tomhip_unet* h, const float* in);
+/* Adam state of a training network, for checkpoints: first and second
+ * moments m, v in the parameter order above (num_params floats each) and the
+ * number of updates taken (bias correction uses it) */
+int tomhip_unet_get_optimizer(tomhip_unet* h, float* m, float* v, int64_t* step);
+int tomhip_unet_set_optimizer(tomhip_unet* h, const float* m, const float* v, int64_t step);
 /* volume and the optional aux map (NULL = zeros) fill input channels 0 and 1 */
 int tomhip_unet_forward(tomhip_unet* h, const float* volume, const float* aux);
 /* logits [V][80]: channel 0 particle, 1..nbin orientation bins, nbin+1..nbin+3

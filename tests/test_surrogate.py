@@ -1,5 +1,7 @@
 """Host logic of the learned proposal (labels, greedy suppression, recall) and
 the GPU candidate extraction against brute force."""
+import os
+
 import numpy as np
 import pytest
 
@@ -199,3 +201,40 @@ def test_pruned_search_finds_exhaustive_peaks_with_exact_scores(api, oracle):
         x, y, z = pos[i]
         print(f"  candidate {i}: gpu {sc[i]:.6f} fp64 {s64[x, y, z]:.6f}", flush=True)
         assert abs(sc[i] - s64[x, y, z]) < 1e-4
+
+
+@pytest.mark.gpu
+def test_train_script_resumes_across_runs(api, tmp_path):
+    """train_surrogate with a state directory: a run stopped by its time
+    limit during labelling, then one stopped after its first training step,
+    then one without a limit, ends at the weights of an uninterrupted run
+    (to the float-atomic noise of the weight gradients: 1 % of the distance
+    the weights moved), and the picks came from the label cache."""
+    import json
+    from tom_amd import train_surrogate
+    print("train_surrogate resume", flush=True)
+    base = ["--tomos", "2", "--size", "128", "--particles", "6", "--steps", "12", "--crop", "32", "--coarse", "64",
+            "--ckpt-every", "4", "--seed", "7"]
+    st, out_r, out_s = str(tmp_path / "st"), str(tmp_path / "r.npz"), str(tmp_path / "s.npz")
+    for expect_files in (["label-0000.npz"], ["label-0000.npz", "label-0001.npz", "train.npz"]):
+        with pytest.raises(SystemExit) as e:
+            train_surrogate.main(base + ["--state", st, "--time-limit", "0", "--out", out_r])
+        assert e.value.code == 3
+        assert sorted(os.listdir(st)) == expect_files
+    z = np.load(os.path.join(st, "train.npz"), allow_pickle=False)
+    assert int(z["step"]) == 1 and len(z["history"]) == 1
+    train_surrogate.main(base + ["--state", st, "--out", out_r])
+    z = np.load(os.path.join(st, "train.npz"), allow_pickle=False)
+    assert int(z["step"]) == 12 and len(z["history"]) == 12
+    train_surrogate.main(base + ["--out", out_s])
+    r, s = np.load(out_r, allow_pickle=False), np.load(out_s, allow_pickle=False)
+    assert json.loads(str(r["meta"]))["peaks_per_tomogram"] == json.loads(str(s["meta"]))["peaks_per_tomogram"]
+    with api.Unet((32, 32, 32), nbin=64, seed=7, train=True) as net:
+        theta0 = net.get_params()
+    rel = np.linalg.norm(r["params"] - s["params"]) / np.linalg.norm(s["params"] - theta0)
+    print(f"  |resumed - uninterrupted| / |moved| = {rel:.2e}", flush=True)
+    assert rel <= 1e-2
+    # a state directory of other settings is refused
+    with pytest.raises(SystemExit) as e:
+        train_surrogate.main(base[:-1] + ["8", "--state", st, "--out", out_r])
+    assert "other settings" in str(e.value.code)

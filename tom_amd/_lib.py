@@ -64,6 +64,8 @@ SIGNATURES = {
     "tomhip_unet_num_params": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_get_params": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_set_params": (ctypes.c_int, [vp, vp]),
+    "tomhip_unet_get_optimizer": (ctypes.c_int, [vp, vp, vp, vp]),
+    "tomhip_unet_set_optimizer": (ctypes.c_int, [vp, vp, vp, ctypes.c_int64]),
     "tomhip_unet_forward": (ctypes.c_int, [vp, vp, vp]),
     "tomhip_unet_get_logits": (ctypes.c_int, [vp, vp]),
     "tomhip_unet_backward": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_float, vp]),

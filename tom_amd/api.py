@@ -705,6 +705,23 @@ class Unet:
             raise ValueError("parameter vector has the wrong length")
         self.ctx.check(self.ctx.lib.tomhip_unet_set_params(self.h, _p(t)))
 
+    def get_optimizer(self):
+        """Adam state (m, v, step) of a training network."""
+        n = self.num_params()
+        m = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        st = np.zeros(1, np.int64)
+        self.ctx.check(self.ctx.lib.tomhip_unet_get_optimizer(self.h, _p(m), _p(v), _p(st)))
+        return m, v, int(st[0])
+
+    def set_optimizer(self, m, v, step: int):
+        n = self.num_params()
+        m = np.ascontiguousarray(np.asarray(m, np.float32))
+        v = np.ascontiguousarray(np.asarray(v, np.float32))
+        if m.size != n or v.size != n:
+            raise ValueError("optimizer state has the wrong length")
+        self.ctx.check(self.ctx.lib.tomhip_unet_set_optimizer(self.h, _p(m), _p(v), int(step)))
+
     def forward(self, vol, aux=None):
         """vol (and the optional aux map): normalised inputs of the network size."""
         vb = _Buf(vol)

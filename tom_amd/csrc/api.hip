@@ -1203,6 +1203,22 @@ int tomhip_unet_set_params(tomhip_unet* h, const float* in) {
   });
 }
 
+int tomhip_unet_get_optimizer(tomhip_unet* h, float* m, float* v, int64_t* step) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && m && v && step, ERR_WRONG_INPUT, "null pointer");
+    h->u->optimizer(m, v, step, true);
+  });
+}
+
+int tomhip_unet_set_optimizer(tomhip_unet* h, const float* m, const float* v, int64_t step) {
+  return guard(h ? h->ctx : nullptr, [&] {
+    TOMHIP_REQUIRE(h && m && v, ERR_WRONG_INPUT, "null pointer");
+    const long n = h->u->num_params();
+    std::vector<float> hm(m, m + n), hv(v, v + n);
+    h->u->optimizer(hm.data(), hv.data(), &step, false);
+  });
+}
+
 int tomhip_unet_forward(tomhip_unet* h, const float* volume, const float* aux) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && volume, ERR_WRONG_INPUT, "null pointer");

@@ -29,6 +29,9 @@ class Unet {
   // all parameters in layer order, each layer W [taps][Cout][Cin] then bias
   void params(float* host, bool get);
   void grads(float* host);
+  // Adam state for checkpoints: first and second moments in the parameter
+  // order, and the number of updates taken (training networks only)
+  void optimizer(float* m, float* v, int64_t* step, bool get);
   // vol: device f32 [D][H][W] (x fastest), already normalised; aux (optional,
   // same layout, normalised): a second input channel
   // head = false stops before the 1x1x1 head (no FP32 logits written); the

@@ -914,6 +914,26 @@ void Unet::grads(float* out) {
   TOMHIP_CHECK(hipStreamSynchronize(I.s));
 }
 
+void Unet::optimizer(float* m, float* v, int64_t* step, bool get) {
+  Impl& I = *d_;
+  TOMHIP_REQUIRE(I.train, ERR_WRONG_INPUT, "U-Net was created for inference only");
+  if (!get) TOMHIP_REQUIRE(*step >= 0 && *step < (1 << 30), ERR_WRONG_INPUT, "Adam step out of range");
+  long off = 0;
+  for (auto& c : I.L) {
+    const hipMemcpyKind k = get ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    TOMHIP_CHECK(hipMemcpyAsync(get ? (void*)(m + off) : (void*)c.m, get ? (const void*)c.m : (const void*)(m + off),
+                                sizeof(float) * c.np, k, I.s));
+    TOMHIP_CHECK(hipMemcpyAsync(get ? (void*)(v + off) : (void*)c.v, get ? (const void*)c.v : (const void*)(v + off),
+                                sizeof(float) * c.np, k, I.s));
+    off += c.np;
+  }
+  if (get)
+    *step = I.step;
+  else
+    I.step = (int)*step;
+  TOMHIP_CHECK(hipStreamSynchronize(I.s));
+}
+
 void Unet::forward(const float* vol, const float* aux, bool head) {
   Impl& I = *d_;
   hipStream_t s = I.s;

@@ -21,6 +21,8 @@ the best orientation there gives the bin.
 """
 from __future__ import annotations
 
+import json
+import os
 import time
 
 import numpy as np
@@ -213,16 +215,62 @@ class Trainer:
         self.history.append((bce, ce, off))
         return bce, ce, off
 
-    def train(self, steps, log=None, every=100):
-        """Cosine learning-rate decay from lr to lr_min over the steps."""
+    def train(self, steps, log=None, every=100, start=0, checkpoint=None, checkpoint_every=0, deadline=None):
+        """Cosine learning-rate decay from lr to lr_min over the steps, from
+        loop index `start` (a resumed run). checkpoint(i) is called with the
+        next loop index every checkpoint_every steps; past `deadline`
+        (time.perf_counter() value) the loop checkpoints and stops early.
+        self.finished tells whether it reached `steps`."""
         t0 = time.perf_counter()
-        for i in range(steps):
+        self.finished = False
+        for i in range(start, steps):
             lr = self.lr_min + 0.5 * (self.lr - self.lr_min) * (1 + np.cos(np.pi * i / max(1, steps - 1)))
             bce, ce, off = self.step(lr)
             if log and (i % every == 0 or i == steps - 1):
                 log(f"[train] step {i}: bce {bce:.4f} ce {ce:.4f} offset {off:.4f} "
                     f"({time.perf_counter() - t0:.1f} s)")
+            if checkpoint is not None and i + 1 < steps:
+                late = deadline is not None and time.perf_counter() > deadline
+                if late or (checkpoint_every and (i + 1) % checkpoint_every == 0):
+                    checkpoint(i + 1)
+                if late:
+                    return time.perf_counter() - t0
+        self.finished = True
         return time.perf_counter() - t0
+
+    # ---- checkpoints: network parameters, Adam moments and step, the crop
+    # generator's state and the loss history, so a resumed run takes the same
+    # crops and updates as one that was never stopped
+    def state(self):
+        m, v, t = self.net.get_optimizer()
+        return dict(params=self.net.get_params(), adam_m=m, adam_v=v, adam_step=np.int64(t),
+                    rng=json.dumps(self.rng.bit_generator.state),
+                    history=np.asarray(self.history, np.float64).reshape(-1, 3))
+
+    def restore(self, st):
+        self.net.set_params(st["params"])
+        self.net.set_optimizer(st["adam_m"], st["adam_v"], int(st["adam_step"]))
+        self.rng.bit_generator.state = json.loads(str(st["rng"]))
+        self.history = [tuple(float(x) for x in h) for h in np.asarray(st["history"])]
+
+    def save(self, path, step, **extra):
+        """Write the checkpoint at loop index `step` (atomically: a run
+        killed while writing leaves the previous checkpoint)."""
+        save_npz_atomic(path, step=np.int64(step), **self.state(), **extra)
+
+    def load(self, path):
+        """Restore a checkpoint; returns (loop index to resume at, the npz)."""
+        z = np.load(path, allow_pickle=False)
+        self.restore(z)
+        return int(z["step"]), z
+
+
+def save_npz_atomic(path, **arrays):
+    """np.savez to a temporary file in the same directory, then rename."""
+    tmp = path + ".part"
+    with open(tmp, "wb") as f:
+        np.savez(f, **arrays)
+    os.replace(tmp, path)
 
 
 # ---------------------------------------------------------------- proposal + refinement
