@@ -5,7 +5,9 @@
 // candidates so the exact FLCF only runs as a local refinement.
 //
 // Numerics: BF16 activations and weights as MFMA operands, FP32 accumulation
-// (v_mfma_f32_16x16x16_bf16), FP32 master weights, FP32 gradients, Adam.
+// (v_mfma_f32_16x16x32_bf16 for the 3x3x3 convolutions, 16x16x16 for the
+// 1x1x1 head and the weight gradients), FP32 master weights, FP32 gradients,
+// Adam.
 // Layout: channels-last [z][y][x][c], c padded to a multiple of 16.
 //
 // Convolution = implicit GEMM per 16-voxel x 16-channel tile:
@@ -13,6 +15,7 @@
 //   grad data   dX[u][ci] = sum_t sum_co dY[u - d_t][co] W[t][co][ci]
 //               = the forward with W'[26-t][ci][co] = W[t][co][ci]
 //   grad weight dW[t][co][ci] = sum_v dY[v][co] X[v + d_t][ci]  (K = voxels)
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -29,6 +32,7 @@ namespace tomhip {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -214,10 +218,257 @@ __global__ __launch_bounds__(NTU) void k_conv_t(const uint16_t* __restrict__ X, 
   }
 }
 
-// dW[t][co][ci] += sum over a chunk of voxels of dY[v][co] X[v + d_t][ci].
+// 3x3x3 convolution on v_mfma_f32_16x16x32_bf16 with the output as
+// D[cout][voxel] = W[cout][k] X[k][voxel]. K runs over chunks of CC input
+// channels; within a chunk a K step of 32 covers NQ = 32 / CC in-plane tap
+// positions (dy, dx) x CC channels for one dz. A block stages the input tile
+// (16 + 2) x (4 + 2) x (TZ + 2) voxels x CC channels of one chunk in LDS
+// (zero outside the volume; chunk q starts xq elements after chunk q - 1:
+// CC in an interleaved record, a plane stride in a chunk-planar layout;
+// voxel pitch CC + 8 bf16 in LDS, so 8 lanes reading 16
+// bytes at consecutive voxels hit distinct bank groups). Wave w owns row
+// y0 + w and TZ planes: for each group of tap positions it reads the TZ + 2
+// plane fragments once and feeds each to the three dz taps that use it
+// (3 TZ MFMAs per TZ + 2 LDS fragment reads per column tile), with the three
+// dz weight fragments per column tile loaded once from global memory in
+// fragment order (1 KB contiguous per wave). Blocks are numbered so each XCD
+// takes a contiguous range of tiles (neighbouring tiles share their halos in
+// that XCD's L2).
+template <int CC>
+struct Frag {
+  static constexpr int NQ = 32 / CC;             // tap positions per K step
+  static constexpr int NG = (9 + NQ - 1) / NQ;   // K steps per dz and chunk
+};
+
+template <int CT, int CC, int TZ>
+__global__ __launch_bounds__(NTU) void k_conv3(const uint16_t* __restrict__ X, int xs, int xo, long xq, int Cin,
+                                               Dims d, const uint4* __restrict__ Wm, const float* __restrict__ bias,
+                                               int relu, uint16_t* Yb, int ys, int yo, float* Yf, int nbx, int nby) {
+  extern __shared__ uint4 tile4[];
+  constexpr int SX = CTX + 2, SY = CTY + 2, SZ = TZ + 2;
+  constexpr int CS = CC / 8;       // 16-byte slots of one voxel's channels
+  constexpr int PV = CS + 1;       // voxel pitch in slots
+  constexpr int NQ = Frag<CC>::NQ, NG = Frag<CC>::NG;
+  constexpr int PL = SY * SX * PV;  // plane pitch in slots
+  int b = blockIdx.x;
+  const int nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int x0 = (b % nbx) * CTX, y0 = ((b / nbx) % nby) * CTY, z0 = (b / (nbx * nby)) * TZ;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = lane & 15, kg = lane >> 4;
+  const int y = y0 + wv;
+  const bool live = y < d.H;
+  f32x4 acc[TZ][CT];
+#pragma unroll
+  for (int zp = 0; zp < TZ; ++zp)
+#pragma unroll
+    for (int j = 0; j < CT; ++j) acc[zp][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // this lane's K slice within a step: tap position kq of the group, channel
+  // slot cs of the chunk
+  const int kq = (8 * kg) / CC, cs = ((8 * kg) % CC) / 8;
+  const int nch = Cin / CC;
+  for (int q = 0; q < nch; ++q) {
+    if (q) __syncthreads();
+    for (int i = threadIdx.x; i < SX * SY * SZ * CS; i += NTU) {
+      const int sl = i % CS, v = i / CS;
+      const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / (SX * SY) - 1;
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D)
+        val = *reinterpret_cast<const uint4*>(X + q * xq + (((long)gz * d.H + gy) * d.W + gx) * xs + xo + sl * 8);
+      tile4[v * PV + sl] = val;
+    }
+    __syncthreads();
+    if (live) {
+      const uint4* wq = Wm + (long)q * NG * 3 * CT * 64 + lane;
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int qi = g * NQ + kq;       // in-plane tap position 3 (dy + 1) + (dx + 1)
+        const bool ok = qi < 9;
+        const int qc = ok ? qi : 0;
+        const uint4* ap = tile4 + ((wv + qc / 3) * SX + (n + qc % 3)) * PV + cs;
+        bf16x8 wf[3][CT];
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+          for (int j = 0; j < CT; ++j) wf[dz][j] = __builtin_bit_cast(bf16x8, wq[((g * 3 + dz) * CT + j) * 64]);
+        bf16x8 fr[SZ];
+#pragma unroll
+        for (int p = 0; p < SZ; ++p) {
+          uint4 r = ap[p * PL];
+          if (!ok) r = make_uint4(0u, 0u, 0u, 0u);
+          fr[p] = __builtin_bit_cast(bf16x8, r);
+        }
+#pragma unroll
+        for (int zp = 0; zp < TZ; ++zp)
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int j = 0; j < CT; ++j)
+              acc[zp][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[dz][j], fr[zp + dz], acc[zp][j], 0, 0, 0);
+      }
+    }
+  }
+  if (!live) return;
+  // D[m = 4 kg + i][n]: channel 16 j + 4 kg + i of voxel x0 + n
+  const int x = x0 + n;
+  if (x >= d.W) return;
+#pragma unroll
+  for (int zp = 0; zp < TZ; ++zp) {
+    const int z = z0 + zp;
+    if (z >= d.D) break;
+    const long vox = ((long)z * d.H + y) * d.W + x;
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int co = j * 16 + 4 * kg;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[i] = acc[zp][j][i] + (bias ? bias[co + i] : 0.f);
+        if (relu) v[i] = fmaxf(v[i], 0.f);
+      }
+      if (Yb) {
+        uint2 o;
+        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2*>(Yb + vox * ys + yo + co) = o;
+      }
+      if (Yf) *reinterpret_cast<float4*>(Yf + vox * (16 * CT) + co) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+}
+
+// Where the network input comes from: the tile of network size at (ox, oy,
+// oz) of a volume X x Y x Z (zero outside it), each channel normalised by
+// the mean and standard deviation of its whole volume when sums (device:
+// sum and sum of squares of the tomogram, then of the aux map, n voxels) is
+// given.
+struct InWin {
+  const float* vol;
+  const float* aux;
+  int X, Y, Z, ox, oy, oz;
+  const double* sums;
+  long n;
+};
+
+__device__ __forceinline__ void norm_of(const double* sums, long n, float* m, float* is) {
+  const double mean = sums[0] / (double)n;
+  const double var = sums[1] / (double)n - mean * mean;
+  *m = (float)mean;
+  *is = var > 0 ? (float)(1.0 / sqrt(var)) : 1.f;
+}
+
+// First layer: 3x3x3 from the two real input channels (the f32 tomogram and
+// aux map, normalised and rounded to bf16 as they are staged) to 16
+// channels, without the 14 zero channels of the padded input. K = 27 taps x
+// 2 channels = 54, padded to two K steps of 32: element e of lane group kg
+// in step s is tap 16 s + 4 kg + e / 2, channel e % 2, so each B element pair
+// is one packed 4-byte LDS word. Tile and output as k_conv3 (TZ planes per
+// wave).
+template <int TZ>
+__global__ __launch_bounds__(NTU) void k_conv_in(InWin in, Dims d, const uint16_t* __restrict__ Wt,
+                                                 const float* __restrict__ bias, uint16_t* Yb, int nbx, int nby) {
+  constexpr int SX = CTX + 2, SY = CTY + 2, SZ = TZ + 2, PL = SX * SY;
+  __shared__ uint32_t t2[SX * SY * SZ];
+  int b = blockIdx.x;
+  const int nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int x0 = (b % nbx) * CTX, y0 = ((b / nbx) % nby) * CTY, z0 = (b / (nbx * nby)) * TZ;
+  float m0 = 0.f, s0 = 1.f, m1 = 0.f, s1 = 1.f;
+  if (in.sums) {
+    norm_of(in.sums, in.n, &m0, &s0);
+    if (in.aux) norm_of(in.sums + 2, in.n, &m1, &s1);
+  }
+  for (int v = threadIdx.x; v < SX * SY * SZ; v += NTU) {
+    const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / PL - 1;
+    const int vx = in.ox + gx, vy = in.oy + gy, vz = in.oz + gz;
+    uint32_t w = 0;
+    if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D && vx >= 0 && vx < in.X && vy >= 0 &&
+        vy < in.Y && vz >= 0 && vz < in.Z) {
+      const long i = ((long)vz * in.Y + vy) * in.X + vx;
+      w = (uint32_t)f2bf((in.vol[i] - m0) * s0) | (in.aux ? (uint32_t)f2bf((in.aux[i] - m1) * s1) << 16 : 0u);
+    }
+    t2[v] = w;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = lane & 15, kg = lane >> 4;
+  const int y = y0 + wv, x = x0 + n;
+  if (y >= d.H) return;  // no barrier follows
+  bf16x8 wa[2];
+  int off[2][4];
+  bool ok[2][4];
+#pragma unroll
+  for (int st = 0; st < 2; ++st)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int t = 16 * st + 4 * kg + h;
+      ok[st][h] = t < 27;
+      const int tc = t < 27 ? t : 0;
+      off[st][h] = ((tc / 9) * SY + (wv + (tc / 3) % 3)) * SX + (n + tc % 3);
+      uint16_t w0 = 0, w1 = 0;
+      if (t < 27) {
+        w0 = Wt[((long)t * 16 + n) * 16];
+        w1 = Wt[((long)t * 16 + n) * 16 + 1];
+      }
+      wa[st][2 * h] = __builtin_bit_cast(__bf16, w0);
+      wa[st][2 * h + 1] = __builtin_bit_cast(__bf16, w1);
+    }
+  float bb[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bb[i] = bias[4 * kg + i];
+#pragma unroll
+  for (int zp = 0; zp < TZ; ++zp) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      uint4 r;
+      r.x = ok[st][0] ? t2[off[st][0] + zp * PL] : 0u;
+      r.y = ok[st][1] ? t2[off[st][1] + zp * PL] : 0u;
+      r.z = ok[st][2] ? t2[off[st][2] + zp * PL] : 0u;
+      r.w = ok[st][3] ? t2[off[st][3] + zp * PL] : 0u;
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[st], __builtin_bit_cast(bf16x8, r), acc, 0, 0, 0);
+    }
+    const int z = z0 + zp;
+    if (z >= d.D || x >= d.W) continue;
+    const long vox = ((long)z * d.H + y) * d.W + x;
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[i] + bb[i], 0.f);
+    uint2 o;
+    o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+    o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+    *reinterpret_cast<uint2*>(Yb + vox * 16 + 4 * kg) = o;
+  }
+}
+
+// Fragment order of 3x3x3 weights Wt [27][Cout][Cin] for k_conv3:
+// Wm[q][g][dz][j][lane][e] = Wt[9 dz + qi][16 j + (lane & 15)][q CC + c] with
+// k = 8 (lane >> 4) + e, qi = g NQ + k / CC, c = k % CC (0 where qi >= 9).
+template <int CC>
+__global__ void k_frag(const uint16_t* Wt, int Cout, int Cin, uint16_t* Wm, long n) {
+  constexpr int NQ = Frag<CC>::NQ, NG = Frag<CC>::NG;
+  const int CT = Cout / 16;
+  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
+    const int e = (int)(i % 8), lane = (int)((i / 8) % 64);
+    long r = i / 512;
+    const int j = (int)(r % CT);
+    r /= CT;
+    const int dz = (int)(r % 3);
+    r /= 3;
+    const int g = (int)(r % NG);
+    const int q = (int)(r / NG);
+    const int k = 8 * (lane >> 4) + e;
+    const int qi = g * NQ + k / CC, c = q * CC + k % CC;
+    Wm[i] = qi < 9 ? Wt[((long)(9 * dz + qi) * Cout + 16 * j + (lane & 15)) * Cin + c] : (uint16_t)0;
+  }
+}
+
+// dW[t][co][ci] += sum over a chunk of voxels of dY[v][co] X[v + d_t][ci]
+// (X chunk-planar when xq != 0: channel ci of voxel v at
+// (ci / xs) xq + v xs + ci % xs).
 // grid: x = voxel chunks (VCH voxels each), y = taps * (Cout/16) * (Cin/16).
 constexpr int VCH = 1024;
-__global__ __launch_bounds__(64) void k_conv_dw(const uint16_t* __restrict__ X, int xs, int xo, int Cin,
+__global__ __launch_bounds__(64) void k_conv_dw(const uint16_t* __restrict__ X, int xs, int xo, long xq, int Cin,
                                                 const uint16_t* __restrict__ dY, int Cout, Dims d, int taps,
                                                 float* dW) {
   const int lane = threadIdx.x;
@@ -250,7 +501,7 @@ __global__ __launch_bounds__(64) void k_conv_dw(const uint16_t* __restrict__ X, 
         const int x = (int)(v % d.W), y = (int)((v / d.W) % d.H), z = (int)(v / ((long)d.W * d.H));
         const int xx = x + dx, yy = y + dy, zz = z + dz;
         if (xx >= 0 && xx < d.W && yy >= 0 && yy < d.H && zz >= 0 && zz < d.D)
-          bv = (short)X[(((long)zz * d.H + yy) * d.W + xx) * xs + xo + ci];
+          bv = (short)X[(xq ? (ci / xs) * xq + ci % xs : ci) + (((long)zz * d.H + yy) * d.W + xx) * xs + xo];
       }
       a[i] = av;
       b[i] = bv;
@@ -298,21 +549,35 @@ __global__ void k_relu_bwd(const float* dpost, const uint16_t* post, int ps, int
 }
 
 // 2x2x2 average pool, channels-last (an average integrates the signal of a
-// low-SNR tomogram; a maximum would pick up noise)
-__global__ void k_pool(const uint16_t* X, int xs, int xo, int C, Dims d, uint16_t* Y) {
-  const Dims o{d.D / 2, d.H / 2, d.W / 2};
-  const long n = o.V() * C;
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
-    const int c = (int)(i % C);
-    const long v = i / C;
-    const int x = (int)(v % o.W), y = (int)((v / o.W) % o.H), z = (int)(v / ((long)o.W * o.H));
-    float sum = 0.f;
-    for (int k = 0; k < 8; ++k) {
-      const long s = (((long)(2 * z + (k >> 2)) * d.H + (2 * y + ((k >> 1) & 1))) * d.W + (2 * x + (k & 1)));
-      sum += bf2f(X[s * xs + xo + c]);
+// low-SNR tomogram; a maximum would pick up noise). grid: x = ceil(w h C / 8
+// / NTU) at the low resolution, y = its depth; thread = 8 channels of one
+// output voxel (16-byte loads and store).
+__global__ __launch_bounds__(NTU) void k_pool(const uint16_t* __restrict__ X, int xs, int xo, int C, Dims d,
+                                              uint16_t* Y) {
+  const int ow = d.W / 2, oh = d.H / 2;
+  const int C8 = C / 8;
+  const int r = blockIdx.x * NTU + threadIdx.x;
+  if (r >= ow * oh * C8) return;
+  const int c = (r % C8) * 8, p = r / C8;
+  const int x = p % ow, y = p / ow, z = blockIdx.y;
+  float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const long src = (((long)(2 * z + (k >> 2)) * d.H + (2 * y + ((k >> 1) & 1))) * d.W + (2 * x + (k & 1)));
+    const uint4 q = *reinterpret_cast<const uint4*>(X + src * xs + xo + c);
+    const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sum[2 * e] += bf2f((uint16_t)(u[e] & 0xffffu));
+      sum[2 * e + 1] += bf2f((uint16_t)(u[e] >> 16));
     }
-    Y[v * C + c] = f2bf(sum * 0.125f);
   }
+  uint4 o;
+  o.x = (uint32_t)f2bf(sum[0] * 0.125f) | ((uint32_t)f2bf(sum[1] * 0.125f) << 16);
+  o.y = (uint32_t)f2bf(sum[2] * 0.125f) | ((uint32_t)f2bf(sum[3] * 0.125f) << 16);
+  o.z = (uint32_t)f2bf(sum[4] * 0.125f) | ((uint32_t)f2bf(sum[5] * 0.125f) << 16);
+  o.w = (uint32_t)f2bf(sum[6] * 0.125f) | ((uint32_t)f2bf(sum[7] * 0.125f) << 16);
+  *reinterpret_cast<uint4*>(Y + (((long)z * oh + y) * ow + x) * C + c) = o;
 }
 
 // pool backward: dX (f32, [Vin][C], accumulated) += dY / 8 on each of the 8
@@ -351,39 +616,60 @@ __device__ __forceinline__ void up_taps(int o, int n, int* i0, int* i1, float* w
 // One thread per output voxel and 8 channels: the taps and weights are
 // computed once for the 8 channels, which move as 16-byte vectors (C, ys and
 // yo multiples of 8).
-__global__ void k_up(const uint16_t* X, int C, Dims lo, uint16_t* Y, int ys, int yo) {
-  const Dims hi{lo.D * 2, lo.H * 2, lo.W * 2};
-  const int C8 = C / 8;
-  const long n = hi.V() * C8;
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
-    const int c = (int)(i % C8) * 8;
-    const long v = i / C8;
-    const int x = (int)(v % hi.W), y = (int)((v / hi.W) % hi.H), z = (int)(v / ((long)hi.W * hi.H));
+// Trilinear x2 upsampling through LDS: a block stages the low-resolution
+// tile of UX x UY x LZ voxels plus one voxel on each side (edge-clamped, which
+// is the clamping of up_taps) and writes the 2UX x 2UY x 2LZ outputs above
+// it; thread = 8 channels of one output voxel, channel groups fastest.
+// Output chunk-planar: channel c of voxel v at (c / oc) oq + v oc + c % oc.
+constexpr int UX = 8, UY = 8;
+template <int C, int LZ>
+__global__ __launch_bounds__(NTU) void k_up(const uint16_t* __restrict__ X, Dims lo, uint16_t* Y, int oc, long oq,
+                                            int nbx, int nby) {
+  constexpr int C8 = C / 8, SX = UX + 2, SY = UY + 2, SZ = LZ + 2;
+  __shared__ uint4 t[SX * SY * SZ * C8];
+  const int X0 = (blockIdx.x % nbx) * UX, Y0 = ((blockIdx.x / nbx) % nby) * UY, Z0 = (blockIdx.x / (nbx * nby)) * LZ;
+  for (int i = threadIdx.x; i < SX * SY * SZ * C8; i += NTU) {
+    const int c8 = i % C8, v = i / C8;
+    const int gx = min(max(X0 + v % SX - 1, 0), lo.W - 1), gy = min(max(Y0 + (v / SX) % SY - 1, 0), lo.H - 1),
+              gz = min(max(Z0 + v / (SX * SY) - 1, 0), lo.D - 1);
+    t[i] = *reinterpret_cast<const uint4*>(X + (((long)gz * lo.H + gy) * lo.W + gx) * C + c8 * 8);
+  }
+  __syncthreads();
+  const int HW = 2 * lo.W, HH = 2 * lo.H, HD = 2 * lo.D;
+  for (int i = threadIdx.x; i < 8 * UX * UY * LZ * C8; i += NTU) {
+    const int c8 = i % C8, v = i / C8;
+    const int ox = v % (2 * UX), oy = (v / (2 * UX)) % (2 * UY), oz = v / (4 * UX * UY);
+    const int x = 2 * X0 + ox, y = 2 * Y0 + oy, z = 2 * Z0 + oz;
+    if (x >= HW || y >= HH || z >= HD) continue;
     int xi[2], yi[2], zi[2];
     float xw[2], yw[2], zw[2];
     up_taps(x, lo.W, &xi[0], &xi[1], &xw[0], &xw[1]);
     up_taps(y, lo.H, &yi[0], &yi[1], &yw[0], &yw[1]);
     up_taps(z, lo.D, &zi[0], &zi[1], &zw[0], &zw[1]);
-    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
       for (int b = 0; b < 2; ++b)
+#pragma unroll
         for (int e = 0; e < 2; ++e) {
           const float w = zw[a] * yw[b] * xw[e];
-          const uint4 q =
-              *reinterpret_cast<const uint4*>(X + (((long)zi[a] * lo.H + yi[b]) * lo.W + xi[e]) * C + c);
+          const int lv = ((zi[a] - Z0 + 1) * SY + (yi[b] - Y0 + 1)) * SX + (xi[e] - X0 + 1);
+          const uint4 q = t[lv * C8 + c8];
           const uint32_t u[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            s[2 * k] += w * bf2f((uint16_t)(u[k] & 0xffffu));
-            s[2 * k + 1] += w * bf2f((uint16_t)(u[k] >> 16));
+            sum[2 * k] += w * bf2f((uint16_t)(u[k] & 0xffffu));
+            sum[2 * k + 1] += w * bf2f((uint16_t)(u[k] >> 16));
           }
         }
     uint4 o;
-    o.x = (uint32_t)f2bf(s[0]) | ((uint32_t)f2bf(s[1]) << 16);
-    o.y = (uint32_t)f2bf(s[2]) | ((uint32_t)f2bf(s[3]) << 16);
-    o.z = (uint32_t)f2bf(s[4]) | ((uint32_t)f2bf(s[5]) << 16);
-    o.w = (uint32_t)f2bf(s[6]) | ((uint32_t)f2bf(s[7]) << 16);
-    *reinterpret_cast<uint4*>(Y + v * ys + yo + c) = o;
+    o.x = (uint32_t)f2bf(sum[0]) | ((uint32_t)f2bf(sum[1]) << 16);
+    o.y = (uint32_t)f2bf(sum[2]) | ((uint32_t)f2bf(sum[3]) << 16);
+    o.z = (uint32_t)f2bf(sum[4]) | ((uint32_t)f2bf(sum[5]) << 16);
+    o.w = (uint32_t)f2bf(sum[6]) | ((uint32_t)f2bf(sum[7]) << 16);
+    const int c = c8 * 8;
+    *reinterpret_cast<uint4*>(Y + (c / oc) * oq + (((long)z * HH + y) * HW + x) * oc + c % oc) = o;
   }
 }
 
@@ -561,58 +847,122 @@ __global__ __launch_bounds__(NTU) void k_predict(const float* logits, int CH, in
   }
 }
 
-// Inference head fused with the prediction: the 1x1x1 head (16 -> 1 + nbin
-// + 3 channels, bf16 weights, FP32 sums over the 16 inputs in order) at one
-// voxel per thread, writing only prob (or the logit), the first argmax bin
-// and the offsets instead of the FP32 logits of every head channel.
-__global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
-                                                      const float* __restrict__ bias, int nbin, long V, float* prob,
-                                                      int32_t* bin, float* offs, int raw) {
-  __shared__ float w[80 * 16];
-  __shared__ float bb[80];
+// Inference head fused with the prediction: the 1x1x1 head (16 -> 80
+// channels: 1 + nbin + 3 used) on v_mfma_f32_16x16x16_bf16 as
+// D[channel][voxel] (the same products and K order as the head of forward()),
+// then per voxel the particle logit (or its sigmoid), the first argmax of the
+// bin logits and the offsets, written straight into the output volume: the
+// tile voxel (a, b, c) of the network-size tile at (ox, oy, oz) goes to
+// (ox + a, oy + b, oz + c) when a, b, c lie in [halo, halo + core) and inside
+// X x Y x Z. One wave takes 16 consecutive tile voxels per step.
+struct OutWin {
+  int halo, cx, cy, cz, ox, oy, oz, X, Y, Z;
+};
+
+__global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict__ Xb, Dims t,
+                                                      const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                                      int nbin, OutWin ow, float* prob, int32_t* bin, float* offs,
+                                                      int raw) {
+  const int lane = threadIdx.x & 63, n = lane & 15, kg = lane >> 4;
   const int nc = 1 + nbin + 3;
-  for (int i = threadIdx.x; i < nc * 16; i += NTU) w[i] = bf2f(Wt[i]);
-  for (int i = threadIdx.x; i < nc; i += NTU) bb[i] = bias[i];
-  __syncthreads();
-  for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
-    const uint4* src = reinterpret_cast<const uint4*>(X + v * 16);
-    const uint4 q0 = src[0], q1 = src[1];
-    const uint32_t u[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-    float x[16];
+  bf16x4 wa[5];
+  float bb[5][4];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      x[2 * k] = bf2f((uint16_t)(u[k] & 0xffffu));
-      x[2 * k + 1] = bf2f((uint16_t)(u[k] >> 16));
+  for (int j = 0; j < 5; ++j) {
+    const uint2 rw = *reinterpret_cast<const uint2*>(Wt + (16 * j + n) * 16 + 4 * kg);
+    wa[j][0] = (short)(rw.x & 0xffff);
+    wa[j][1] = (short)(rw.x >> 16);
+    wa[j][2] = (short)(rw.y & 0xffff);
+    wa[j][3] = (short)(rw.y >> 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = 16 * j + 4 * kg + i;
+      bb[j][i] = c < nc ? bias[c] : 0.f;
     }
-    auto dot = [&](int c) {
-      float a = 0.f;
+  }
+  const int ngrp = (int)(t.V() / 16);  // V is a multiple of 64 and below 2^31
+  const int wid = (blockIdx.x * NTU + threadIdx.x) >> 6, nw = (gridDim.x * NTU) >> 6;
+  const int tHW = t.W * t.H;
+  // four groups of 16 voxels per step, their loads issued together
+  for (int g0 = 4 * wid; g0 < ngrp; g0 += 4 * nw) {
+    uint2 rxs[4];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) a = fmaf(x[k], w[c * 16 + k], a);
-      return a + bb[c];
-    };
-    const float z = dot(0);
-    if (prob) prob[v] = raw ? z : 1.f / (1.f + expf(-z));
-    if (bin) {
-      float best = dot(1);
-      int bi = 0;
-      for (int c = 1; c < nbin; ++c) {
-        const float l = dot(1 + c);
-        if (l > best) {
-          best = l;
-          bi = c;
+    for (int u = 0; u < 4; ++u) rxs[u] = *reinterpret_cast<const uint2*>(Xb + ((long)(g0 + u) * 16 + n) * 16 + 4 * kg);
+#pragma unroll 1
+  for (int u = 0; u < 4; ++u) {
+    const int v = (g0 + u) * 16 + n;
+    const uint2 rx = rxs[u];
+    bf16x4 xb;
+    xb[0] = (short)(rx.x & 0xffff);
+    xb[1] = (short)(rx.x >> 16);
+    xb[2] = (short)(rx.y & 0xffff);
+    xb[3] = (short)(rx.y >> 16);
+    f32x4 acc[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[j], xb, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    // this lane's bin candidates: first index of its largest non-NaN logit
+    float best = 0.f;
+    int bi = INT_MAX;
+    bool first_nan = false;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = 16 * j + 4 * kg + i;
+        const float l = acc[j][i] + bb[j][i];
+        if (c >= 1 && c <= nbin) {
+          if (c == 1 && l != l) first_nan = true;
+          if (l == l && (bi == INT_MAX || l > best)) {
+            best = l;
+            bi = c - 1;
+          }
         }
       }
-      bin[v] = bi;
+    for (int o = 16; o < 64; o <<= 1) {
+      const float ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      first_nan = first_nan || __shfl_xor((int)first_nan, o, 64);
+      if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) {
+        best = ob;
+        bi = oi;
+      }
     }
-    if (offs)
-      for (int c = 0; c < 3; ++c) offs[v * 3 + c] = dot(1 + nbin + c);
+    const int c = v / tHW, rr = v - c * tHW, b = rr / t.W, a = rr - b * t.W;
+    const int x = ow.ox + a, y = ow.oy + b, z = ow.oz + c;
+    if (a < ow.halo || a >= ow.halo + ow.cx || b < ow.halo || b >= ow.halo + ow.cy || c < ow.halo ||
+        c >= ow.halo + ow.cz || x >= ow.X || y >= ow.Y || z >= ow.Z)
+      continue;
+    const long dst = ((long)z * ow.Y + y) * ow.X + x;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ch = 16 * j + 4 * kg + i;
+        const float l = acc[j][i] + bb[j][i];
+        if (ch == 0 && prob) prob[dst] = raw ? l : 1.f / (1.f + expf(-l));
+        if (offs && ch > nbin && ch < nc) offs[dst * 3 + (ch - nbin - 1)] = l;
+      }
+    if (bin && kg == 0) bin[dst] = (first_nan || bi == INT_MAX) ? 0 : bi;
+  }
   }
 }
 
-// sums of a volume (for the global normalisation of the input)
-__global__ void k_vsum(const float* a, long n, double* out) {
+// sums and sums of squares of a volume (for the global normalisation of the
+// input), in two deterministic stages: FP64 partial sums per block, then one
+// block adds the partials in block order.
+constexpr int VSB = 1024;
+__global__ __launch_bounds__(NTU) void k_vsum(const float* __restrict__ a, long n, double* part) {
+  __shared__ double red[2][NTU / 64];
   double s = 0, q = 0;
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
+  // 16-byte loads when the volume is 16-byte aligned, else 4-byte loads
+  const long n4 = (reinterpret_cast<uintptr_t>(a) & 15) ? 0 : n / 4;
+  const float4* a4 = reinterpret_cast<const float4*>(a);
+  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < n4; i += (long)gridDim.x * NTU) {
+    const float4 v = a4[i];
+    s += (double)v.x + (double)v.y + (double)v.z + (double)v.w;
+    q += (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w;
+  }
+  for (long i = n4 * 4 + (long)blockIdx.x * NTU + threadIdx.x; i < n; i += (long)gridDim.x * NTU) {
     const double v = a[i];
     s += v;
     q += v * v;
@@ -622,41 +972,34 @@ __global__ void k_vsum(const float* a, long n, double* out) {
     q += __shfl_xor(q, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&out[0], s);
-    atomicAdd(&out[1], q);
+    red[0][threadIdx.x >> 6] = s;
+    red[1][threadIdx.x >> 6] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S = 0, Q = 0;
+    for (int w = 0; w < NTU / 64; ++w) {
+      S += red[0][w];
+      Q += red[1][w];
+    }
+    part[2 * blockIdx.x] = S;
+    part[2 * blockIdx.x + 1] = Q;
   }
 }
 
-// tile input: (vol - mean) * inv_std over [o, o + T) (zero outside the volume)
-__global__ void k_tile_in(const float* vol, int X, int Y, int Z, int ox, int oy, int oz, Dims t,
-                          const double* sums, long n, float* out) {
-  const double mean = sums[0] / (double)n;
-  const double var = sums[1] / (double)n - mean * mean;
-  const float m = (float)mean, is = var > 0 ? (float)(1.0 / sqrt(var)) : 1.f;
-  const long nt = t.V();
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < nt; i += (long)gridDim.x * NTU) {
-    const int x = (int)(i % t.W) + ox, y = (int)((i / t.W) % t.H) + oy, z = (int)(i / ((long)t.W * t.H)) + oz;
-    float v = 0.f;
-    if (x >= 0 && x < X && y >= 0 && y < Y && z >= 0 && z < Z) v = (vol[((long)z * Y + y) * X + x] - m) * is;
-    out[i] = v;
+__global__ __launch_bounds__(64) void k_vsum_fin(const double* part, int nb, double* out) {
+  double s = 0, q = 0;
+  for (int i = threadIdx.x; i < nb; i += 64) {
+    s += part[2 * i];
+    q += part[2 * i + 1];
   }
-}
-
-// write the tile core [halo, halo + core) clipped to the volume
-__global__ void k_tile_out(const float* tp, const int32_t* tb, const float* to, Dims t, int halo, int cx, int cy,
-                           int cz, int ox, int oy, int oz, int X, int Y, int Z, float* prob, int32_t* bin,
-                           float* offs) {
-  const long nc = (long)cx * cy * cz;
-  for (long i = (long)blockIdx.x * NTU + threadIdx.x; i < nc; i += (long)gridDim.x * NTU) {
-    const int a = (int)(i % cx), b = (int)((i / cx) % cy), c = (int)(i / ((long)cx * cy));
-    const int x = ox + halo + a, y = oy + halo + b, z = oz + halo + c;
-    if (x >= X || y >= Y || z >= Z) continue;
-    const long src = ((long)(halo + c) * t.H + (halo + b)) * t.W + (halo + a);
-    const long dst = ((long)z * Y + y) * X + x;
-    if (prob) prob[dst] = tp[src];
-    if (bin) bin[dst] = tb[src];
-    if (offs)
-      for (int c = 0; c < 3; ++c) offs[dst * 3 + c] = to[src * 3 + c];
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, 64);
+    q += __shfl_xor(q, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = q;
   }
 }
 
@@ -669,13 +1012,68 @@ inline unsigned gridv(long n) {
 }  // namespace
 
 // ============================================================== host side
-void conv_forward(const uint16_t* X, int xs, int xo, int Cin, int D, int H, int W, const uint16_t* Wt,
-                  const float* bias, int Cout, int taps, int relu, uint16_t* Yb, int ys, int yo, float* Yf,
-                  hipStream_t s) {
+static int conv3_chunk(int Cin) { return Cin % 32 == 0 ? 32 : 16; }
+
+long conv_fragment_elems(int Cout, int Cin) {
+  const int cc = conv3_chunk(Cin);
+  const int ng = cc == 32 ? Frag<32>::NG : Frag<16>::NG;
+  return (long)(Cin / cc) * ng * 3 * (Cout / 16) * 512;
+}
+
+void conv_fragments(const uint16_t* Wt, int Cout, int Cin, uint16_t* Wm, hipStream_t s) {
+  const long n = conv_fragment_elems(Cout, Cin);
+  if (conv3_chunk(Cin) == 32)
+    hipLaunchKernelGGL(k_frag<32>, dim3(gridv(n)), dim3(NTU), 0, s, Wt, Cout, Cin, Wm, n);
+  else
+    hipLaunchKernelGGL(k_frag<16>, dim3(gridv(n)), dim3(NTU), 0, s, Wt, Cout, Cin, Wm, n);
+  TOMHIP_CHECK(hipGetLastError());
+}
+
+template <int CT, int CC, int TZ>
+static void launch_conv3(const uint16_t* X, int xs, int xo, long xq, int Cin, Dims d, const uint16_t* Wm,
+                         const float* bias, int relu, uint16_t* Yb, int ys, int yo, float* Yf, hipStream_t s) {
+  const int nbx = (d.W + CTX - 1) / CTX, nby = (d.H + CTY - 1) / CTY, nbz = (d.D + TZ - 1) / TZ;
+  const size_t lds = sizeof(uint4) * (CTX + 2) * (CTY + 2) * (TZ + 2) * (CC / 8 + 1);
+  hipLaunchKernelGGL((k_conv3<CT, CC, TZ>), dim3((unsigned)(nbx * nby * nbz)), dim3(NTU), lds, s, X, xs, xo, xq ? xq : CC, Cin, d,
+                     reinterpret_cast<const uint4*>(Wm), bias, relu, Yb, ys, yo, Yf, nbx, nby);
+}
+
+void conv_forward(const uint16_t* X, int xs, int xo, long xq, int Cin, int D, int H, int W, const uint16_t* Wt,
+                  const uint16_t* Wm, const float* bias, int Cout, int taps, int relu, uint16_t* Yb, int ys, int yo,
+                  float* Yf, hipStream_t s) {
   TOMHIP_REQUIRE(Cin % 16 == 0 && Cout % 16 == 0 && Cout >= 16 && Cout <= 96 && (taps == 27 || taps == 1),
                  ERR_UNSUPPORTED, "conv: channels must be multiples of 16 (Cout <= 96), taps 27 or 1");
   TOMHIP_REQUIRE(xo % 4 == 0 && xs % 4 == 0, ERR_UNSUPPORTED, "conv: input channel offset/stride must be multiples of 4");
   Dims d{D, H, W};
+  if (taps == 27) {
+    TOMHIP_REQUIRE(Wm && xo % 8 == 0 && xs % 8 == 0 && ys % 4 == 0 && yo % 4 == 0, ERR_UNSUPPORTED,
+                   "conv: 3x3x3 needs fragment-ordered weights and 8-aligned input / 4-aligned output channels");
+    TOMHIP_REQUIRE(xq == 0 || (xs == conv3_chunk(Cin) && xo == 0 && xq % 8 == 0), ERR_UNSUPPORTED,
+                   "conv: a chunk-planar input has records of one chunk");
+    const int ct = Cout / 16;
+    if (conv3_chunk(Cin) == 16) {
+      switch (ct) {
+        case 1: launch_conv3<1, 16, 8>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 2: launch_conv3<2, 16, 8>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 3: launch_conv3<3, 16, 4>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 4: launch_conv3<4, 16, 4>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 5: launch_conv3<5, 16, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        default: launch_conv3<6, 16, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+      }
+    } else {
+      switch (ct) {
+        case 1: launch_conv3<1, 32, 4>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 2: launch_conv3<2, 32, 4>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 3: launch_conv3<3, 32, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 4: launch_conv3<4, 32, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        case 5: launch_conv3<5, 32, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+        default: launch_conv3<6, 32, 2>(X, xs, xo, xq, Cin, d, Wm, bias, relu, Yb, ys, yo, Yf, s); break;
+      }
+    }
+    TOMHIP_CHECK(hipGetLastError());
+    return;
+  }
+  TOMHIP_REQUIRE(xq == 0, ERR_UNSUPPORTED, "conv: 1x1x1 needs an interleaved input");
   // LDS tile of (CTZ + 2) planes; 4 planes per wave when the tile fits in
   // 64 KB, else 2
   auto lds_for = [&](int tz) { return sizeof(uint16_t) * (CTX + 2) * (CTY + 2) * (tz + 2) * (size_t)(Cin + 4); };
@@ -723,12 +1121,12 @@ void conv_forward(const uint16_t* X, int xs, int xo, int Cin, int D, int H, int 
   TOMHIP_CHECK(hipGetLastError());
 }
 
-void conv_weight_grad(const uint16_t* X, int xs, int xo, int Cin, const uint16_t* dY, int Cout, int D, int H, int W,
-                      int taps, float* dW, hipStream_t s) {
+void conv_weight_grad(const uint16_t* X, int xs, int xo, long xq, int Cin, const uint16_t* dY, int Cout, int D, int H,
+                      int W, int taps, float* dW, hipStream_t s) {
   Dims d{D, H, W};
   const long V = (long)D * H * W;
   dim3 g((unsigned)((V + VCH - 1) / VCH), (unsigned)(taps * (Cout / 16) * (Cin / 16)));
-  hipLaunchKernelGGL(k_conv_dw, g, dim3(64), 0, s, X, xs, xo, Cin, dY, Cout, d, taps, dW);
+  hipLaunchKernelGGL(k_conv_dw, g, dim3(64), 0, s, X, xs, xo, xq, Cin, dY, Cout, d, taps, dW);
   TOMHIP_CHECK(hipGetLastError());
 }
 
@@ -742,6 +1140,8 @@ struct ConvLayer {
   float* v;
   uint16_t* wb;   // bf16 W
   uint16_t* wf;   // bf16 flipped W' for the data gradient
+  uint16_t* wm;   // 3x3x3: W and W' in MFMA fragment order (conv_fragments)
+  uint16_t* wfm;
 };
 
 struct Unet::Impl {
@@ -759,12 +1159,8 @@ struct Unet::Impl {
   double* loss;
   bool train = false;
   int step = 0;
-  // tiled inference buffers
-  float* tile_in = nullptr;
-  float* tile_aux = nullptr;
-  float* tile_offs = nullptr;
-  float* tile_prob = nullptr;
-  int32_t* tile_bin = nullptr;
+  // whole-volume inference: per-block partial sums and the input statistics
+  double* part = nullptr;
   double* sums = nullptr;
   std::vector<void*> owned;
 
@@ -790,6 +1186,7 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   I.W = W;
   I.nbin = nbin;
   I.V0 = (long)D * H * W;
+  TOMHIP_REQUIRE(I.V0 < (1L << 31), ERR_UNSUPPORTED, "U-Net input above 2^31 voxels");
   I.V1 = I.V0 / 8;
   I.V2 = I.V1 / 8;
   // e1a e1b e2a e2b b1 b2 d2a d2b d1a d1b head
@@ -811,6 +1208,11 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
     c.v = I.alloc<float>(c.np);
     c.wb = I.alloc<uint16_t>(nw);
     c.wf = I.alloc<uint16_t>(nw);
+    c.wm = c.wfm = nullptr;
+    if (c.taps == 27) {
+      c.wm = I.alloc<uint16_t>(conv_fragment_elems(c.Cout, c.Cin));
+      c.wfm = I.alloc<uint16_t>(conv_fragment_elems(c.Cin, c.Cout));
+    }
     // He initialisation over the real fan-in (the first layer sees 1 channel)
     const int fan_in = c.taps * (I.L.empty() ? 2 : c.Cin);
     std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan_in));
@@ -828,7 +1230,7 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
     I.L.push_back(c);
   }
   const long V0 = I.V0, V1 = I.V1, V2 = I.V2;
-  I.x0 = I.alloc<uint16_t>(V0 * 16);
+  I.x0 = train ? I.alloc<uint16_t>(V0 * 16) : nullptr;  // first-layer weight gradient only
   I.a1 = I.alloc<uint16_t>(V0 * 16);
   I.c1 = I.alloc<uint16_t>(V0 * 48);
   I.p1 = I.alloc<uint16_t>(V1 * 16);
@@ -885,6 +1287,10 @@ void Unet::refresh_weights() {
     const long nw = (long)c.taps * c.Cout * c.Cin;
     hipLaunchKernelGGL(k_to_bf16, dim3(gridv(nw)), dim3(NTU), 0, I.s, c.p, c.wb, nw);
     hipLaunchKernelGGL(k_flip, dim3(gridv(nw)), dim3(NTU), 0, I.s, c.p, c.taps, c.Cout, c.Cin, c.wf);
+    if (c.taps == 27) {
+      conv_fragments(c.wb, c.Cout, c.Cin, c.wm, I.s);
+      conv_fragments(c.wf, c.Cin, c.Cout, c.wfm, I.s);
+    }
   }
   TOMHIP_CHECK(hipGetLastError());
 }
@@ -936,41 +1342,71 @@ void Unet::optimizer(float* m, float* v, int64_t* step, bool get) {
 
 void Unet::forward(const float* vol, const float* aux, bool head) {
   Impl& I = *d_;
+  // the padded 16-channel input is kept for the first layer's weight
+  // gradient only; the first layer itself reads the two f32 channels
+  if (I.train) hipLaunchKernelGGL(k_input, dim3(gridv(I.V0 * 16)), dim3(NTU), 0, I.s, vol, aux, I.V0, I.x0);
+  forward_window(vol, aux, I.W, I.H, I.D, 0, 0, 0, nullptr, 0, head);
+}
+
+void Unet::forward_window(const float* vol, const float* aux, int X, int Y, int Z, int ox, int oy, int oz,
+                          const double* sums, long n, bool head) {
+  Impl& I = *d_;
   hipStream_t s = I.s;
   const int D = I.D, H = I.H, W = I.W;
   auto& L = I.L;
-  hipLaunchKernelGGL(k_input, dim3(gridv(I.V0 * 16)), dim3(NTU), 0, s, vol, aux, I.V0, I.x0);
-  auto conv = [&](int li, const uint16_t* X, int xs, int xo, int d, int h, int w, uint16_t* Yb, int ys, int yo,
-                  float* Yf) {
+  auto conv = [&](int li, const uint16_t* X, int xs, int xo, long xq, int d, int h, int w, uint16_t* Yb, int ys,
+                  int yo, float* Yf) {
     ConvLayer& c = L[li];
-    conv_forward(X, xs, xo, c.Cin, d, h, w, c.wb, c.p + (long)c.taps * c.Cout * c.Cin, c.Cout, c.taps, c.relu, Yb,
-                 ys, yo, Yf, s);
+    conv_forward(X, xs, xo, xq, c.Cin, d, h, w, c.wb, c.wm, c.p + (long)c.taps * c.Cout * c.Cin, c.Cout, c.taps,
+                 c.relu, Yb, ys, yo, Yf, s);
   };
-  conv(0, I.x0, 16, 0, D, H, W, I.a1, 16, 0, nullptr);
-  conv(1, I.a1, 16, 0, D, H, W, I.c1, 48, 32, nullptr);  // skip1 -> c1[32:48]
+  {
+    ConvLayer& c = L[0];
+    constexpr int TZ = 8;
+    const int nbx = (W + CTX - 1) / CTX, nby = (H + CTY - 1) / CTY, nbz = (D + TZ - 1) / TZ;
+    const InWin in{vol, aux, X, Y, Z, ox, oy, oz, sums, n};
+    hipLaunchKernelGGL(k_conv_in<TZ>, dim3((unsigned)(nbx * nby * nbz)), dim3(NTU), 0, s, in, Dims{D, H, W},
+                       (const uint16_t*)c.wb, (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.a1, nbx, nby);
+  }
+  // c1 = [up(d2b) | skip1] and c2 = [up(b2) | skip2] are chunk-planar (planes
+  // of 16 and 32 channels): every kernel reads and writes whole records
+  const long q1 = I.V0 * 16, q2 = I.V1 * 32;
+  uint16_t* skip1 = I.c1 + 2 * q1;
+  uint16_t* skip2 = I.c2 + 2 * q2;
+  conv(1, I.a1, 16, 0, 0, D, H, W, skip1, 16, 0, nullptr);
   Dims d0{D, H, W}, d1{D / 2, H / 2, W / 2}, d2{D / 4, H / 4, W / 4};
-  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, I.c1, 48, 32, 16, d0, I.p1);
-  conv(2, I.p1, 16, 0, d1.D, d1.H, d1.W, I.a2, 32, 0, nullptr);
-  conv(3, I.a2, 32, 0, d1.D, d1.H, d1.W, I.c2, 96, 64, nullptr);  // skip2 -> c2[64:96]
-  hipLaunchKernelGGL(k_pool, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, I.c2, 96, 64, 32, d1, I.p2);
-  conv(4, I.p2, 32, 0, d2.D, d2.H, d2.W, I.b1, 64, 0, nullptr);
-  conv(5, I.b1, 64, 0, d2.D, d2.H, d2.W, I.b2, 64, 0, nullptr);
-  hipLaunchKernelGGL(k_up, dim3(gridv(I.V1 * 8)), dim3(NTU), 0, s, I.b2, 64, d2, I.c2, 96, 0);
-  conv(6, I.c2, 96, 0, d1.D, d1.H, d1.W, I.d2a, 32, 0, nullptr);
-  conv(7, I.d2a, 32, 0, d1.D, d1.H, d1.W, I.d2b, 32, 0, nullptr);
-  hipLaunchKernelGGL(k_up, dim3(gridv(I.V0 * 4)), dim3(NTU), 0, s, I.d2b, 32, d1, I.c1, 48, 0);
-  conv(8, I.c1, 48, 0, D, H, W, I.d1a, 16, 0, nullptr);
-  conv(9, I.d1a, 16, 0, D, H, W, I.d1b, 16, 0, nullptr);
-  if (head) conv(10, I.d1b, 16, 0, D, H, W, nullptr, 0, 0, I.logits);
+  auto plane_grid = [](const Dims& o, int C) {
+    return dim3((unsigned)(((long)o.W * o.H * (C / 8) + NTU - 1) / NTU), (unsigned)o.D);
+  };
+  hipLaunchKernelGGL(k_pool, plane_grid(d1, 16), dim3(NTU), 0, s, skip1, 16, 0, 16, d0, I.p1);
+  conv(2, I.p1, 16, 0, 0, d1.D, d1.H, d1.W, I.a2, 32, 0, nullptr);
+  conv(3, I.a2, 32, 0, 0, d1.D, d1.H, d1.W, skip2, 32, 0, nullptr);
+  hipLaunchKernelGGL(k_pool, plane_grid(d2, 32), dim3(NTU), 0, s, skip2, 32, 0, 32, d1, I.p2);
+  conv(4, I.p2, 32, 0, 0, d2.D, d2.H, d2.W, I.b1, 64, 0, nullptr);
+  conv(5, I.b1, 64, 0, 0, d2.D, d2.H, d2.W, I.b2, 64, 0, nullptr);
+  auto up_grid = [](const Dims& lo, int lz) {
+    return dim3((unsigned)(((lo.W + UX - 1) / UX) * ((lo.H + UY - 1) / UY) * ((lo.D + lz - 1) / lz)));
+  };
+  hipLaunchKernelGGL((k_up<64, 2>), up_grid(d2, 2), dim3(NTU), 0, s, I.b2, d2, I.c2, 32, q2, (d2.W + UX - 1) / UX,
+                     (d2.H + UY - 1) / UY);
+  conv(6, I.c2, 32, 0, q2, d1.D, d1.H, d1.W, I.d2a, 32, 0, nullptr);
+  conv(7, I.d2a, 32, 0, 0, d1.D, d1.H, d1.W, I.d2b, 32, 0, nullptr);
+  hipLaunchKernelGGL((k_up<32, 4>), up_grid(d1, 4), dim3(NTU), 0, s, I.d2b, d1, I.c1, 16, q1, (d1.W + UX - 1) / UX,
+                     (d1.H + UY - 1) / UY);
+  conv(8, I.c1, 16, 0, q1, D, H, W, I.d1a, 16, 0, nullptr);
+  conv(9, I.d1a, 16, 0, 0, D, H, W, I.d1b, 16, 0, nullptr);
+  if (head) conv(10, I.d1b, 16, 0, 0, D, H, W, nullptr, 0, 0, I.logits);
   TOMHIP_CHECK(hipGetLastError());
 }
 
-void Unet::head_predict(float* prob, int32_t* bin, bool logit, float* offs) {
+void Unet::head_window(int halo, int ox, int oy, int oz, int X, int Y, int Z, float* prob, int32_t* bin,
+                       bool logit, float* offs) {
   Impl& I = *d_;
   ConvLayer& c = I.L[10];
-  hipLaunchKernelGGL(k_head_predict, dim3(gridv(I.V0)), dim3(NTU), 0, I.s, (const uint16_t*)I.d1b,
-                     (const uint16_t*)c.wb, (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.nbin, I.V0, prob,
-                     bin, offs, logit ? 1 : 0);
+  const OutWin ow{halo, I.W - 2 * halo, I.H - 2 * halo, I.D - 2 * halo, ox, oy, oz, X, Y, Z};
+  hipLaunchKernelGGL(k_head_predict, dim3(gridv(I.V0 / 16)), dim3(NTU), 0, I.s, (const uint16_t*)I.d1b,
+                     Dims{I.D, I.H, I.W}, (const uint16_t*)c.wb,
+                     (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.nbin, ow, prob, bin, offs, logit ? 1 : 0);
   TOMHIP_CHECK(hipGetLastError());
 }
 
@@ -1014,40 +1450,44 @@ void Unet::backward(const float* tprob, const int32_t* tbin, const float* toff, 
   // one layer: gpost (f32 [V][Cout]) + its post-activation -> weight/bias
   // grads and (optionally) the data gradient into gin (f32, [V][Cin]) at
   // channel stride gs
+  const long q1 = I.V0 * 16, q2 = I.V1 * 32;
+  const uint16_t* skip1 = I.c1 + 2 * q1;
+  const uint16_t* skip2 = I.c2 + 2 * q2;
   auto layer_bwd = [&](int li, const float* gpost, const uint16_t* post, int ps, int po, const uint16_t* X, int xs,
-                       int xo, Dims d, float* gin) {
+                       int xo, long xq, Dims d, float* gin) {
     ConvLayer& c = L[li];
     const long V = d.V();
     dim3 g(gridv(V) > 1024 ? 1024 : gridv(V), c.Cout);
     hipLaunchKernelGGL(k_relu_bwd, g, dim3(NTU), 0, s, gpost, post, ps, po, c.Cout, V, c.relu, I.gb,
                        c.g + (long)c.taps * c.Cout * c.Cin);
-    conv_weight_grad(X, xs, xo, c.Cin, I.gb, c.Cout, d.D, d.H, d.W, c.taps, c.g, s);
-    if (gin) conv_forward(I.gb, c.Cout, 0, c.Cout, d.D, d.H, d.W, c.wf, nullptr, c.Cin, c.taps, 0, nullptr, 0, 0,
-                          gin, s);
+    conv_weight_grad(X, xs, xo, xq, c.Cin, I.gb, c.Cout, d.D, d.H, d.W, c.taps, c.g, s);
+    if (gin)
+      conv_forward(I.gb, c.Cout, 0, 0, c.Cout, d.D, d.H, d.W, c.wf, c.wfm, nullptr, c.Cin, c.taps, 0, nullptr, 0, 0,
+                   gin, s);
   };
   // head (no relu; post unused)
-  layer_bwd(10, I.g_logits, I.d1b, 16, 0, I.d1b, 16, 0, d0, I.g_d1b);
-  layer_bwd(9, I.g_d1b, I.d1b, 16, 0, I.d1a, 16, 0, d0, I.g_d1a);
-  layer_bwd(8, I.g_d1a, I.d1a, 16, 0, I.c1, 48, 0, d0, I.g_c1);
+  layer_bwd(10, I.g_logits, I.d1b, 16, 0, I.d1b, 16, 0, 0, d0, I.g_d1b);
+  layer_bwd(9, I.g_d1b, I.d1b, 16, 0, I.d1a, 16, 0, 0, d0, I.g_d1a);
+  layer_bwd(8, I.g_d1a, I.d1a, 16, 0, I.c1, 16, 0, q1, d0, I.g_c1);
   // c1 = [up(d2b) | skip1]
   hipLaunchKernelGGL(k_split_add, dim3(gridv(I.V0 * 16)), dim3(NTU), 0, s, (const float*)I.g_c1, 48, 32, 16, I.V0,
                      I.g_s1);
   hipLaunchKernelGGL(k_up_bwd, dim3(gridv(I.V1 * 32)), dim3(NTU), 0, s, (const float*)I.g_c1, 48, 0, 32, d1,
                      I.g_d2b);
-  layer_bwd(7, I.g_d2b, I.d2b, 32, 0, I.d2a, 32, 0, d1, I.g_d2a);
-  layer_bwd(6, I.g_d2a, I.d2a, 32, 0, I.c2, 96, 0, d1, I.g_c2);
+  layer_bwd(7, I.g_d2b, I.d2b, 32, 0, I.d2a, 32, 0, 0, d1, I.g_d2a);
+  layer_bwd(6, I.g_d2a, I.d2a, 32, 0, I.c2, 32, 0, q2, d1, I.g_c2);
   hipLaunchKernelGGL(k_split_add, dim3(gridv(I.V1 * 32)), dim3(NTU), 0, s, (const float*)I.g_c2, 96, 64, 32, I.V1,
                      I.g_s2);
   hipLaunchKernelGGL(k_up_bwd, dim3(gridv(I.V2 * 64)), dim3(NTU), 0, s, (const float*)I.g_c2, 96, 0, 64, d2,
                      I.g_b2);
-  layer_bwd(5, I.g_b2, I.b2, 64, 0, I.b1, 64, 0, d2, I.g_b1);
-  layer_bwd(4, I.g_b1, I.b1, 64, 0, I.p2, 32, 0, d2, I.g_p2);
+  layer_bwd(5, I.g_b2, I.b2, 64, 0, I.b1, 64, 0, 0, d2, I.g_b1);
+  layer_bwd(4, I.g_b1, I.b1, 64, 0, I.p2, 32, 0, 0, d2, I.g_p2);
   hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V2 * 32)), dim3(NTU), 0, s, (const float*)I.g_p2, 32, d1, I.g_s2);
-  layer_bwd(3, I.g_s2, I.c2, 96, 64, I.a2, 32, 0, d1, I.g_a2);
-  layer_bwd(2, I.g_a2, I.a2, 32, 0, I.p1, 16, 0, d1, I.g_p1);
+  layer_bwd(3, I.g_s2, skip2, 32, 0, I.a2, 32, 0, 0, d1, I.g_a2);
+  layer_bwd(2, I.g_a2, I.a2, 32, 0, I.p1, 16, 0, 0, d1, I.g_p1);
   hipLaunchKernelGGL(k_pool_bwd, dim3(gridv(I.V1 * 16)), dim3(NTU), 0, s, (const float*)I.g_p1, 16, d0, I.g_s1);
-  layer_bwd(1, I.g_s1, I.c1, 48, 32, I.a1, 16, 0, d0, I.g_a1);
-  layer_bwd(0, I.g_a1, I.a1, 16, 0, I.x0, 16, 0, d0, nullptr);
+  layer_bwd(1, I.g_s1, skip1, 16, 0, I.a1, 16, 0, 0, d0, I.g_a1);
+  layer_bwd(0, I.g_a1, I.a1, 16, 0, I.x0, 16, 0, 0, d0, nullptr);
   TOMHIP_CHECK(hipGetLastError());
   if (loss_out) {
     double h[3];
@@ -1079,32 +1519,25 @@ void Unet::infer(const float* vol, const float* aux, int X, int Y, int Z, int ha
   TOMHIP_REQUIRE(halo >= 0 && halo % 4 == 0, ERR_WRONG_INPUT, "halo must be a non-negative multiple of 4");
   const int cx = I.W - 2 * halo, cy = I.H - 2 * halo, cz = I.D - 2 * halo;
   TOMHIP_REQUIRE(cx > 0 && cy > 0 && cz > 0, ERR_WRONG_INPUT, "halo leaves no tile core");
-  if (!I.tile_in) {
-    I.tile_in = I.alloc<float>(I.V0);
-    I.tile_prob = I.alloc<float>(I.V0);
-    I.tile_bin = I.alloc<int32_t>(I.V0);
+  if (!I.sums) {
+    I.part = I.alloc<double>(2 * VSB);
     I.sums = I.alloc<double>(4);
   }
-  if (aux && !I.tile_aux) I.tile_aux = I.alloc<float>(I.V0);
-  if (offs && !I.tile_offs) I.tile_offs = I.alloc<float>(I.V0 * 3);
   const long n = (long)X * Y * Z;
-  TOMHIP_CHECK(hipMemsetAsync(I.sums, 0, sizeof(double) * 4, s));
-  hipLaunchKernelGGL(k_vsum, dim3(gridv(n)), dim3(NTU), 0, s, vol, n, I.sums);
-  if (aux) hipLaunchKernelGGL(k_vsum, dim3(gridv(n)), dim3(NTU), 0, s, aux, n, I.sums + 2);
-  const Dims t{I.D, I.H, I.W};
+  const unsigned nb = (unsigned)std::min<long>(VSB, std::max<long>(1, (n / 4 + NTU - 1) / NTU));
+  hipLaunchKernelGGL(k_vsum, dim3(nb), dim3(NTU), 0, s, vol, n, I.part);
+  hipLaunchKernelGGL(k_vsum_fin, dim3(1), dim3(64), 0, s, (const double*)I.part, (int)nb, I.sums);
+  if (aux) {
+    hipLaunchKernelGGL(k_vsum, dim3(nb), dim3(NTU), 0, s, aux, n, I.part);
+    hipLaunchKernelGGL(k_vsum_fin, dim3(1), dim3(64), 0, s, (const double*)I.part, (int)nb, I.sums + 2);
+  }
+  // tiles of the network size: the first layer reads (and normalises) its
+  // window of the volume, the head writes the tile core into the outputs
   for (int oz = -halo; oz + halo < Z; oz += cz)
     for (int oy = -halo; oy + halo < Y; oy += cy)
       for (int ox = -halo; ox + halo < X; ox += cx) {
-        hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, vol, X, Y, Z, ox, oy, oz, t,
-                           (const double*)I.sums, n, I.tile_in);
-        if (aux)
-          hipLaunchKernelGGL(k_tile_in, dim3(gridv(I.V0)), dim3(NTU), 0, s, aux, X, Y, Z, ox, oy, oz, t,
-                             (const double*)(I.sums + 2), n, I.tile_aux);
-        forward(I.tile_in, aux ? I.tile_aux : nullptr, false);
-        head_predict(I.tile_prob, I.tile_bin, logit, offs ? I.tile_offs : nullptr);
-        hipLaunchKernelGGL(k_tile_out, dim3(gridv((long)cx * cy * cz)), dim3(NTU), 0, s, (const float*)I.tile_prob,
-                           (const int32_t*)I.tile_bin, (const float*)I.tile_offs, t, halo, cx, cy, cz, ox, oy, oz,
-                           X, Y, Z, prob, bin, offs);
+        forward_window(vol, aux, X, Y, Z, ox, oy, oz, I.sums, n, false);
+        head_window(halo, ox, oy, oz, X, Y, Z, prob, bin, logit, offs);
       }
   TOMHIP_CHECK(hipGetLastError());
 }
