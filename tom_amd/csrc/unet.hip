@@ -267,15 +267,34 @@ __global__ __launch_bounds__(NTU) void k_conv3(const uint16_t* __restrict__ X, i
   // slot cs of the chunk
   const int kq = (8 * kg) / CC, cs = ((8 * kg) % CC) / 8;
   const int nch = Cin / CC;
+  // staging: this thread's tile elements (compile-time count), their source
+  // offsets in 16-byte units from the chunk base (-1: zero, outside the
+  // volume) computed once, and all loads of a chunk issued together (a
+  // register prefetch of the next chunk measured slower: the weight loads of
+  // the compute then wait behind it on the in-order load counter)
+  constexpr int NEL = SX * SY * SZ * CS, NIT = (NEL + NTU - 1) / NTU;
+  int soff[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int i = threadIdx.x + k * NTU;
+    const int sl = i % CS, v = i / CS;
+    const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / (SX * SY) - 1;
+    const bool in = i < NEL && gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D;
+    soff[k] = in ? (int)(((((long)gz * d.H + gy) * d.W + gx) * xs + xo) / 8 + sl) : -1;
+  }
+  uint4 sv[NIT];
+  auto load = [&](int q) {
+    const uint4* src = reinterpret_cast<const uint4*>(X + q * xq);
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) sv[k] = soff[k] >= 0 ? src[soff[k]] : make_uint4(0u, 0u, 0u, 0u);
+  };
   for (int q = 0; q < nch; ++q) {
+    load(q);
     if (q) __syncthreads();
-    for (int i = threadIdx.x; i < SX * SY * SZ * CS; i += NTU) {
-      const int sl = i % CS, v = i / CS;
-      const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / (SX * SY) - 1;
-      uint4 val = make_uint4(0u, 0u, 0u, 0u);
-      if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D)
-        val = *reinterpret_cast<const uint4*>(X + q * xq + (((long)gz * d.H + gy) * d.W + gx) * xs + xo + sl * 8);
-      tile4[v * PV + sl] = val;
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) {
+      const int i = threadIdx.x + k * NTU;
+      if (i < NEL) tile4[(i / CS) * PV + i % CS] = sv[k];
     }
     __syncthreads();
     if (live) {
@@ -378,16 +397,27 @@ __global__ __launch_bounds__(NTU) void k_conv_in(InWin in, Dims d, const uint16_
     norm_of(in.sums, in.n, &m0, &s0);
     if (in.aux) norm_of(in.sums + 2, in.n, &m1, &s1);
   }
-  for (int v = threadIdx.x; v < SX * SY * SZ; v += NTU) {
+  // all of this thread's loads issued before any is used
+  constexpr int NEL = SX * SY * SZ, NIT = (NEL + NTU - 1) / NTU;
+  float fv[NIT], fa[NIT];
+  bool fin[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int v = threadIdx.x + k * NTU;
     const int gx = x0 + v % SX - 1, gy = y0 + (v / SX) % SY - 1, gz = z0 + v / PL - 1;
     const int vx = in.ox + gx, vy = in.oy + gy, vz = in.oz + gz;
-    uint32_t w = 0;
-    if (gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D && vx >= 0 && vx < in.X && vy >= 0 &&
-        vy < in.Y && vz >= 0 && vz < in.Z) {
-      const long i = ((long)vz * in.Y + vy) * in.X + vx;
-      w = (uint32_t)f2bf((in.vol[i] - m0) * s0) | (in.aux ? (uint32_t)f2bf((in.aux[i] - m1) * s1) << 16 : 0u);
-    }
-    t2[v] = w;
+    fin[k] = v < NEL && gx >= 0 && gx < d.W && gy >= 0 && gy < d.H && gz >= 0 && gz < d.D && vx >= 0 &&
+             vx < in.X && vy >= 0 && vy < in.Y && vz >= 0 && vz < in.Z;
+    const long i = fin[k] ? ((long)vz * in.Y + vy) * in.X + vx : 0;
+    fv[k] = fin[k] ? in.vol[i] : 0.f;
+    fa[k] = (fin[k] && in.aux) ? in.aux[i] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int v = threadIdx.x + k * NTU;
+    if (v < NEL)
+      t2[v] = fin[k] ? ((uint32_t)f2bf((fv[k] - m0) * s0) | (in.aux ? (uint32_t)f2bf((fa[k] - m1) * s1) << 16 : 0u))
+                     : 0u;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -484,27 +514,45 @@ __global__ __launch_bounds__(64) void k_conv_dw(const uint16_t* __restrict__ X, 
     dy = (t / 3) % 3 - 1;
     dx = t % 3 - 1;
   }
-  const long V = d.V();
-  const long v0 = (long)blockIdx.x * VCH;
+  const int V = (int)d.V();  // below 2^31 (checked when the network is made)
+  const int v0 = blockIdx.x * VCH;
   const int row = lane & 15, kq = lane >> 4;
   const int co = cot * 16 + row;  // A row
   const int ci = cit * 16 + row;  // B col
+  const long cofs = (xq ? (ci / xs) * xq + ci % xs : ci) + xo;
+  // coordinates of this lane's four voxels v0 + vb + 4 kq + i, advanced by
+  // 16 voxels per step without divisions
+  int cx[4], cy[4], cz[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int v = v0 + 4 * kq + i;
+    cz[i] = v / (d.W * d.H);
+    cy[i] = (v / d.W) % d.H;
+    cx[i] = v % d.W;
+  }
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int vb = 0; vb < VCH; vb += 16) {
     bf16x4 a, b;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const long v = v0 + vb + 4 * kq + i;
+      const int v = v0 + vb + 4 * kq + i;
       short av = 0, bv = 0;
       if (v < V) {
-        av = (short)dY[v * Cout + co];
-        const int x = (int)(v % d.W), y = (int)((v / d.W) % d.H), z = (int)(v / ((long)d.W * d.H));
-        const int xx = x + dx, yy = y + dy, zz = z + dz;
+        av = (short)dY[(long)v * Cout + co];
+        const int xx = cx[i] + dx, yy = cy[i] + dy, zz = cz[i] + dz;
         if (xx >= 0 && xx < d.W && yy >= 0 && yy < d.H && zz >= 0 && zz < d.D)
-          bv = (short)X[(xq ? (ci / xs) * xq + ci % xs : ci) + (((long)zz * d.H + yy) * d.W + xx) * xs + xo];
+          bv = (short)X[cofs + (((long)zz * d.H + yy) * d.W + xx) * xs];
       }
       a[i] = av;
       b[i] = bv;
+      cx[i] += 16;
+      while (cx[i] >= d.W) {
+        cx[i] -= d.W;
+        if (++cy[i] == d.H) {
+          cy[i] = 0;
+          ++cz[i];
+        }
+      }
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
@@ -527,25 +575,48 @@ __global__ void k_input(const float* vol, const float* aux, long V, uint16_t* X0
   }
 }
 
-// relu backward: dpre = dpost * (post > 0) -> bf16 for the MFMA, + bias grad
-__global__ void k_relu_bwd(const float* dpost, const uint16_t* post, int ps, int po, int C, long V, int relu,
-                           uint16_t* dpre, float* dbias) {
-  __shared__ float red[NTU];
-  const int c = blockIdx.y;
-  float s = 0;
-  for (long v = (long)blockIdx.x * NTU + threadIdx.x; v < V; v += (long)gridDim.x * NTU) {
-    float g = dpost[v * C + c];
-    if (relu && !(bf2f(post[v * ps + po + c]) > 0.f)) g = 0.f;
-    dpre[v * C + c] = f2bf(g);
-    s += g;
+// relu backward: dpre = dpost * (post > 0) -> bf16 for the MFMA, + bias
+// grad. Thread = 4 consecutive channels of a voxel (16-byte loads, 8-byte
+// stores); the thread count is a multiple of C / 4 so every thread keeps one
+// channel group and sums it, then the block adds its sums per channel.
+__global__ __launch_bounds__(NTU) void k_relu_bwd(const float* __restrict__ dpost, const uint16_t* __restrict__ post,
+                                                  int ps, int po, int C, long V, int relu, uint16_t* dpre,
+                                                  float* dbias) {
+  __shared__ float red[NTU][4];
+  const int C4 = C / 4;
+  const int per = NTU / C4;                 // voxels per block step
+  const int t = threadIdx.x;
+  float sum[4] = {0.f, 0.f, 0.f, 0.f};
+  if (t < per * C4) {
+    const int cg = t % C4;
+    for (long v = (long)blockIdx.x * per + t / C4; v < V; v += (long)gridDim.x * per) {
+      const float4 g = *reinterpret_cast<const float4*>(dpost + v * C + 4 * cg);
+      float gv[4] = {g.x, g.y, g.z, g.w};
+      if (relu) {
+        const uint2 pr = *reinterpret_cast<const uint2*>(post + v * ps + po + 4 * cg);
+        const uint16_t pb[4] = {(uint16_t)(pr.x & 0xffffu), (uint16_t)(pr.x >> 16), (uint16_t)(pr.y & 0xffffu),
+                                (uint16_t)(pr.y >> 16)};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (!(bf2f(pb[k]) > 0.f)) gv[k] = 0.f;
+      }
+      uint2 o;
+      o.x = (uint32_t)f2bf(gv[0]) | ((uint32_t)f2bf(gv[1]) << 16);
+      o.y = (uint32_t)f2bf(gv[2]) | ((uint32_t)f2bf(gv[3]) << 16);
+      *reinterpret_cast<uint2*>(dpre + v * C + 4 * cg) = o;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sum[k] += gv[k];
+    }
   }
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) red[t][k] = sum[k];
   __syncthreads();
-  for (int k = NTU / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
+  if (t < C) {
+    const int cg = t / 4, k = t % 4;
+    float a = 0.f;
+    for (int r = cg; r < per * C4; r += C4) a += red[r][k];
+    atomicAdd(&dbias[t], a);
   }
-  if (threadIdx.x == 0) atomicAdd(&dbias[c], red[0]);
 }
 
 // 2x2x2 average pool, channels-last (an average integrates the signal of a
@@ -628,11 +699,21 @@ __global__ __launch_bounds__(NTU) void k_up(const uint16_t* __restrict__ X, Dims
   constexpr int C8 = C / 8, SX = UX + 2, SY = UY + 2, SZ = LZ + 2;
   __shared__ uint4 t[SX * SY * SZ * C8];
   const int X0 = (blockIdx.x % nbx) * UX, Y0 = ((blockIdx.x / nbx) % nby) * UY, Z0 = (blockIdx.x / (nbx * nby)) * LZ;
-  for (int i = threadIdx.x; i < SX * SY * SZ * C8; i += NTU) {
+  constexpr int NEL = SX * SY * SZ * C8, NIT = (NEL + NTU - 1) / NTU;
+  uint4 sv[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int i = threadIdx.x + k * NTU;
     const int c8 = i % C8, v = i / C8;
     const int gx = min(max(X0 + v % SX - 1, 0), lo.W - 1), gy = min(max(Y0 + (v / SX) % SY - 1, 0), lo.H - 1),
               gz = min(max(Z0 + v / (SX * SY) - 1, 0), lo.D - 1);
-    t[i] = *reinterpret_cast<const uint4*>(X + (((long)gz * lo.H + gy) * lo.W + gx) * C + c8 * 8);
+    sv[k] = i < NEL ? *reinterpret_cast<const uint4*>(X + (((long)gz * lo.H + gy) * lo.W + gx) * C + c8 * 8)
+                    : make_uint4(0u, 0u, 0u, 0u);
+  }
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int i = threadIdx.x + k * NTU;
+    if (i < NEL) t[i] = sv[k];
   }
   __syncthreads();
   const int HW = 2 * lo.W, HH = 2 * lo.H, HD = 2 * lo.D;
@@ -900,49 +981,62 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
     f32x4 acc[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[j], xb, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-    // this lane's bin candidates: first index of its largest non-NaN logit
+    // this lane's logits: channel 16 j + 4 kg + i
+    float lg[5][4];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) lg[j][i] = acc[j][i] + bb[j][i];
+    // bin candidates: first index of the largest non-NaN logit (selects, no
+    // branches), then the lowest index among the four lanes of the voxel
     float best = 0.f;
     int bi = INT_MAX;
-    bool first_nan = false;
 #pragma unroll
     for (int j = 0; j < 5; ++j)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int c = 16 * j + 4 * kg + i;
-        const float l = acc[j][i] + bb[j][i];
-        if (c >= 1 && c <= nbin) {
-          if (c == 1 && l != l) first_nan = true;
-          if (l == l && (bi == INT_MAX || l > best)) {
-            best = l;
-            bi = c - 1;
-          }
-        }
+        const float l = lg[j][i];
+        const bool take = c >= 1 && c <= nbin && l == l && (bi == INT_MAX || l > best);
+        best = take ? l : best;
+        bi = take ? c - 1 : bi;
       }
+    int first_nan = (kg == 0 && lg[0][1] != lg[0][1]) ? 1 : 0;
     for (int o = 16; o < 64; o <<= 1) {
       const float ob = __shfl_xor(best, o, 64);
       const int oi = __shfl_xor(bi, o, 64);
-      first_nan = first_nan || __shfl_xor((int)first_nan, o, 64);
+      first_nan |= __shfl_xor(first_nan, o, 64);
       if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) {
         best = ob;
         bi = oi;
       }
     }
+    // the three offset channels nbin + 1 + k, each held by the lanes of one kg
+    float ov[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int ch = nbin + 1 + k;
+      float val = 0.f;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) val = (16 * j + 4 * kg + i == ch) ? lg[j][i] : val;
+      ov[k] = val;
+    }
     const int c = v / tHW, rr = v - c * tHW, b = rr / t.W, a = rr - b * t.W;
     const int x = ow.ox + a, y = ow.oy + b, z = ow.oz + c;
-    if (a < ow.halo || a >= ow.halo + ow.cx || b < ow.halo || b >= ow.halo + ow.cy || c < ow.halo ||
-        c >= ow.halo + ow.cz || x >= ow.X || y >= ow.Y || z >= ow.Z)
-      continue;
+    const bool inside = a >= ow.halo && a < ow.halo + ow.cx && b >= ow.halo && b < ow.halo + ow.cy &&
+                        c >= ow.halo && c < ow.halo + ow.cz && x < ow.X && y < ow.Y && z < ow.Z;
+    if (!inside) continue;
     const long dst = ((long)z * ow.Y + y) * ow.X + x;
+    if (kg == 0) {
+      if (prob) prob[dst] = raw ? lg[0][0] : 1.f / (1.f + expf(-lg[0][0]));
+      if (bin) bin[dst] = (first_nan || bi == INT_MAX) ? 0 : bi;
+    }
+    if (offs)
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ch = 16 * j + 4 * kg + i;
-        const float l = acc[j][i] + bb[j][i];
-        if (ch == 0 && prob) prob[dst] = raw ? l : 1.f / (1.f + expf(-l));
-        if (offs && ch > nbin && ch < nc) offs[dst * 3 + (ch - nbin - 1)] = l;
-      }
-    if (bin && kg == 0) bin[dst] = (first_nan || bi == INT_MAX) ? 0 : bi;
+      for (int k = 0; k < 3; ++k)
+        if (kg == (((nbin + 1 + k) >> 2) & 3)) offs[dst * 3 + k] = ov[k];
   }
   }
 }
@@ -1414,7 +1508,8 @@ double Unet::forward_flops() const {
   const Impl& I = *d_;
   const long vol[11] = {I.V0, I.V0, I.V1, I.V1, I.V2, I.V2, I.V1, I.V1, I.V0, I.V0, I.V0};
   double f = 0;
-  for (int i = 0; i < 11; ++i) f += 2.0 * vol[i] * I.L[i].taps * I.L[i].Cin * I.L[i].Cout;
+  // the first layer computes its two real input channels only
+  for (int i = 0; i < 11; ++i) f += 2.0 * vol[i] * I.L[i].taps * (i == 0 ? 2 : I.L[i].Cin) * I.L[i].Cout;
   return f;
 }
 
@@ -1457,7 +1552,7 @@ void Unet::backward(const float* tprob, const int32_t* tbin, const float* toff, 
                        int xo, long xq, Dims d, float* gin) {
     ConvLayer& c = L[li];
     const long V = d.V();
-    dim3 g(gridv(V) > 1024 ? 1024 : gridv(V), c.Cout);
+    const dim3 g(gridv(V) > 2048 ? 2048 : gridv(V));
     hipLaunchKernelGGL(k_relu_bwd, g, dim3(NTU), 0, s, gpost, post, ps, po, c.Cout, V, c.relu, I.gb,
                        c.g + (long)c.taps * c.Cout * c.Cin);
     conv_weight_grad(X, xs, xo, xq, c.Cin, I.gb, c.Cout, d.D, d.H, d.W, c.taps, c.g, s);

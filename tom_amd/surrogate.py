@@ -36,10 +36,15 @@ UNET_LAYERS = [(16, 16, 27, 0), (16, 16, 27, 0), (16, 32, 27, 1), (32, 32, 27, 1
 
 
 def unet_forward_flops(dims) -> float:
-    """MFMA flops of one forward pass over a volume of `dims` voxels (the
-    first layer counts its 15 zero padding input channels, as it computes them)."""
+    """Flops of one forward pass over a volume of `dims` voxels: every layer's
+    multiply-adds as the network defines them, the first layer with its two
+    real input channels (the tomogram and the aux map; its 14 zero padding
+    channels are not computed) and the head with all 80 output channels it
+    computes. The padding the MFMA tiles add (a tenth K step of zeros for
+    16-channel inputs) is not counted."""
     V = float(np.prod(dims))
-    return sum(2.0 * V / 8 ** lvl * t * ci * co for ci, co, t, lvl in UNET_LAYERS)
+    return sum(2.0 * V / 8 ** lvl * t * (2 if i == 0 else ci) * co
+               for i, (ci, co, t, lvl) in enumerate(UNET_LAYERS))
 
 
 # ---------------------------------------------------------------- auxiliary input
