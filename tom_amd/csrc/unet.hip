@@ -844,10 +844,18 @@ __global__ void k_loss(const float* logits, int CH, int nbin, const float* tprob
     l2 += __shfl_xor(l2, o, 64);
     l3 += __shfl_xor(l3, o, 64);
   }
+  // one set of FP64 atomics per block (the grid is capped at a few hundred)
+  __shared__ double red[3][NTU / 64];
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&loss[0], l1);
-    atomicAdd(&loss[1], l2);
-    atomicAdd(&loss[2], l3);
+    red[0][threadIdx.x >> 6] = l1;
+    red[1][threadIdx.x >> 6] = l2;
+    red[2][threadIdx.x >> 6] = l3;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double a = 0;
+    for (int w = 0; w < NTU / 64; ++w) a += red[threadIdx.x][w];
+    atomicAdd(&loss[threadIdx.x], a);
   }
 }
 
@@ -1532,7 +1540,7 @@ void Unet::backward(const float* tprob, const int32_t* tbin, const float* toff, 
   for (auto& c : L) TOMHIP_CHECK(hipMemsetAsync(c.g, 0, sizeof(float) * c.np, s));
   const float inv_v = 1.f / (float)I.V0, inv_np = npos > 0 ? 1.f / (float)npos : 0.f;
   const float inv_no = noff > 0 ? 1.f / (float)noff : 0.f;
-  hipLaunchKernelGGL(k_loss, dim3(gridv(I.V0)), dim3(NTU), 0, s, (const float*)I.logits, kHeadCh, I.nbin, tprob,
+  hipLaunchKernelGGL(k_loss, dim3(std::min(gridv(I.V0), 512u)), dim3(NTU), 0, s, (const float*)I.logits, kHeadCh, I.nbin, tprob,
                      tbin, toff, I.V0, wpos, inv_v, inv_np, inv_no, I.g_logits, I.loss);
   // zero the accumulated data gradients
   auto zero = [&](float* p, long n) { TOMHIP_CHECK(hipMemsetAsync(p, 0, sizeof(float) * n, s)); };
