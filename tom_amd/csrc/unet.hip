@@ -241,7 +241,7 @@ struct Frag {
 };
 
 template <int CT, int CC, int TZ>
-__global__ __launch_bounds__(NTU) void k_conv3(const uint16_t* __restrict__ X, int xs, int xo, long xq, int Cin,
+__global__ __launch_bounds__(NTU) __attribute__((amdgpu_waves_per_eu(3))) void k_conv3(const uint16_t* __restrict__ X, int xs, int xo, long xq, int Cin,
                                                Dims d, const uint4* __restrict__ Wm, const float* __restrict__ bias,
                                                int relu, uint16_t* Yb, int ys, int yo, float* Yf, int nbx, int nby) {
   extern __shared__ uint4 tile4[];
@@ -948,14 +948,17 @@ struct OutWin {
   int halo, cx, cy, cz, ox, oy, oz, X, Y, Z;
 };
 
+template <int NB>  // NB > 0: nbin fixed at compile time (the offset lanes and bin masks fold)
 __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict__ Xb, Dims t,
                                                       const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
-                                                      int nbin, OutWin ow, float* prob, int32_t* bin, float* offs,
+                                                      int nbin_rt, OutWin ow, float* prob, int32_t* bin, float* offs,
                                                       int raw) {
+  const int nbin = NB > 0 ? NB : nbin_rt;
   const int lane = threadIdx.x & 63, n = lane & 15, kg = lane >> 4;
   const int nc = 1 + nbin + 3;
   bf16x4 wa[5];
-  float bb[5][4];
+  float bb[5][4];  // bias of every used channel
+  float bm[5][4];  // bias of the bin channels, -inf elsewhere (never the maximum)
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     const uint2 rw = *reinterpret_cast<const uint2*>(Wt + (16 * j + n) * 16 + 4 * kg);
@@ -967,6 +970,7 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
     for (int i = 0; i < 4; ++i) {
       const int c = 16 * j + 4 * kg + i;
       bb[j][i] = c < nc ? bias[c] : 0.f;
+      bm[j][i] = (c >= 1 && c <= nbin) ? bb[j][i] : -INFINITY;
     }
   }
   const int ngrp = (int)(t.V() / 16);  // V is a multiple of 64 and below 2^31
@@ -989,47 +993,46 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
     f32x4 acc[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[j], xb, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-    // this lane's logits: channel 16 j + 4 kg + i
-    float lg[5][4];
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) lg[j][i] = acc[j][i] + bb[j][i];
-    // bin candidates: first index of the largest non-NaN logit (selects, no
-    // branches), then the lowest index among the four lanes of the voxel
-    float best = 0.f;
-    int bi = INT_MAX;
+    // bins: the largest non-NaN bin logit over the voxel's four lanes (fmaxf
+    // drops NaN), then the lowest channel holding it; channel 1 NaN -> bin 0
+    // (the sequential first-maximum rule)
+    float lb[5][4];
+    float m = -INFINITY;
 #pragma unroll
     for (int j = 0; j < 5; ++j)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int c = 16 * j + 4 * kg + i;
-        const float l = lg[j][i];
-        const bool take = c >= 1 && c <= nbin && l == l && (bi == INT_MAX || l > best);
-        best = take ? l : best;
-        bi = take ? c - 1 : bi;
+        lb[j][i] = acc[j][i] + bm[j][i];
+        m = fmaxf(m, lb[j][i]);
       }
-    int first_nan = (kg == 0 && lg[0][1] != lg[0][1]) ? 1 : 0;
-    for (int o = 16; o < 64; o <<= 1) {
-      const float ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      first_nan |= __shfl_xor(first_nan, o, 64);
-      if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    // the three offset channels nbin + 1 + k, each held by the lanes of one kg
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    int k = 127;  // local channel offset 16 j + i of the first maximum
+#pragma unroll
+    for (int j = 4; j >= 0; --j)
+#pragma unroll
+      for (int i = 3; i >= 0; --i) k = (lb[j][i] == m) ? 16 * j + i : k;
+    int bi = (k < 127 && m > -INFINITY) ? k + 4 * kg - 1 : INT_MAX;
+    bi = min(bi, __shfl_xor(bi, 16, 64));
+    bi = min(bi, __shfl_xor(bi, 32, 64));
+    int first_nan = (kg == 0 && acc[0][1] != acc[0][1]) ? 1 : 0;
+    first_nan |= __shfl_xor(first_nan, 16, 64);
+    first_nan |= __shfl_xor(first_nan, 32, 64);
+    // the three offset channels nbin + 1 + q, each held by the lanes of one kg
     float ov[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int ch = nbin + 1 + k;
-      float val = 0.f;
+    for (int q = 0; q < 3; ++q) {
+      const int ch = nbin + 1 + q;
+      if (NB > 0) {
+        ov[q] = acc[ch / 16][ch % 4] + bb[ch / 16][ch % 4];
+      } else {
+        float val = 0.f;
 #pragma unroll
-      for (int j = 0; j < 5; ++j)
+        for (int j = 0; j < 5; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) val = (16 * j + 4 * kg + i == ch) ? lg[j][i] : val;
-      ov[k] = val;
+          for (int i = 0; i < 4; ++i) val = (16 * j + 4 * kg + i == ch) ? acc[j][i] + bb[j][i] : val;
+        ov[q] = val;
+      }
     }
     const int c = v / tHW, rr = v - c * tHW, b = rr / t.W, a = rr - b * t.W;
     const int x = ow.ox + a, y = ow.oy + b, z = ow.oz + c;
@@ -1038,13 +1041,14 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
     if (!inside) continue;
     const long dst = ((long)z * ow.Y + y) * ow.X + x;
     if (kg == 0) {
-      if (prob) prob[dst] = raw ? lg[0][0] : 1.f / (1.f + expf(-lg[0][0]));
+      const float l0 = acc[0][0] + bb[0][0];
+      if (prob) prob[dst] = raw ? l0 : 1.f / (1.f + expf(-l0));
       if (bin) bin[dst] = (first_nan || bi == INT_MAX) ? 0 : bi;
     }
     if (offs)
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (kg == (((nbin + 1 + k) >> 2) & 3)) offs[dst * 3 + k] = ov[k];
+      for (int q = 0; q < 3; ++q)
+        if (kg == (((nbin + 1 + q) >> 2) & 3)) offs[dst * 3 + q] = ov[q];
   }
   }
 }
@@ -1506,9 +1510,15 @@ void Unet::head_window(int halo, int ox, int oy, int oz, int X, int Y, int Z, fl
   Impl& I = *d_;
   ConvLayer& c = I.L[10];
   const OutWin ow{halo, I.W - 2 * halo, I.H - 2 * halo, I.D - 2 * halo, ox, oy, oz, X, Y, Z};
-  hipLaunchKernelGGL(k_head_predict, dim3(gridv(I.V0 / 16)), dim3(NTU), 0, I.s, (const uint16_t*)I.d1b,
-                     Dims{I.D, I.H, I.W}, (const uint16_t*)c.wb,
-                     (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.nbin, ow, prob, bin, offs, logit ? 1 : 0);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(gridv(I.V0 / 16)), dim3(NTU), 0, I.s, (const uint16_t*)I.d1b, Dims{I.D, I.H, I.W},
+                       (const uint16_t*)c.wb, (const float*)(c.p + (long)c.taps * c.Cout * c.Cin), I.nbin, ow, prob,
+                       bin, offs, logit ? 1 : 0);
+  };
+  if (I.nbin == 64)
+    go(k_head_predict<64>);
+  else
+    go(k_head_predict<0>);
   TOMHIP_CHECK(hipGetLastError());
 }
 
