@@ -134,11 +134,11 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
     T, M, W, tn, mu_t, sig_t = plan_template
     ref = S.exhaustive_peaks(score, ang, cfg["n"], 0.3, max_peaks=max(1000, 3 * cfg["P"]))
 
-    def timed(ps, proposer=None):
-        ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer)  # warm: graph capture, code objects
+    def timed(ps, proposer=None, reranker=None):
+        ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer, reranker=reranker)  # warm: capture, code objects
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sc, eul, pos = ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer)
+        sc, eul, pos = ps.run(plan, V, tn, W, mu_t, sig_t, proposer=proposer, reranker=reranker)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0 + pre_s  # needs the plan's precompute, as the exhaustive run does
         rec = S.recall(ref[:, :3], pos[sc > 0.3])
@@ -175,12 +175,17 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
             ti = net.timings["infer_s"]
             withnet["unet_infer"] = {"flops": fl, "s": round(ti, 4), "tflops_per_s": round(fl / ti / 1e12, 2),
                                      "frac_of_bf16_dense_peak": round(fl / ti / 2.5e15, 4)}
+            # the network orders the coarse maxima instead of replacing them:
+            # robust z of the coarse score + 2 x its logit near the maximum
+            reranked = timed(S.PrunedSearch(n_coarse=nc, K=K, levels=lv), reranker=net)
             out["network_vs_coarse_maxima"] = {
                 "weights": os.path.relpath(wpath, ROOT), "coarse_maxima": base, "network_proposer": withnet,
+                "network_reranked_maxima": reranked,
                 "training": {k: meta[k] for k in ("tomos", "size", "particles", "steps", "crop", "aux", "coarse",
                                                   "targets", "label_s", "train_s") if k in meta}}
             log(f"[bench] coarse {nc}: maxima recall {base['recall_gt_0.3']:.4f} ({base['wall_s']:.3f} s), "
-                f"network proposer recall {withnet['recall_gt_0.3']:.4f} ({withnet['wall_s']:.3f} s)")
+                f"network proposer recall {withnet['recall_gt_0.3']:.4f} ({withnet['wall_s']:.3f} s), "
+                f"maxima reranked by the network {reranked['recall_gt_0.3']:.4f} ({reranked['wall_s']:.3f} s)")
     return out
 
 

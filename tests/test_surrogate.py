@@ -238,3 +238,31 @@ def test_train_script_resumes_across_runs(api, tmp_path):
     with pytest.raises(SystemExit) as e:
         train_surrogate.main(base[:-1] + ["8", "--state", st, "--out", out_r])
     assert "other settings" in str(e.value.code)
+
+
+@pytest.mark.gpu
+def test_rerank_is_robust_z_plus_logit_near_the_maximum(api):
+    """Surrogate.rerank = (coarse - median) / sigma + 2 x the largest network
+    logit in the 5^3 cube around each row (checked against the logit map of
+    Unet.infer and a brute-force cube maximum, borders clipped)."""
+    import torch
+    print("rerank", flush=True)
+    w = np.load(os.path.join(os.path.dirname(S.__file__), "weights", "unet_c2.npz"), allow_pickle=False)
+    net = S.Surrogate(w["params"], w["bins"])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    vol = torch.randn((40, 48, 64), generator=g, device="cuda")
+    coarse = torch.rand((40, 48, 64), generator=g, device="cuda") * 0.3
+    rng = np.random.default_rng(4)
+    xyz = np.stack([rng.integers(0, 64, 50), rng.integers(0, 48, 50), rng.integers(0, 40, 50)], 1)
+    xyz[:3] = [[0, 0, 0], [63, 47, 39], [1, 46, 0]]  # cubes clipped at the borders
+    noise = (0.15, 0.08)
+    got = net.rerank(vol, coarse, xyz, noise)
+    unet, halo = net._network((64, 48, 40))
+    lg = unet.infer(vol, halo, aux=coarse, logit=True)[0].cpu().numpy()  # (z, y, x)
+    cs = coarse.cpu().numpy()
+    want = []
+    for x, y, z in xyz:
+        cube = lg[max(z - 2, 0):z + 3, max(y - 2, 0):y + 3, max(x - 2, 0):x + 3]
+        want.append((cs[z, y, x] - noise[0]) / noise[1] + 2.0 * cube.max())
+    np.testing.assert_allclose(got, np.asarray(want), rtol=1e-6, atol=1e-6)

@@ -370,6 +370,29 @@ class Surrogate:
         self.infer_flops = unet_forward_flops(net.dims) * _tiles(dims, net.dims, halo)
         return np.concatenate([lm[:, :3], bins[:, None].astype(np.float32), lm[:, 3:4]], 1)
 
+    def rerank(self, volume, coarse, xyz, noise, alpha=2.0, radius=2):
+        """Rank values of coarse-scan local maxima (rows x y z): the robust z
+        of the coarse best score, (s - median) / sigma with noise = (median,
+        sigma) of the coarse map, plus alpha times the largest particle logit
+        of the network within the cube of half-edge `radius` around the
+        maximum (the network's peak may sit a voxel or two off the matched
+        filter's). The network sees the tomogram and the coarse map, as in
+        propose()."""
+        dims = tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume)
+        net, halo = self._network(dims)
+        t0 = time.perf_counter()
+        logit = net.infer(volume, halo, aux=coarse, logit=True)[0]
+        _sync(logit)
+        t1 = time.perf_counter()
+        xyz = np.asarray(xyz, np.int64).reshape(-1, 3)
+        peak = api.snap_maxima(logit, xyz, radius).astype(np.int64)
+        lv = _gather(logit, peak).astype(np.float64)
+        cs = _gather(coarse, xyz).astype(np.float64)
+        med, sig = noise
+        self.timings.update(infer_s=t1 - t0, rerank_s=time.perf_counter() - t1)
+        self.infer_flops = unet_forward_flops(net.dims) * _tiles(dims, net.dims, halo)
+        return (cs - med) / max(sig, 1e-30) + alpha * lv
+
     def refine(self, plan, volume, cand):
         """Exact FLCF over the local grid of each candidate's bin and its 27
         voxels. Returns (score[P], euler[P, 3], pos[P, 3] x y z)."""
@@ -492,7 +515,13 @@ class PrunedSearch:
 
     def noise_threshold(self, score):
         """median + sigmas * 1.4826 MAD of the coarse best-score map (at most
-        2^22 voxels sampled with a fixed stride), and the (median, sigma)."""
+        2^22 voxels sampled with a fixed stride); sets self.noise."""
+        med, sig = self.noise_of(score)
+        return med + self.sigmas * sig
+
+    def noise_of(self, score):
+        """(median, 1.4826 MAD) of the coarse best-score map (at most 2^22
+        voxels sampled with a fixed stride), also kept in self.noise."""
         if hasattr(score, "cpu"):
             import torch
             flat = score.reshape(-1)
@@ -507,7 +536,7 @@ class PrunedSearch:
             mad = float(np.median(np.abs(x - med)))
         sig = 1.4826 * mad
         self.noise = (med, sig)
-        return med + self.sigmas * sig
+        return self.noise
 
     def candidates(self, score, index, rank=None, volume=None):
         """(P, 3) x y z and (P, 3) coarse Euler angles of the kept maxima."""
@@ -540,11 +569,13 @@ class PrunedSearch:
             eul = grids[ai, np.arange(P)]
         return sc, eul, pos
 
-    def run(self, plan, volume, tmpl, W, mu_t, sig_t, rank=None, proposer=None):
+    def run(self, plan, volume, tmpl, W, mu_t, sig_t, rank=None, proposer=None, reranker=None):
         """proposer (optional): a Surrogate whose network takes the coarse
         score map as its second input channel; its candidates replace the
         local maxima of the coarse map (orientations still come from the
-        coarse scan at each candidate)."""
+        coarse scan at each candidate). reranker (optional): a Surrogate that
+        orders the coarse map's own local maxima (Surrogate.rerank) before the
+        suppression keeps K of them."""
         t0 = time.perf_counter()
         if self.bin:
             score, index = self.coarse_binned(volume)
@@ -559,6 +590,10 @@ class PrunedSearch:
             xyz, eul = self.candidates(score, index, rank, volume)
             dims = np.asarray(tuple(reversed(volume.shape)) if hasattr(volume, "cpu") else np.shape(volume))
             xyz = np.minimum(xyz << self.bin, dims - 1)
+        elif reranker is not None:
+            self.noise_of(score)  # (median, sigma) of the coarse map for the robust z
+            xyz, eul = self.candidates(score, index, lambda vol, c: reranker.rerank(vol, score, c, self.noise),
+                                       volume)
         elif proposer is None:
             xyz, eul = self.candidates(score, index, rank, volume)
         else:
