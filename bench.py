@@ -178,11 +178,30 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
             # the network orders the coarse maxima instead of replacing them:
             # robust z of the coarse score + 2 x its logit near the maximum
             reranked = timed(S.PrunedSearch(n_coarse=nc, K=K, levels=lv), reranker=net)
+            # training throughput: Adam steps on crops of the network's training
+            # size (forward, backward, update, host-side crop) over one
+            # synthetic labelled volume
+            crop = int(meta.get("crop", 96))
+            rng = np.random.default_rng(5)
+            vs = (crop + 32,) * 3
+            tp = (rng.random(vs) < 0.01).astype(np.float32)
+            tb = np.where(tp > 0, rng.integers(0, 64, vs), -1).astype(np.int32)
+            tr = S.Trainer(crop=crop, nbin=64, seed=1)
+            tr.add(rng.standard_normal(vs).astype(np.float32), tp, tb, rng.standard_normal(vs).astype(np.float32))
+            tr.train(3)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tr.train(20)
+            torch.cuda.synchronize()
+            train_ms = (time.perf_counter() - t0) / 20 * 1e3
+            tr.net.close()
             out["network_vs_coarse_maxima"] = {
                 "weights": os.path.relpath(wpath, ROOT), "coarse_maxima": base, "network_proposer": withnet,
                 "network_reranked_maxima": reranked,
                 "training": {k: meta[k] for k in ("tomos", "size", "particles", "steps", "crop", "aux", "coarse",
-                                                  "targets", "label_s", "train_s") if k in meta}}
+                                                  "targets", "label_s", "train_s") if k in meta},
+                "train_step_ms": {"crop": crop, "ms": round(train_ms, 3),
+                                  "what": "forward + backward + Adam + host crop, mean of 20 steps"}}
             log(f"[bench] coarse {nc}: maxima recall {base['recall_gt_0.3']:.4f} ({base['wall_s']:.3f} s), "
                 f"network proposer recall {withnet['recall_gt_0.3']:.4f} ({withnet['wall_s']:.3f} s), "
                 f"maxima reranked by the network {reranked['recall_gt_0.3']:.4f} ({reranked['wall_s']:.3f} s)")
