@@ -1,4 +1,4 @@
-// LDS-resident batched Stockham FFT engine for gfx950 (radix 2/3/4/8).
+// LDS-resident batched Stockham FFT engine for gfx950 (radix 2/3/4/5/8).
 //
 // A workgroup of NT threads transforms NL lines of length L that sit in LDS.
 // Two LDS layouts:
@@ -16,8 +16,8 @@
 namespace tomhip {
 
 // ---------------------------------------------------------------- radix plan
-// L = 2^a 3^b. Stages: radix 8 while it leaves no lone 2 behind, then 4/2,
-// then the 3s.
+// L = 2^a 3^b 5^c. Stages: radix 8 while it leaves no lone 2 behind, then
+// 4/2, then the 3s, then the 5s.
 __host__ __device__ constexpr int ilog2_exact(int n) {
   int a = 0;
   while (n % 2 == 0) { n /= 2; ++a; }
@@ -28,18 +28,23 @@ __host__ __device__ constexpr int count3(int n) {
   while (n % 3 == 0) { n /= 3; ++b; }
   return b;
 }
+__host__ __device__ constexpr int count5(int n) {
+  int c = 0;
+  while (n % 5 == 0) { n /= 5; ++c; }
+  return c;
+}
 __host__ __device__ constexpr int plan_nstages(int L) {
-  int a = ilog2_exact(L), b = count3(L);
+  int a = ilog2_exact(L), b = count3(L), c = count5(L);
   int n8 = a / 3, rem = a % 3, n4 = 0, n2 = 0;
   if (rem == 1) {
     if (n8 > 0) { n8 -= 1; n4 = 2; } else { n2 = 1; }
   } else if (rem == 2) {
     n4 = 1;
   }
-  return n8 + n4 + n2 + b;
+  return n8 + n4 + n2 + b + c;
 }
 __host__ __device__ constexpr int plan_radix(int L, int s) {
-  int a = ilog2_exact(L), b = count3(L);
+  int a = ilog2_exact(L), b = count3(L), c = count5(L);
   int n8 = a / 3, rem = a % 3, n4 = 0, n2 = 0;
   if (rem == 1) {
     if (n8 > 0) { n8 -= 1; n4 = 2; } else { n2 = 1; }
@@ -52,7 +57,9 @@ __host__ __device__ constexpr int plan_radix(int L, int s) {
   s -= n4;
   if (s < n2) return 2;
   s -= n2;
-  return (s < b) ? 3 : 1;
+  if (s < b) return 3;
+  s -= b;
+  return (s < c) ? 5 : 1;
 }
 __host__ __device__ constexpr int plan_ns(int L, int s) {
   int ns = 1;
@@ -144,6 +151,31 @@ struct Bfly<T, 3, DIR> {
     v[0] = v[0] + t;
     v[1] = m + is;
     v[2] = m - is;
+  }
+};
+
+// y_k = sum_n x_n e^{DIR 2 pi i nk/5}: with t1 = x1 + x4, t2 = x2 + x3,
+// d1 = x1 - x4, d2 = x2 - x3, a1 = x0 + c1 t1 + c2 t2, a2 = x0 + c2 t1 + c1 t2,
+// b1 = s1 d1 + s2 d2, b2 = s2 d1 - s1 d2 (c_k = cos 2 pi k/5, s_k = sin 2 pi k/5):
+// y1,4 = a1 +- DIR i b1, y2,3 = a2 +- DIR i b2.
+template <typename T, int DIR>
+struct Bfly<T, 5, DIR> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    const T c1 = T(0.30901699437494742410229341718282), c2 = T(-0.80901699437494742410229341718282);
+    const T s1 = T(DIR) * T(0.95105651629515357211643933337938);
+    const T s2 = T(DIR) * T(0.58778525229247312916870595463907);
+    const cpx<T> t1 = v[1] + v[4], t2 = v[2] + v[3];
+    const cpx<T> d1 = v[1] - v[4], d2 = v[2] - v[3];
+    const cpx<T> a1 = mk(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const cpx<T> a2 = mk(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    // i * (DIR b1), i * (DIR b2) with the DIR folded into s1, s2
+    const cpx<T> ib1 = mk(-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x);
+    const cpx<T> ib2 = mk(-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x);
+    v[0] = v[0] + t1 + t2;
+    v[1] = a1 + ib1;
+    v[4] = a1 - ib1;
+    v[2] = a2 + ib2;
+    v[3] = a2 - ib2;
   }
 };
 

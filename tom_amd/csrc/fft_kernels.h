@@ -81,7 +81,8 @@ __global__ __launch_bounds__(NT) void k_c2c_strided(cpx<T>* data, int ncols, lon
 }
 
 // ---------------------------------------------------------------- dispatch
-// Supported transform lengths: 2^a 3^b from 16 to 1024.
+// Supported transform lengths: 2^a 3^b from 16 to 1024, and 2^a 3^b 5 of
+// 40 to 960 with at least three factors of 2.
 template <class F>
 void dispatch_len(int L, F&& f) {
   switch (L) {
@@ -102,16 +103,27 @@ void dispatch_len(int L, F&& f) {
     TOMHIP_LEN(512)
     TOMHIP_LEN(768)
     TOMHIP_LEN(1024)
+    TOMHIP_LEN(40)
+    TOMHIP_LEN(80)
+    TOMHIP_LEN(120)
+    TOMHIP_LEN(160)
+    TOMHIP_LEN(240)
+    TOMHIP_LEN(320)
+    TOMHIP_LEN(480)
+    TOMHIP_LEN(640)
+    TOMHIP_LEN(960)
 #undef TOMHIP_LEN
     default:
       throw Error(ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(L) +
-                                       " (supported: 2^a 3^b in {16,24,32,48,64,96,128,192,256,384,512,768,1024})");
+                                       " (supported: {16,24,32,48,64,96,128,192,256,384,512,768,1024} and "
+                                       "{40,80,120,160,240,320,480,640,960})");
   }
 }
 inline bool supported_len(int L) {
   switch (L) {
     case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
     case 384: case 512: case 768: case 1024:
+    case 40: case 80: case 120: case 160: case 240: case 320: case 480: case 640: case 960:
       return true;
     default:
       return false;

@@ -1005,10 +1005,24 @@ void dispatch_xh(int L, F&& f) {
     TOMHIP_XH(256)
     TOMHIP_XH(384)
     TOMHIP_XH(512)
+    TOMHIP_XH(80)
+    TOMHIP_XH(160)
+    TOMHIP_XH(240)
+    TOMHIP_XH(320)
+    TOMHIP_XH(480)
 #undef TOMHIP_XH
     default:
       throw Error(ERR_UNSUPPORTED, "unsupported X/2 = " + std::to_string(L));
   }
+}
+bool supported_xh(int L) {
+  bool ok = true;
+  try {
+    dispatch_xh(L, [](auto) {});
+  } catch (const Error&) {
+    ok = false;
+  }
+  return ok;
 }
 
 inline unsigned grid_for(long n) {
@@ -1026,7 +1040,7 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     : X(X_), Y(Y_), Z(Z_), Xh(X_ / 2), nx(nx_), ny(ny_), nz(nz_), mode(mode_), tw_(tw), s_(s),
       KA_(batch) {
   N = (long)X * Y * Z;
-  TOMHIP_REQUIRE(X % 32 == 0 && supported_len(X / 2) && supported_len(Y) && supported_len(Z), ERR_UNSUPPORTED,
+  TOMHIP_REQUIRE(X % 32 == 0 && supported_xh(X / 2) && supported_len(Y) && supported_len(Z), ERR_UNSUPPORTED,
                  "volume size must be X = 2*L, Y = L, Z = L with L in the supported FFT lengths and X a "
                  "multiple of 32");
   TOMHIP_REQUIRE(X <= 1024, ERR_UNSUPPORTED, "X must be <= 1024");
