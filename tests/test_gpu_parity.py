@@ -195,17 +195,17 @@ def test_find_template_c1_subset(api, oracle):
     _check_search(ccc, ang, allm, "C1 subset")
 
 
-@pytest.mark.parametrize("N", [(160, 80, 40), (96, 120, 240)])
+@pytest.mark.parametrize("N", [(160, 80, 40), (96, 120, 240), (800, 24, 200)])
 def test_find_template_radix5_sizes(api, oracle, N):
     """Volumes with a factor 5 in their FFT lengths (radix-5 stages in every
-    pass: X/2 = 80, Y = 80 / 120, Z = 40 / 240) against the FP64 oracle,
+    pass: X/2 = 80 / 400, Y = 80 / 120, Z = 40 / 240 / 200) against the FP64 oracle,
     with a wedge weight and planted copies."""
     say(f"findTemplate radix-5 sizes {N}")
     rng = np.random.default_rng(31)
     n = 16
     T = _phantom(rng, n, 4, 1.5, 5)
     V = rng.standard_normal(N).astype(np.float32)
-    for c in [(20, 20, 20), (N[0] - 24, N[1] // 2, N[2] // 2)]:
+    for c in [(20, min(20, N[1] // 2), 20), (N[0] - 24, N[1] // 2, N[2] // 2)]:
         r = oracle.rotate(T, rng.uniform(0, 360, 3))
         V[c[0] - 8:c[0] + 8, c[1] - 8:c[1] + 8, c[2] - 8:c[2] + 8] += 3 * r
     M = oracle.sphere_mask_binary(n, 7)

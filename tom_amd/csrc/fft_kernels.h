@@ -81,8 +81,8 @@ __global__ __launch_bounds__(NT) void k_c2c_strided(cpx<T>* data, int ncols, lon
 }
 
 // ---------------------------------------------------------------- dispatch
-// Supported transform lengths: 2^a 3^b from 16 to 1024, and 2^a 3^b 5 of
-// 40 to 960 with at least three factors of 2.
+// Supported transform lengths: 2^a 3^b from 16 to 1024, 2^a 3^b 5 from 40 to
+// 960 and 2^a 25 from 200 to 800 (at least three factors of 2 each).
 template <class F>
 void dispatch_len(int L, F&& f) {
   switch (L) {
@@ -112,11 +112,14 @@ void dispatch_len(int L, F&& f) {
     TOMHIP_LEN(480)
     TOMHIP_LEN(640)
     TOMHIP_LEN(960)
+    TOMHIP_LEN(200)
+    TOMHIP_LEN(400)
+    TOMHIP_LEN(800)
 #undef TOMHIP_LEN
     default:
       throw Error(ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(L) +
                                        " (supported: {16,24,32,48,64,96,128,192,256,384,512,768,1024} and "
-                                       "{40,80,120,160,240,320,480,640,960})");
+                                       "{40,80,120,160,200,240,320,400,480,640,800,960})");
   }
 }
 inline bool supported_len(int L) {
@@ -124,6 +127,7 @@ inline bool supported_len(int L) {
     case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
     case 384: case 512: case 768: case 1024:
     case 40: case 80: case 120: case 160: case 240: case 320: case 480: case 640: case 960:
+    case 200: case 400: case 800:
       return true;
     default:
       return false;

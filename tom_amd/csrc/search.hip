@@ -1010,6 +1010,7 @@ void dispatch_xh(int L, F&& f) {
     TOMHIP_XH(240)
     TOMHIP_XH(320)
     TOMHIP_XH(480)
+    TOMHIP_XH(400)
 #undef TOMHIP_XH
     default:
       throw Error(ERR_UNSUPPORTED, "unsupported X/2 = " + std::to_string(L));

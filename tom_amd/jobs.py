@@ -22,8 +22,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-# z lengths the search supports (2^a 3^b, and 2^a 3^b 5 with a radix-5 stage)
-_LENGTHS = (16, 24, 32, 40, 48, 64, 80, 96, 120, 128, 160, 192, 240, 256, 320, 384, 480, 512, 640, 768, 960, 1024)
+# z lengths the search supports (2^a 3^b, and 2^a 3^b 5 / 2^a 25 with radix-5 stages)
+_LENGTHS = (16, 24, 32, 40, 48, 64, 80, 96, 120, 128, 160, 192, 200, 240, 256, 320, 384, 400, 480, 512, 640, 768,
+            800, 960, 1024)
 
 
 @dataclass(frozen=True)
