@@ -1234,6 +1234,7 @@ int tomhip_unet_get_logits(tomhip_unet* h, float* out) {
   return guard(h ? h->ctx : nullptr, [&] {
     TOMHIP_REQUIRE(h && out, ERR_WRONG_INPUT, "null pointer");
     hipStream_t s = h->ctx->stream;
+    TOMHIP_REQUIRE(h->u->logits(), ERR_WRONG_INPUT, "no logits yet: run tomhip_unet_forward first");
     TOMHIP_CHECK(hipMemcpyAsync(out, h->u->logits(), sizeof(float) * 80 * h->V, hipMemcpyDefault, s));
     TOMHIP_CHECK(hipStreamSynchronize(s));
   });

@@ -63,7 +63,7 @@ class Unet {
   // prob / bin (device) receive every voxel
   void infer(const float* vol, const float* aux, int X, int Y, int Z, int halo, float* prob, int32_t* bin,
              bool logit = false, float* offs = nullptr);
-  const float* logits() const;  // device f32 [V][80]
+  const float* logits() const;  // device f32 [V][80]; null before the first forward() with the head
   int D() const;
   int H() const;
   int W() const;

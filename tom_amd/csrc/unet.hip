@@ -1292,7 +1292,8 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   I.W = W;
   I.nbin = nbin;
   I.V0 = (long)D * H * W;
-  TOMHIP_REQUIRE(I.V0 < (1L << 31), ERR_UNSUPPORTED, "U-Net input above 2^31 voxels");
+  // 32-bit voxel and 16-byte-slot indices in the kernels: 2 V0 < 2^31
+  TOMHIP_REQUIRE(I.V0 < (1L << 30), ERR_UNSUPPORTED, "U-Net input above 2^30 voxels");
   I.V1 = I.V0 / 8;
   I.V2 = I.V1 / 8;
   // e1a e1b e2a e2b b1 b2 d2a d2b d1a d1b head
@@ -1349,7 +1350,10 @@ Unet::Unet(hipStream_t s, int D, int H, int W, int nbin, uint64_t seed, bool tra
   I.d2b = I.alloc<uint16_t>(V1 * 32);
   I.d1a = I.alloc<uint16_t>(V0 * 16);
   I.d1b = I.alloc<uint16_t>(V0 * 16);
-  I.logits = I.alloc<float>(V0 * kHeadCh);
+  // FP32 logits of every head channel (21 GB at 512x512x256): made for
+  // training, and on the first forward() with the head otherwise; the tiled
+  // inference never needs them
+  I.logits = train ? I.alloc<float>(V0 * kHeadCh) : nullptr;
   I.loss = I.alloc<double>(3);
   I.train = train;
   if (!train) {
@@ -1501,7 +1505,10 @@ void Unet::forward_window(const float* vol, const float* aux, int X, int Y, int 
                      (d1.H + UY - 1) / UY);
   conv(8, I.c1, 16, 0, q1, D, H, W, I.d1a, 16, 0, nullptr);
   conv(9, I.d1a, 16, 0, 0, D, H, W, I.d1b, 16, 0, nullptr);
-  if (head) conv(10, I.d1b, 16, 0, 0, D, H, W, nullptr, 0, 0, I.logits);
+  if (head) {
+    if (!I.logits) I.logits = I.alloc<float>(I.V0 * kHeadCh);
+    conv(10, I.d1b, 16, 0, 0, D, H, W, nullptr, 0, 0, I.logits);
+  }
   TOMHIP_CHECK(hipGetLastError());
 }
 
@@ -1533,6 +1540,7 @@ double Unet::forward_flops() const {
 
 void Unet::predict(float* prob, int32_t* bin, bool logit, float* offs) {
   Impl& I = *d_;
+  TOMHIP_REQUIRE(I.logits, ERR_WRONG_INPUT, "predict() needs a forward() with the head first");
   hipLaunchKernelGGL(k_predict, dim3(gridv(I.V0)), dim3(NTU), 0, I.s, (const float*)I.logits, kHeadCh, I.nbin,
                      I.V0, prob, bin, offs, logit ? 1 : 0);
   TOMHIP_CHECK(hipGetLastError());
