@@ -321,3 +321,33 @@ def test_trainer_resume_matches_uninterrupted_run(api, tmp_path):
     assert d_fresh >= 5 * max(d_resume, 1e-3)
     for n in (a, b, c, ref):
         n.net.close()
+
+
+@pytest.mark.gpu
+def test_tiled_inference_equals_whole_volume(api):
+    """Tiled inference (network tiles of 96^3, a halo of 32 voxels, more than
+    the network's receptive radius of about 28) writes the same maps as one
+    inference over the whole volume wherever that radius stays inside the
+    volume (nearer the faces the tile sees the activations of its zero input
+    beyond the face where the whole-volume network zero-pads): each such core
+    voxel sees the same inputs, read and normalised by the first layer
+    through its window of the volume, and the head writes only the core."""
+    print("unet tiled inference", flush=True)
+    rng = np.random.default_rng(17)
+    shape = (96, 80, 96)  # z, y, x
+    vol = rng.standard_normal(shape).astype(np.float32)
+    aux = (rng.random(shape) * 2).astype(np.float32)
+    theta = None
+    with api.Unet((96, 80, 96), nbin=NBIN, seed=11, train=False) as whole:
+        theta = whole.get_params()
+        p0, b0, o0 = whole.infer(_as_f(vol), halo=0, aux=_as_f(aux), offsets=True)
+    with api.Unet((96, 96, 96), nbin=NBIN, seed=0, train=False) as tiled:
+        tiled.set_params(theta)
+        p1, b1, o1 = tiled.infer(_as_f(vol), halo=32, aux=_as_f(aux), offsets=True)
+    m = 30  # (x, y, z) maps; the interior at least m voxels from every face
+    core = (slice(m, 96 - m), slice(m, 80 - m), slice(m, 96 - m))
+    np.testing.assert_array_equal(b1[core], b0[core])
+    np.testing.assert_array_equal(p1[core], p0[core])
+    np.testing.assert_array_equal(o1[(slice(None),) + core], o0[(slice(None),) + core])
+    # every voxel was written by exactly one tile core
+    assert np.all(np.isfinite(p1)) and np.all((b1 >= 0) & (b1 < NBIN))
