@@ -1103,6 +1103,7 @@ int tomhip_subtomo_score_batch(tomhip_ctx* ctx, int nbox, const int32_t* box_edg
       b.M = dM.get<float>() + toff[k];
       b.Mcc = masks_cc ? dMcc.get<float>() + toff[k] : nullptr;
       b.T = nullptr;
+      b.Tp = nullptr;
       b.FM = b.FB = nullptr;
     }
     // wedge tangents; rotation matrices

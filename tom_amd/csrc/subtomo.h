@@ -14,6 +14,7 @@ struct SubtomoBox {
   long n;            // #{M != 0}
   double SM, SM2;    // sum of M, sum of M^2
   const float* T;    // device: template minus its mean
+  const float2* Tp;  // device: its x-neighbour pairs (rot3d_voxel_pairs)
   const float* M;    // device: mask
   const float* Mcc;  // device: peak-search mask (unshifted) or null
   const c32* FM;     // device: half spectrum of M
@@ -21,8 +22,8 @@ struct SubtomoBox {
 };
 
 long subtomo_scratch_floats();
-// boxes_h[k].M / Mcc / B / n / SM / SM2 filled by the caller; T, FM, FB are
-// produced here. pbox_h: [P] box of each particle (host). rms_dev: [P][A][9]
+// boxes_h[k].M / Mcc / B / n / SM / SM2 filled by the caller; T, Tp, FM, FB
+// are produced here. pbox_h: [P] box of each particle (host). rms_dev: [P][A][9]
 // rotation matrices. tanr_h: [nbox] tangent of each box template's wedge
 // semi-angle (< 0: no template wedge); wref_P_dev: [P][A][12] float maps of
 // that wedge turned by each orientation (read only for boxes with a template

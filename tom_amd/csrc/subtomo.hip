@@ -299,7 +299,7 @@ __global__ __launch_bounds__(NTB) void k_b_load(Chunk C) {
   }
   const float cB = (float)((B + 1) / 2);
   for (int i = threadIdx.x; i < B * B; i += NTB) {
-    const float v = TMPL ? rot3d_voxel(C.bx.T, B, B, B, rmr, cB, cB, cB, i % B, i / B, z) : V[i];
+    const float v = TMPL ? rot3d_voxel_pairs(C.bx.Tp, B, B, B, rmr, cB, cB, cB, i % B, i / B, z) : V[i];
     const float m = M[i];
     buf[i] = mk(v * m, 0.f);
     if (m != 0.f) {
@@ -593,6 +593,12 @@ __global__ __launch_bounds__(NTS) void k_box_prep(SubtomoBox* boxes, float* scra
     box_prep_one<64>(bx, R, H, buf, tw, tw64);
 }
 
+// pairs[b] = (T[b], T[b + 1]), the second 0 in the last x column
+__global__ void k_pairs(const float* T, int B, long n, float2* pairs) {
+  for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += (long)gridDim.x * blockDim.x)
+    pairs[b] = make_float2(T[b], (b % B) == B - 1 ? 0.f : T[b + 1]);
+}
+
 // template -= mean(template) (over the whole box, corr.cpp:1283-1289)
 __global__ __launch_bounds__(1024) void k_center(const float* T, float* out, long n) {
   __shared__ double red[1024];
@@ -703,6 +709,10 @@ void subtomo_run(Twiddles& tws, hipStream_t s, std::vector<SubtomoBox>& boxes_h,
       hipLaunchKernelGGL(k_center, dim3(1), dim3(1024), 0, s, templates_dev + tmpl_off[k], Tc, n3);
       TOMHIP_CHECK(hipGetLastError());
       b.T = Tc;
+      float2* Tp = (float2*)alloc(sizeof(float2) * n3);
+      hipLaunchKernelGGL(k_pairs, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, (const float*)Tc, b.B, n3, Tp);
+      TOMHIP_CHECK(hipGetLastError());
+      b.Tp = Tp;
       b.FM = (c32*)alloc(sizeof(c32) * nhmax);
       b.FB = (c32*)alloc(sizeof(c32) * nhmax);
     }
