@@ -8,9 +8,14 @@ void rotmat_ref(const float* euler, int neuler, float* rm);
 // dst[r] = rot_r(src) (.* mask if mask != nullptr), dst[r] at dst + r*dst_stride.
 // desc (optional, device): {batch, first}; the matrices then start at
 // rm_dev + batch*rm_step (graph replays advance batch on the device).
+// pairs (optional): make_x_pairs of src; the taps are then read as four
+// 8-byte pairs, the result unchanged.
 void launch_rotate(const float* src, float* dst, const float* rm_dev, int nrot, const float* mask, int sx,
                    int sy, int sz, float px, float py, float pz, long dst_stride, hipStream_t stream,
-                   const long* desc = nullptr, long rm_step = 0);
+                   const long* desc = nullptr, long rm_step = 0, const float2* pairs = nullptr);
+// pairs[b] = (src[b], src[b + 1]), the second 0 in the last x column (n voxels,
+// rows of sx)
+void make_x_pairs(const float* src, int sx, long n, float2* pairs, hipStream_t stream);
 // rot2d (Sptrans/tom_rotatec.c:49-142): phi in degrees (n == 1) or a 2x2
 // matrix given as rm00 rm10 rm01 rm11 (n == 4) -> rm00 rm10 rm01 rm11.
 void rotmat2d_ref(const float* phi_or_R, int n, float* rm);

@@ -128,6 +128,7 @@ class Search {
   float* maxc_ = nullptr; // [N] running max of the raw correlation
   int32_t* argm_ = nullptr;  // [N] best angle index
   float* tmpl_ = nullptr;    // [n3] prepared template (owned copy)
+  float2* tpairs_ = nullptr; // [n3] its x-neighbour pairs for the rotation
   float* wsh_ = nullptr;     // [n3] ifftshift(W) or null
   float* rm_ = nullptr;      // [cap][9] rotation matrices of the current run
   long rm_cap_ = 0;

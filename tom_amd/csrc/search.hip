@@ -1073,6 +1073,7 @@ Search::Search(Twiddles& tw, hipStream_t s, const float* volume, int X_, int Y_,
     TOMHIP_CHECK(hipMalloc(&maxc_, sizeof(float) * N));
     TOMHIP_CHECK(hipMalloc(&argm_, sizeof(int32_t) * N));
     TOMHIP_CHECK(hipMalloc(&tmpl_, sizeof(float) * n3));
+    TOMHIP_CHECK(hipMalloc(&tpairs_, sizeof(float2) * n3));
     TOMHIP_CHECK(hipMalloc(&t1_, sizeof(float) * n3 * KA_));
     TOMHIP_CHECK(hipMalloc(&tbox_, sizeof(c32) * n3 * KA_));
     TOMHIP_CHECK(hipMalloc(&Ty_, sizeof(c32) * (long)KA_ * nz * Y * Xh));
@@ -1118,6 +1119,8 @@ void Search::release() {
   (void)hipFree(maxc_);
   (void)hipFree(argm_);
   (void)hipFree(tmpl_);
+  (void)hipFree(tpairs_);
+  tpairs_ = nullptr;
   (void)hipFree(t1_);
   (void)hipFree(tbox_);
   (void)hipFree(Ty_);
@@ -1262,6 +1265,7 @@ void Search::precompute_stats(const float* volume) {
 void Search::set_template(const float* tmpl, const float* wsh, double mu_t_, double sig_t_, bool remask) {
   const long n3 = (long)nx * ny * nz;
   TOMHIP_CHECK(hipMemcpyAsync(tmpl_, tmpl, sizeof(float) * n3, hipMemcpyDefault, s_));
+  make_x_pairs(tmpl_, nx, n3, tpairs_, s_);
   remask_ = remask;
   if (wsh) {
     TOMHIP_REQUIRE(mode == MODE_FLCF && remask, ERR_WRONG_INPUT, "a weight needs FLCF mode with remask");
@@ -1379,7 +1383,7 @@ void Search::rotate_templates(const float* rm_dev, int count, float* tout, const
   // is masked again inside tom_os3_corr)
   const float* rmask = nullptr;
   if (mode == MODE_FLCF) rmask = (wsh_ || !remask_) ? mask_ : mask_ + n3;
-  launch_rotate(tmpl_, tout, rm_dev, count, rmask, nx, ny, nz, px, py, pz, n3, s_, desc, rm_step);
+  launch_rotate(tmpl_, tout, rm_dev, count, rmask, nx, ny, nz, px, py, pz, n3, s_, desc, rm_step, tpairs_);
 }
 
 void Search::box_filter(int count, float* t, c32* work) {
