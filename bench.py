@@ -161,7 +161,9 @@ def pruned_leg(args, cfg, api, torch, plan, V, score, ang, wpath, exhaustive_s, 
                precompute_s=round(pre_s, 4))
     log(f"[bench] pruned search: recall {out['recall_gt_0.3']:.4f} of {len(ref)} exhaustive peaks > 0.3, "
         f"{out['wall_s']:.3f} s vs exhaustive {exhaustive_s:.3f} s ({out['speedup']}x); {out['timings_s']}")
-    if wpath and os.path.exists(wpath):
+    # the shipped network was trained for the C2 template (train_surrogate's
+    # workload_template); other workloads search other templates
+    if wpath and os.path.exists(wpath) and args.config == "c2":
         w = np.load(wpath, allow_pickle=False)
         meta = json.loads(str(w["meta"]))
         if meta.get("aux") == "coarse":
