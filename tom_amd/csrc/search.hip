@@ -299,8 +299,8 @@ __global__ __launch_bounds__(NT) void k_tmpl_xdft(const float* t2, c32* Tx, cons
 // y: Stockham FFT of the ny template rows placed with the template centre at
 // the origin. grid (Xh/TW, nz, ka). Output Ty[a][uz][ky][kx] for kx < Xh,
 // Tyn[a][uz][ky] for kx = Xh (tile 0 only).
-template <int Y, int TW>
-__global__ __launch_bounds__(NT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* Tyn, const c32* twYg, int X, int ny,
+template <int Y, int TW, int NTT = NT>
+__global__ __launch_bounds__(NTT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* Tyn, const c32* twYg, int X, int ny,
                                                    int nz) {
   constexpr int P = TW + 1;
   __shared__ c32 buf[Y * P];
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* 
   const int Xh = X / 2, XP = tx_pitch(Xh);
   const int tile = blockIdx.x, uz = blockIdx.y, a = blockIdx.z;
   const bool nyq = (tile == 0);
-  load_table<float, Y, NT>(twY, twYg);
+  load_table<float, Y, NTT>(twY, twYg);
   __syncthreads();
   const c32* src = Tx + ((long)a * nz + uz) * ny * XP + tile * TW;
   const c32* srn = Tx + ((long)a * nz + uz) * ny * XP + Xh;
@@ -333,9 +333,9 @@ __global__ __launch_bounds__(NT) void k_tmpl_fwd_y(const c32* Tx, c32* Ty, c32* 
       dn[ky] = v;
   };
   if (nyq)
-    fft_g2g<float, Y, P, NT, -1, Inter<P>>(buf, twY, ld, st);
+    fft_g2g<float, Y, P, NTT, -1, Inter<P>>(buf, twY, ld, st);
   else
-    fft_g2g<float, Y, TW, NT, -1, Inter<P>>(buf, twY, ld, st);
+    fft_g2g<float, Y, TW, NTT, -1, Inter<P>>(buf, twY, ld, st);
 }
 
 // ---------------------------------------------------------------- pass A
@@ -1530,8 +1530,17 @@ void Search::run_batch(const float* rm_dev, int ka, long first_index, const long
     dispatch_len(Y, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
       // the widest tile (longest contiguous rows of Ty) whose block LDS
-      // still lets two blocks share a CU
-      if constexpr (L >= 1024) {
+      // still lets two blocks share a CU; at Y = 512 one block of 1,024
+      // threads per CU on 32-wide tiles measured fastest (0.43 -> 0.36 ms
+      // per group of 16 against 16-wide tiles of 256 threads)
+      if constexpr (L == 512) {
+        if (Xh % 32 == 0)
+          hipLaunchKernelGGL((k_tmpl_fwd_y<L, 32, 1024>), dim3(Xh / 32, nz, ka), dim3(1024), 0, s_, (const c32*)Tx,
+                             Ty_, Tyn_, tw_.f32(L), X, ny, nz);
+        else
+          hipLaunchKernelGGL((k_tmpl_fwd_y<L, 16, 512>), dim3(Xh / 16, nz, ka), dim3(512), 0, s_, (const c32*)Tx,
+                             Ty_, Tyn_, tw_.f32(L), X, ny, nz);
+      } else if constexpr (L >= 1024) {
         hipLaunchKernelGGL((k_tmpl_fwd_y<L, 8>), dim3(Xh / 8, nz, ka), dim3(NT), 0, s_, (const c32*)Tx, Ty_, Tyn_,
                            tw_.f32(L), X, ny, nz);
       } else if constexpr (L >= 512) {
