@@ -504,10 +504,13 @@ __device__ __forceinline__ void dft16(c32* v) {
     }
 }
 
-// NYQ: the launch for kx tile 0 (grid (1, Y, ka), one orientation per
-// block), whose blocks carry a 17th
+// NYQ: the launch for kx tile 0 (grid (1, Y, ka / kNyqApb), kNyqApb
+// orientations per block: its F_I slice is read once per block, and the
+// launch still has Y ka / kNyqApb blocks; 8 measured best of 1, 2, 4, 8),
+// whose blocks carry a 17th
 // line, the kx = X/2 column, packed into column 0 as P(0) + i P(X/2) before
 // the inverse; else grid (Xh/TW - 1, Y) for tiles 1.., 16 lines.
+constexpr int kNyqApb = 8;
 template <bool NYQ>
 __global__ __launch_bounds__(NYQ ? 17 * 16 : 256) void k_passA4(const c32* Ty, const c32* Tyn, const c32* FI,
                                                                 const c32* FIn, c32* Q, const c32* twg, int Y,
@@ -536,8 +539,8 @@ __global__ __launch_bounds__(NYQ ? 17 * 16 : 256) void k_passA4(const c32* Ty, c
   // this thread's template column: Ty[a][uz][ky][kx], or Tyn[a][uz][ky]
   const long su = nyq_line ? (long)Y : (long)Y * Xh;
   const c32* src = nyq_line ? Tyn + ky : Ty + (long)ky * Xh + tile * TW + line;
-  // tile 0 (few blocks) spreads its orientations over blocks: grid z
-  const int a0 = NYQ ? blockIdx.z : 0, a1 = NYQ ? blockIdx.z + 1 : ka;
+  // tile 0 (few blocks) spreads its orientations over blocks, kNyqApb each: grid z
+  const int a0 = NYQ ? blockIdx.z * kNyqApb : 0, a1 = NYQ ? min(a0 + kNyqApb, ka) : ka;
   src += a0 * nz * su;
   for (int a = a0; a < a1; ++a, src += nz * su) {
     c32 v[16];
@@ -1484,7 +1487,7 @@ void Search::launch_passA(int ka) {
     dispatch_len(Z, [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
       if constexpr (L == 256 && TW == 16) {
-        hipLaunchKernelGGL(k_passA4<true>, dim3(1, Y, ka), dim3(17 * 16), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
+        hipLaunchKernelGGL(k_passA4<true>, dim3(1, Y, (ka + kNyqApb - 1) / kNyqApb), dim3(17 * 16), 0, s_, (const c32*)Ty_, (const c32*)Tyn_,
                            (const c32*)FI_, (const c32*)FIn_, Q_, tw_.f32(L), Y, Xh, nz, ka);
         if (Xh > TW)
           hipLaunchKernelGGL(k_passA4<false>, dim3(Xh / TW - 1, Y), dim3(256), 0, s_, (const c32*)Ty_,
