@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch group measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (fetch scaled x2.000, write x1.000 by a 1 GiB device copy in the
 # same runs); profiles/flcf_c2_hbm_traffic.md has the commands and counts.
-MEASURED_TRAFFIC = {("c2", 16): {"passA": 5.6560e9, "passB": 8.5902e9, "passC": 4.9832e9}}
+MEASURED_TRAFFIC = {("c2", 16): {"passA": 5.4165e+09, "passB": 8.5902e+09, "passC": 4.9830e+09}}
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
