@@ -683,14 +683,12 @@ __device__ __forceinline__ void up_taps(int o, int n, int* i0, int* i1, float* w
   }
 }
 
-// X [Vlow][C] bf16 -> Y rows (stride ys, offset yo) at twice the size
-// One thread per output voxel and 8 channels: the taps and weights are
-// computed once for the 8 channels, which move as 16-byte vectors (C, ys and
-// yo multiples of 8).
-// Trilinear x2 upsampling through LDS: a block stages the low-resolution
-// tile of UX x UY x LZ voxels plus one voxel on each side (edge-clamped, which
-// is the clamping of up_taps) and writes the 2UX x 2UY x 2LZ outputs above
-// it; thread = 8 channels of one output voxel, channel groups fastest.
+// Trilinear x2 upsampling through LDS, X [Vlow][C] bf16 -> Y at twice the
+// size: a block stages the low-resolution tile of UX x UY x LZ voxels plus
+// one voxel on each side (edge-clamped, which is the clamping of up_taps)
+// and writes the 2UX x 2UY x 2LZ outputs above it; thread = 8 channels of
+// one low-resolution voxel, channel groups fastest, channels moving as
+// 16-byte vectors.
 // Output chunk-planar: channel c of voxel v at (c / oc) oq + v oc + c % oc.
 constexpr int UX = 8, UY = 8;
 template <int C, int LZ>
@@ -716,41 +714,73 @@ __global__ __launch_bounds__(NTU) void k_up(const uint16_t* __restrict__ X, Dims
     if (i < NEL) t[i] = sv[k];
   }
   __syncthreads();
-  const int HW = 2 * lo.W, HH = 2 * lo.H, HD = 2 * lo.D;
-  for (int i = threadIdx.x; i < 8 * UX * UY * LZ * C8; i += NTU) {
+  const int HW = 2 * lo.W, HH = 2 * lo.H;
+  // thread = 8 channels of one low-resolution voxel: its 3 x 3 x 3 staged
+  // neighbourhood gives the 2 x 2 x 2 outputs above it, separably (x, then
+  // y, then z; the half-pixel weights are 1/4 and 3/4 on every axis and the
+  // staged halo carries the edge clamping)
+  for (int i = threadIdx.x; i < UX * UY * LZ * C8; i += NTU) {
     const int c8 = i % C8, v = i / C8;
-    const int ox = v % (2 * UX), oy = (v / (2 * UX)) % (2 * UY), oz = v / (4 * UX * UY);
-    const int x = 2 * X0 + ox, y = 2 * Y0 + oy, z = 2 * Z0 + oz;
-    if (x >= HW || y >= HH || z >= HD) continue;
-    int xi[2], yi[2], zi[2];
-    float xw[2], yw[2], zw[2];
-    up_taps(x, lo.W, &xi[0], &xi[1], &xw[0], &xw[1]);
-    up_taps(y, lo.H, &yi[0], &yi[1], &yw[0], &yw[1]);
-    up_taps(z, lo.D, &zi[0], &zi[1], &zw[0], &zw[1]);
-    float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int lx = v % UX, ly = (v / UX) % UY, lz = v / (UX * UY);
+    if (X0 + lx >= lo.W || Y0 + ly >= lo.H || Z0 + lz >= lo.D) continue;
+    float acc[2][2][2][8];  // [oz][oy][ox][channel]
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int dz = 0; dz < 3; ++dz) {
+      float row[3][2][8];  // [dy][ox][channel]: the x pass of three rows
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int dy = 0; dy < 3; ++dy) {
+        float a[3][8];
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const float w = zw[a] * yw[b] * xw[e];
-          const int lv = ((zi[a] - Z0 + 1) * SY + (yi[b] - Y0 + 1)) * SX + (xi[e] - X0 + 1);
-          const uint4 q = t[lv * C8 + c8];
+        for (int dx = 0; dx < 3; ++dx) {
+          const uint4 q = t[(((lz + dz) * SY + ly + dy) * SX + lx + dx) * C8 + c8];
           const uint32_t u[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            sum[2 * k] += w * bf2f((uint16_t)(u[k] & 0xffffu));
-            sum[2 * k + 1] += w * bf2f((uint16_t)(u[k] >> 16));
+            a[dx][2 * k] = bf2f((uint16_t)(u[k] & 0xffffu));
+            a[dx][2 * k + 1] = bf2f((uint16_t)(u[k] >> 16));
           }
         }
-    uint4 o;
-    o.x = (uint32_t)f2bf(sum[0]) | ((uint32_t)f2bf(sum[1]) << 16);
-    o.y = (uint32_t)f2bf(sum[2]) | ((uint32_t)f2bf(sum[3]) << 16);
-    o.z = (uint32_t)f2bf(sum[4]) | ((uint32_t)f2bf(sum[5]) << 16);
-    o.w = (uint32_t)f2bf(sum[6]) | ((uint32_t)f2bf(sum[7]) << 16);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          row[dy][0][k] = 0.25f * a[0][k] + 0.75f * a[1][k];
+          row[dy][1][k] = 0.75f * a[1][k] + 0.25f * a[2][k];
+        }
+      }
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const float p = oy ? 0.75f * row[1][ox][k] + 0.25f * row[2][ox][k]
+                               : 0.25f * row[0][ox][k] + 0.75f * row[1][ox][k];
+            if (dz == 0) {
+              acc[0][oy][ox][k] = 0.25f * p;
+            } else if (dz == 1) {
+              acc[0][oy][ox][k] += 0.75f * p;
+              acc[1][oy][ox][k] = 0.75f * p;
+            } else {
+              acc[1][oy][ox][k] += 0.25f * p;
+            }
+          }
+    }
     const int c = c8 * 8;
-    *reinterpret_cast<uint4*>(Y + (c / oc) * oq + (((long)z * HH + y) * HW + x) * oc + c % oc) = o;
+    uint16_t* const dst = Y + (c / oc) * oq + c % oc;
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox) {
+          const float* const f = acc[oz][oy][ox];
+          uint4 o;
+          o.x = (uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16);
+          o.y = (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
+          o.z = (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16);
+          o.w = (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16);
+          const int x = 2 * (X0 + lx) + ox, y = 2 * (Y0 + ly) + oy, z = 2 * (Z0 + lz) + oz;
+          *reinterpret_cast<uint4*>(dst + (((long)z * HH + y) * HW + x) * oc) = o;
+        }
   }
 }
 
