@@ -973,7 +973,7 @@ __global__ __launch_bounds__(NTU) void k_predict(const float* logits, int CH, in
 // bin logits and the offsets, written straight into the output volume: the
 // tile voxel (a, b, c) of the network-size tile at (ox, oy, oz) goes to
 // (ox + a, oy + b, oz + c) when a, b, c lie in [halo, halo + core) and inside
-// X x Y x Z. One wave takes 16 consecutive tile voxels per step.
+// X x Y x Z. One wave takes 64 consecutive tile voxels per step, 16 per MFMA.
 struct OutWin {
   int halo, cx, cy, cz, ox, oy, oz, X, Y, Z;
 };
@@ -1011,9 +1011,10 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
     uint2 rxs[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) rxs[u] = *reinterpret_cast<const uint2*>(Xb + ((long)(g0 + u) * 16 + n) * 16 + 4 * kg);
+    float rp = 0.f, ro[3] = {0.f, 0.f, 0.f};
+    int rb = 0;
 #pragma unroll 1
   for (int u = 0; u < 4; ++u) {
-    const int v = (g0 + u) * 16 + n;
     const uint2 rx = rxs[u];
     bf16x4 xb;
     xb[0] = (short)(rx.x & 0xffff);
@@ -1064,22 +1065,34 @@ __global__ __launch_bounds__(NTU) void k_head_predict(const uint16_t* __restrict
         ov[q] = val;
       }
     }
+    // the results of voxel (g0 + u) * 16 + n move to lane 16 u + n, so that
+    // after the four groups each lane holds one of 64 consecutive voxels and
+    // the stores cover 64 voxels at once
+    const int sn = lane & 15;
+    const float pv = __shfl(acc[0][0] + bb[0][0], sn, 64);  // channel 0: lane sn (kg 0)
+    const int bv = __shfl(bi, sn, 64), fv = __shfl(first_nan, sn, 64);
+    float ovs[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ovs[q] = __shfl(ov[q], 16 * (((nbin + 1 + q) >> 2) & 3) + sn, 64);
+    if ((lane >> 4) == u) {
+      rp = pv;
+      rb = (fv || bv == INT_MAX) ? 0 : bv;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) ro[q] = ovs[q];
+    }
+  }
+    const int v = g0 * 16 + lane;
     const int c = v / tHW, rr = v - c * tHW, b = rr / t.W, a = rr - b * t.W;
     const int x = ow.ox + a, y = ow.oy + b, z = ow.oz + c;
     const bool inside = a >= ow.halo && a < ow.halo + ow.cx && b >= ow.halo && b < ow.halo + ow.cy &&
                         c >= ow.halo && c < ow.halo + ow.cz && x < ow.X && y < ow.Y && z < ow.Z;
     if (!inside) continue;
     const long dst = ((long)z * ow.Y + y) * ow.X + x;
-    if (kg == 0) {
-      const float l0 = acc[0][0] + bb[0][0];
-      if (prob) prob[dst] = raw ? l0 : 1.f / (1.f + expf(-l0));
-      if (bin) bin[dst] = (first_nan || bi == INT_MAX) ? 0 : bi;
-    }
+    if (prob) prob[dst] = raw ? rp : 1.f / (1.f + expf(-rp));
+    if (bin) bin[dst] = rb;
     if (offs)
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if (kg == (((nbin + 1 + q) >> 2) & 3)) offs[dst * 3 + q] = ov[q];
-  }
+      for (int q = 0; q < 3; ++q) offs[dst * 3 + q] = ro[q];
   }
 }
 
