@@ -1515,7 +1515,7 @@ void Unet::forward_window(const float* vol, const float* aux, int X, int Y, int 
   };
   {
     ConvLayer& c = L[0];
-    constexpr int TZ = 8;
+    constexpr int TZ = 16;  // planes per wave (at 512x512x256: 8 planes 1.06 ms, 16 planes 0.89 ms)
     const int nbx = (W + CTX - 1) / CTX, nby = (H + CTY - 1) / CTY, nbz = (D + TZ - 1) / TZ;
     const InWin in{vol, aux, X, Y, Z, ox, oy, oz, sums, n};
     hipLaunchKernelGGL(k_conv_in<TZ>, dim3((unsigned)(nbx * nby * nbz)), dim3(NTU), 0, s, in, Dims{D, H, W},
